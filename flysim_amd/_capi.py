@@ -1,0 +1,154 @@
+"""ctypes binding of include/flysim.h (the engine and the drop-in host side).
+
+The library is the in-tree ``flysim_amd/_lib/libflysim.so`` built by
+``make -C flysim_amd``.  Loading fails loudly when it is missing: there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libflysim.so")
+
+FS_OK = 0
+FS_AMPA, FS_GABA, FS_NMDA, FS_ACH, FS_GCL = 0, 1, 2, 3, 4
+FS_ACT_AMPA, FS_ACT_GABA, FS_ACT_NMDA, FS_ACT_GAP = 1, 2, 4, 8
+FS_ACT_ACH, FS_ACT_GCL, FS_ACT_INJ, FS_ACT_LIF = 32, 64, 128, 256
+FS_FP32, FS_FP64 = 0, 1
+FS_EXT_PHILOX, FS_EXT_REPLAY = 0, 1
+DEFAULT_ACT = (FS_ACT_AMPA | FS_ACT_GABA | FS_ACT_NMDA | FS_ACT_ACH | FS_ACT_GCL
+               | FS_ACT_GAP | FS_ACT_LIF)
+
+
+class PopParams(C.Structure):
+    _fields_ = [("vth", C.c_float), ("el", C.c_float), ("gl", C.c_float), ("cm", C.c_float),
+                ("vreset", C.c_float), ("df", C.c_float), ("ef", C.c_float),
+                ("dvmax", C.c_float), ("rfped", C.c_int32), ("spidy", C.c_int32),
+                ("vrev", C.c_float * 5), ("dfst", C.c_float * 5), ("dfxt_n", C.c_float),
+                ("efst_n", C.c_float), ("as_n", C.c_float), ("mg_n", C.c_float),
+                ("pb", C.c_float * 5), ("gext", C.c_float * 5), ("act", C.c_uint32),
+                ("inj_mean", C.c_float), ("inj_std", C.c_float), ("taum", C.c_float)]
+
+
+class PopConf(C.Structure):
+    _fields_ = [("cm", C.c_float), ("taum", C.c_float), ("vth", C.c_float), ("el", C.c_float),
+                ("vreset", C.c_float), ("refractory_ms", C.c_float),
+                ("spike_delay_ms", C.c_float), ("declared", C.c_int32 * 5),
+                ("tau", C.c_float * 5), ("tau_xt", C.c_float * 5), ("vrev", C.c_float * 5),
+                ("freq_ext", C.c_float * 5), ("ext_eff", C.c_float * 5),
+                ("ext_con", C.c_float * 5)]
+
+
+class NetDesc(C.Structure):
+    _fields_ = [("n_neuron", C.c_int32), ("n_pop", C.c_int32),
+                ("neuron_pop", C.POINTER(C.c_uint32)), ("pops", C.POINTER(PopParams)),
+                ("n_syn", C.c_int64), ("pre", C.POINTER(C.c_uint32)),
+                ("post", C.POINTER(C.c_uint32)), ("w", C.POINTER(C.c_float)),
+                ("receptor", C.POINTER(C.c_uint8)), ("delay", C.POINTER(C.c_uint8)),
+                ("dt", C.c_float), ("precision", C.c_int32), ("ext_mode", C.c_int32),
+                ("record_spikes", C.c_int32)]
+
+
+class Schedule(C.Structure):
+    _fields_ = [("update_block", C.c_int32), ("gather_lanes", C.c_int32),
+                ("push_block", C.c_int32), ("steps_per_graph", C.c_int32),
+                ("use_graph", C.c_int32)]
+
+
+class CliOpts(C.Structure):
+    _fields_ = [("dt", C.c_float), ("threads", C.c_int32), ("repeats", C.c_int32),
+                ("stp", C.c_int32), ("std_", C.c_int32), ("ltp", C.c_int32),
+                ("udf_seed", C.c_int32), ("udf_seed_value", C.c_uint32),
+                ("out_mem", C.c_char_p), ("out_spi", C.c_char_p), ("out_fr", C.c_char_p)]
+
+
+class RefLayout(C.Structure):
+    _fields_ = [("n_gate", C.c_int64), ("n_row", C.c_int64),
+                ("gate_pre", C.POINTER(C.c_uint32)), ("gate_post0", C.POINTER(C.c_uint32)),
+                ("gate_rec", C.POINTER(C.c_uint8)), ("gate_group", C.POINTER(C.c_uint8)),
+                ("gate_g", C.POINTER(C.c_float)), ("row_post", C.POINTER(C.c_uint32)),
+                ("row_beg", C.POINTER(C.c_uint32)), ("term", C.POINTER(C.c_uint32)),
+                ("row_rec", C.POINTER(C.c_uint8)), ("pop_conf", C.POINTER(PopConf)),
+                ("pop_rfped", C.POINTER(C.c_int32)), ("pop_spidy", C.POINTER(C.c_int32)),
+                ("neuron_pop", C.POINTER(C.c_uint32)), ("n_evt", C.c_int64),
+                ("evt_time_s", C.POINTER(C.c_float)), ("evt_var1", C.POINTER(C.c_float)),
+                ("evt_var2", C.POINTER(C.c_float)), ("evt_type", C.POINTER(C.c_int32)),
+                ("evt_pop", C.POINTER(C.c_int32)), ("evt_sub", C.POINTER(C.c_int32)),
+                ("act_ctr", C.c_uint32)]
+
+
+EXPORTS = [
+    "fs_derive_params", "fs_create", "fs_reset", "fs_set_ext_rate", "fs_set_injection",
+    "fs_set_pop_params", "fs_load_ext_trains", "fs_run", "fs_step_index", "fs_get_state",
+    "fs_pop_spike_counts", "fs_get_schedule", "fs_set_schedule", "fs_enable_timing",
+    "fs_kernel_times", "fs_run_counters", "fs_destroy", "fs_last_error", "fs_conf_load",
+    "fs_conf_free", "fs_conf_desc", "fs_conf_sizes", "fs_conf_seed", "fs_conf_ext_trains",
+    "fs_conf_ref_layout", "fs_cli_main",
+]
+
+_lib = None
+
+
+class FlysimError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("flysim error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    """The loaded C library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("flysim_amd native library missing: %s (run make -C flysim_amd)"
+                              % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.fs_last_error.restype = C.c_char_p
+        sig = {
+            "fs_derive_params": [P(PopConf), C.c_float, C.c_uint32, P(PopParams)],
+            "fs_create": [P(NetDesc), P(C.c_void_p)],
+            "fs_reset": [C.c_void_p, C.c_uint64],
+            "fs_set_ext_rate": [C.c_void_p, C.c_int32, C.c_int32, C.c_float],
+            "fs_set_injection": [C.c_void_p, C.c_int32, C.c_float, C.c_float],
+            "fs_set_pop_params": [C.c_void_p, C.c_int32, P(PopParams)],
+            "fs_load_ext_trains": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64],
+            "fs_run": [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64],
+            "fs_step_index": [C.c_void_p, P(C.c_int64)],
+            "fs_get_state": [C.c_void_p, C.c_int32, C.c_void_p],
+            "fs_pop_spike_counts": [C.c_void_p, C.c_void_p],
+            "fs_get_schedule": [C.c_void_p, P(Schedule)],
+            "fs_set_schedule": [C.c_void_p, P(Schedule)],
+            "fs_enable_timing": [C.c_void_p, C.c_int32],
+            "fs_kernel_times": [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double),
+                                P(C.c_int64)],
+            "fs_run_counters": [C.c_void_p, P(C.c_int64), P(C.c_int64), P(C.c_int64)],
+            "fs_destroy": [C.c_void_p],
+            "fs_conf_load": [C.c_char_p, C.c_char_p, P(CliOpts), P(C.c_void_p)],
+            "fs_conf_free": [C.c_void_p],
+            "fs_conf_desc": [C.c_void_p, P(NetDesc)],
+            "fs_conf_sizes": [C.c_void_p, C.c_void_p],
+            "fs_conf_seed": [C.c_void_p, C.c_int32, P(C.c_uint32)],
+            "fs_conf_ext_trains": [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p],
+            "fs_conf_ref_layout": [C.c_void_p, P(RefLayout)],
+            "fs_cli_main": [C.c_int, P(C.c_char_p)],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != FS_OK:
+        raise FlysimError(rc, lib().fs_last_error().decode(errors="replace"))
+
+
+def ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))

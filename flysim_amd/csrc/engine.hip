@@ -1,0 +1,711 @@
+// Host side of the Flysim engine: network upload, CSR construction, the
+// batched step loop (hipGraph-captured), spike readback and the C ABI of
+// include/flysim.h for the engine entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/flysim.h"
+#include "engine_kernels.hpp"
+#include "host/host_common.hpp"
+
+namespace {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(x)                                                                    \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess)                                                              \
+      throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) b = 256;
+    HIPCHECK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <class X> X *as() const { return static_cast<X *>(p); }
+};
+
+struct EngineBase {
+  virtual ~EngineBase() {}
+  virtual void reset(uint64_t seed) = 0;
+  virtual void set_pop(int32_t pop, const fs_pop_params &p) = 0;
+  virtual const fs_pop_params &pop(int32_t i) const = 0;
+  virtual void load_ext(const uint32_t *bits, int64_t first, int64_t n) = 0;
+  virtual void run(int64_t n, int64_t *off, uint32_t *ids, int64_t cap) = 0;
+  virtual void get_state(int32_t f, double *out) = 0;
+  virtual void pop_counts(uint64_t *out) = 0;
+  virtual void times(double *u, double *p, double *g, int64_t *n) = 0;
+  virtual void counters(int64_t *s, int64_t *e, int64_t *t) = 0;
+  fs_schedule sched{};
+  bool timing = false;
+  int64_t step = 0;
+  float dt = 0.1f;
+  int32_t n_pop = 0;
+};
+
+template <typename T>
+struct Engine final : EngineBase {
+  fsk::Net<T> net{};
+  int32_t N = 0, P = 0;
+  int32_t precision = FS_FP32;
+  bool record = true;
+  std::vector<fs_pop_params> hpop;
+  std::vector<fsk::DevPop<T>> dpop;
+  // device buffers
+  DevBuf b_npop, b_pops, b_V, b_cs, b_G, b_ring, b_spre, b_xn, b_shist, b_gn, b_elin, b_enx,
+      b_ens, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
+      b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int32_t graph_k = -1;
+  bool graph_dirty = true;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  double t_upd = 0, t_del = 0;
+  int64_t n_timed = 0;
+  int64_t syn_lin = 0, syn_nmda = 0;
+  std::vector<uint32_t> cum_h;
+  std::vector<uint32_t> ids_h;
+  unsigned long long ctr_last[4] = {0, 0, 0, 0};
+
+  ~Engine() override {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  static fsk::DevPop<T> to_dev(const fs_pop_params &p, const fsk::DevPop<T> &pre) {
+    fsk::DevPop<T> d = pre;  // keeps the presynaptic fields
+    d.vth = T(p.vth);
+    // gl*El*1e3f as the reference evaluates it each step (float, left to right)
+    if (sizeof(T) == 4) d.leak_drive = T(p.gl * p.el * 1e3f);
+    else d.leak_drive = T(p.gl) * T(p.el) * T(1e3f);
+    d.cm = T(p.cm);
+    d.vreset = T(p.vreset);
+    d.df = T(p.df);
+    d.ef = T(p.ef);
+    d.dvmax = T(p.dvmax);
+    d.mg = T(p.mg_n);
+    for (int r = 0; r < 5; ++r) {
+      d.vrev[r] = T(p.vrev[r]);
+      d.dfst[r] = T(p.dfst[r]);
+      d.gext[r] = T(p.gext[r]);
+      d.pb[r] = p.pb[r];
+    }
+    d.dfxt_n = T(p.dfxt_n);
+    if (sizeof(T) == 4) d.efas_n = T(p.efst_n * p.as_n);
+    else d.efas_n = T(p.efst_n) * T(p.as_n);
+    d.inj_mean = T(p.inj_mean);
+    d.inj_std = T(p.inj_std);
+    d.rfped = p.rfped;
+    d.spidy = p.spidy;
+    d.act = p.act;
+    return d;
+  }
+
+  void upload_pops() {
+    HIPCHECK(hipMemcpyAsync(b_pops.p, dpop.data(), sizeof(fsk::DevPop<T>) * P,
+                            hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
+  void update_ext_mask() {
+    int32_t m = 0;
+    for (const fs_pop_params &p : hpop)
+      for (int r = 0; r < 5; ++r)
+        if (p.pb[r] > 0.0f) m |= 1 << r;
+    if ((m | net.ext_mask) != net.ext_mask) {
+      net.ext_mask |= m;
+      graph_dirty = true;
+    }
+  }
+
+  explicit Engine(const fs_net_desc &d) {
+    N = d.n_neuron;
+    P = d.n_pop;
+    n_pop = P;
+    dt = d.dt;
+    precision = d.precision;
+    record = d.record_spikes != 0;
+    if (N <= 0 || P <= 0) throw std::invalid_argument("network needs neurons and populations");
+    if (N >= (1 << 24)) throw std::invalid_argument("at most 2^24-1 neurons per device");
+    HIPCHECK(hipStreamCreate(&stream));
+    for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+    hpop.assign(d.pops, d.pops + P);
+    for (int32_t i = 0; i < N; ++i)
+      if (d.neuron_pop[i] >= (uint32_t)P) throw std::invalid_argument("neuron_pop out of range");
+    // ---------------- synapses: classify, validate, CSR by pre / by post
+    std::vector<uint64_t> cnt_p(N + 1, 0), cnt_n(N + 1, 0);
+    int32_t maxd_p = 0, maxd_n = 0;
+    int32_t lin_mask = 0;
+    // presynaptic gate parameters per population (targets' receptor params)
+    std::vector<int> pre_set(P, 0), pre_set_n(P, 0);
+    std::vector<float> pre_df(P, 0.f), pre_dfxt(P, 0.f), pre_dfst(P, 0.f), pre_ef(P, 0.f),
+        pre_as(P, 0.f);
+    for (int64_t s = 0; s < d.n_syn; ++s) {
+      const uint32_t a = d.pre[s], b = d.post[s];
+      const uint8_t r = d.receptor[s];
+      const int32_t dl = d.delay ? d.delay[s] : 0;
+      if (a >= (uint32_t)N || b >= (uint32_t)N) throw std::invalid_argument("synapse endpoint out of range");
+      if (r > FS_GCL) throw std::invalid_argument("synapse receptor id out of range");
+      const uint32_t pp = d.neuron_pop[a];
+      const fs_pop_params &q = hpop[d.neuron_pop[b]];
+      if (r == FS_NMDA) {
+        if (dl > 255) throw std::invalid_argument("NMDA delay above 255 steps");
+        maxd_n = std::max(maxd_n, dl);
+        cnt_n[b + 1]++;
+        if (!pre_set_n[pp]) {
+          pre_set_n[pp] = 1;
+          pre_dfxt[pp] = q.dfxt_n; pre_dfst[pp] = q.dfst[FS_NMDA];
+          pre_ef[pp] = q.efst_n; pre_as[pp] = q.as_n;
+        } else if (pre_dfxt[pp] != q.dfxt_n || pre_dfst[pp] != q.dfst[FS_NMDA] ||
+                   pre_ef[pp] != q.efst_n || pre_as[pp] != q.as_n) {
+          throw std::invalid_argument(
+              "a population projects NMDA synapses onto populations with different NMDA time "
+              "constants; one presynaptic NMDA gate per population is supported");
+        }
+      } else {
+        if (dl > 63) throw std::invalid_argument("linear-receptor delay above 63 steps");
+        maxd_p = std::max(maxd_p, dl);
+        cnt_p[a + 1]++;
+        const int li = r < 2 ? r : r - 1;
+        lin_mask |= 1 << li;
+        if (!pre_set[pp]) { pre_set[pp] = 1; pre_df[pp] = q.dfst[r]; }
+        else if (pre_df[pp] != q.dfst[r])
+          throw std::invalid_argument(
+              "a population projects linear synapses whose postsynaptic decay constants differ; "
+              "one presynaptic linear gate per population is supported");
+      }
+    }
+    for (int32_t i = 0; i < N; ++i) { cnt_p[i + 1] += cnt_p[i]; cnt_n[i + 1] += cnt_n[i]; }
+    syn_lin = (int64_t)cnt_p[N];
+    syn_nmda = (int64_t)cnt_n[N];
+    std::vector<uint32_t> meta_p(syn_lin), meta_n(syn_nmda);
+    std::vector<float> w_p(syn_lin), w_n(syn_nmda);
+    {
+      std::vector<uint64_t> fp(cnt_p.begin(), cnt_p.end() - 1), fn(cnt_n.begin(), cnt_n.end() - 1);
+      for (int64_t s = 0; s < d.n_syn; ++s) {
+        const uint32_t a = d.pre[s], b = d.post[s];
+        const uint8_t r = d.receptor[s];
+        const uint32_t dl = d.delay ? d.delay[s] : 0u;
+        if (r == FS_NMDA) {
+          const uint64_t e = fn[b]++;
+          meta_n[e] = a | (dl << 24);
+          w_n[e] = d.w[s];
+        } else {
+          const uint32_t li = r < 2 ? r : r - 1u;
+          const uint64_t e = fp[a]++;
+          meta_p[e] = b | (dl << 24) | (li << 30);
+          w_p[e] = d.w[s];
+        }
+      }
+    }
+    // ---------------- populations on device
+    dpop.resize(P);
+    for (int32_t i = 0; i < P; ++i) {
+      fsk::DevPop<T> base{};
+      base.has_lin_out = pre_set[i];
+      base.has_nmda_out = pre_set_n[i];
+      base.pre_df = T(pre_df[i]);
+      base.pre_dfxt = T(pre_dfxt[i]);
+      base.pre_dfst = T(pre_dfst[i]);
+      if (sizeof(T) == 4) base.pre_efas = T(pre_ef[i] * pre_as[i]);
+      else base.pre_efas = T(pre_ef[i]) * T(pre_as[i]);
+      dpop[i] = to_dev(hpop[i], base);
+    }
+    // ---------------- allocations
+    const size_t n = (size_t)N;
+    net.N = N;
+    net.D = maxd_p + 1;
+    net.DN = maxd_n + 1;
+    net.lin_mask = lin_mask;
+    net.has_nmda_syn = syn_nmda > 0;
+    net.ext_mode = d.ext_mode;
+    net.words = (N + 31) / 32;
+    b_npop.alloc(n * 4);
+    HIPCHECK(hipMemcpy(b_npop.p, d.neuron_pop, n * 4, hipMemcpyHostToDevice));
+    b_pops.alloc(sizeof(fsk::DevPop<T>) * P);
+    upload_pops();
+    update_ext_mask();
+    b_V.alloc(n * sizeof(T));
+    b_cs.alloc(n * 4);
+    const int nlin = 4;
+    b_G.alloc((size_t)nlin * n * sizeof(T));
+    b_ring.alloc((size_t)nlin * net.D * n * sizeof(T));
+    b_spre.alloc(n * sizeof(T));
+    b_xn.alloc(n * sizeof(T));
+    b_shist.alloc((size_t)net.DN * n * sizeof(T));
+    b_gn.alloc(n * sizeof(T));
+    b_elin.alloc((size_t)nlin * n * sizeof(T));
+    b_enx.alloc(n * sizeof(T));
+    b_ens.alloc(n * sizeof(T));
+    b_rp_p.alloc((n + 1) * 8);
+    HIPCHECK(hipMemcpy(b_rp_p.p, cnt_p.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+    b_meta_p.alloc(syn_lin * 4);
+    b_w_p.alloc(syn_lin * 4);
+    if (syn_lin) {
+      HIPCHECK(hipMemcpy(b_meta_p.p, meta_p.data(), syn_lin * 4, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(b_w_p.p, w_p.data(), syn_lin * 4, hipMemcpyHostToDevice));
+    }
+    b_rp_n.alloc((n + 1) * 8);
+    HIPCHECK(hipMemcpy(b_rp_n.p, cnt_n.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+    b_meta_n.alloc(syn_nmda * 4);
+    b_w_n.alloc(syn_nmda * 4);
+    if (syn_nmda) {
+      HIPCHECK(hipMemcpy(b_meta_n.p, meta_n.data(), syn_nmda * 4, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(b_w_n.p, w_n.data(), syn_nmda * 4, hipMemcpyHostToDevice));
+    }
+    b_total.alloc(4);
+    b_overflow.alloc(4);
+    b_t0.alloc(8);
+    b_popcnt.alloc((size_t)P * 8);
+    b_counters.alloc(4 * 8);
+    b_tmp.alloc(n * sizeof(T));
+    net.npop = b_npop.as<uint32_t>();
+    net.pops = b_pops.as<fsk::DevPop<T>>();
+    net.V = b_V.as<T>();
+    net.cs = b_cs.as<uint32_t>();
+    net.G = b_G.as<T>();
+    net.ring = b_ring.as<T>();
+    net.spre = b_spre.as<T>();
+    net.xn = b_xn.as<T>();
+    net.shist = b_shist.as<T>();
+    net.gn = b_gn.as<T>();
+    net.elin = b_elin.as<T>();
+    net.enx = b_enx.as<T>();
+    net.ens = b_ens.as<T>();
+    net.rp_p = b_rp_p.as<uint64_t>();
+    net.meta_p = b_meta_p.as<uint32_t>();
+    net.w_p = b_w_p.as<float>();
+    net.rp_n = b_rp_n.as<uint64_t>();
+    net.meta_n = b_meta_n.as<uint32_t>();
+    net.w_n = b_w_n.as<float>();
+    net.total = b_total.as<uint32_t>();
+    net.overflow = b_overflow.as<uint32_t>();
+    net.t0 = b_t0.as<int64_t>();
+    net.pop_cnt = b_popcnt.as<unsigned long long>();
+    net.counters = b_counters.as<unsigned long long>();
+    net.ext_bits = nullptr;
+    net.ext_first = 0;
+    net.ext_steps = 0;
+    // default schedule (the autotuner may replace it)
+    sched.update_block = 256;
+    sched.gather_lanes = 8;
+    sched.push_block = 256;
+    sched.steps_per_graph = 100;
+    sched.use_graph = 1;
+    apply_schedule();
+    reset(0);
+  }
+
+  void apply_schedule() {
+    const int ub = sched.update_block;
+    if (ub < 64 || ub > 1024 || (ub & 63)) throw std::invalid_argument("update_block must be 64..1024, multiple of 64");
+    const int L = sched.gather_lanes;
+    if (L < 1 || L > 64 || (L & (L - 1))) throw std::invalid_argument("gather_lanes must be a power of two <= 64");
+    if (sched.push_block < 64 || sched.push_block > 1024 || (sched.push_block & 63))
+      throw std::invalid_argument("push_block must be 64..1024, multiple of 64");
+    if (sched.steps_per_graph < 1 || sched.steps_per_graph > 100000)
+      throw std::invalid_argument("steps_per_graph out of range");
+    net.gather_lanes = L;
+    const int K = sched.steps_per_graph;
+    // record buffer: every neuron could hold SPIKE_F every step (sticky case)
+    uint64_t cap = (uint64_t)N * (uint64_t)K;
+    const uint64_t cap_max = (uint64_t)1 << 31;
+    if (cap > cap_max) cap = cap_max;
+    if (b_rec_ids.bytes < cap * 4) {
+      b_rec_ids.alloc(cap * 4);
+      b_rec_a.alloc(cap * sizeof(T));
+    }
+    net.rec_cap = cap;
+    net.rec_ids = b_rec_ids.as<uint32_t>();
+    net.rec_a = b_rec_a.as<T>();
+    if (b_cum.bytes < (size_t)(K + 1) * 4) b_cum.alloc((size_t)(K + 1) * 4);
+    net.cum = b_cum.as<uint32_t>();
+    graph_dirty = true;
+  }
+
+  int update_grid() const { return (N + sched.update_block - 1) / sched.update_block; }
+  int deliver_grid(int *push_waves) const {
+    const int wpb = sched.push_block / 64;
+    // push waves: enough for the typical spike count of a step (spikes are
+    // strided over them); gather waves: enough rows for all CUs
+    int pw = std::max(64, std::min(4096, N / 64));
+    int gw = 0;
+    if (net.has_nmda_syn) {
+      const int groups_per_wave = 64 / sched.gather_lanes;
+      gw = std::max(wpb, std::min(16384, (N + groups_per_wave - 1) / groups_per_wave));
+    }
+    pw = (pw + wpb - 1) / wpb * wpb;
+    gw = (gw + wpb - 1) / wpb * wpb;
+    *push_waves = pw;
+    return (pw + gw) / wpb;
+  }
+
+  void launch_step(int32_t k, bool timed) {
+    int pw;
+    const int dgrid = deliver_grid(&pw);
+    net.push_waves = pw;
+    if (timed) HIPCHECK(hipEventRecord(ev[0], stream));
+    if (net.ext_mode == FS_EXT_REPLAY)
+      hipLaunchKernelGGL((fsk::k_update<T, true>), dim3(update_grid()), dim3(sched.update_block), 0, stream, net, k);
+    else
+      hipLaunchKernelGGL((fsk::k_update<T, false>), dim3(update_grid()), dim3(sched.update_block), 0, stream, net, k);
+    HIPCHECK(hipGetLastError());
+    if (timed) HIPCHECK(hipEventRecord(ev[1], stream));
+    hipLaunchKernelGGL((fsk::k_deliver<T>), dim3(dgrid), dim3(sched.push_block), 0, stream, net, k);
+    HIPCHECK(hipGetLastError());
+    if (timed) HIPCHECK(hipEventRecord(ev[2], stream));
+  }
+
+  void build_graph(int32_t K) {
+    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int32_t k = 1; k < K; ++k) launch_step(k, false);
+    HIPCHECK(hipStreamEndCapture(stream, &graph));
+    HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    graph_k = K;
+    graph_dirty = false;
+  }
+
+  void reset(uint64_t seed) override {
+    const size_t n = (size_t)N;
+    auto fill = [&](void *p, size_t count, T v) {
+      if (!count) return;
+      hipLaunchKernelGGL((fsk::k_fill<T>), dim3(1024), dim3(256), 0, stream, (T *)p, v, count);
+      HIPCHECK(hipGetLastError());
+    };
+    fill(b_V.p, n, T(-70.0f));  // VRESTING (code/EqsGnl2.cpp:347)
+    HIPCHECK(hipMemsetAsync(b_cs.p, 0, n * 4, stream));
+    fill(b_G.p, 4 * n, T(0));
+    fill(b_ring.p, (size_t)4 * net.D * n, T(0));
+    fill(b_spre.p, n, T(0));
+    fill(b_xn.p, n, T(0));
+    fill(b_shist.p, (size_t)net.DN * n, T(0));
+    fill(b_gn.p, n, T(0));
+    fill(b_elin.p, 4 * n, T(0));
+    fill(b_enx.p, n, T(0));
+    fill(b_ens.p, n, T(0));
+    HIPCHECK(hipMemsetAsync(b_popcnt.p, 0, (size_t)P * 8, stream));
+    HIPCHECK(hipMemsetAsync(b_counters.p, 0, 32, stream));
+    HIPCHECK(hipMemsetAsync(b_overflow.p, 0, 4, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    net.seed = seed;
+    graph_dirty = true;  // the seed is a kernel argument
+    step = 0;
+    std::memset(ctr_last, 0, sizeof(ctr_last));
+  }
+
+  void set_pop(int32_t i, const fs_pop_params &p) override {
+    if (i < 0 || i >= P) throw std::invalid_argument("population index out of range");
+    hpop[i] = p;
+    dpop[i] = to_dev(p, dpop[i]);
+    HIPCHECK(hipMemcpyAsync(b_pops.as<fsk::DevPop<T>>() + i, &dpop[i], sizeof(fsk::DevPop<T>),
+                            hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    update_ext_mask();
+  }
+  const fs_pop_params &pop(int32_t i) const override { return hpop.at(i); }
+
+  void load_ext(const uint32_t *bits, int64_t first, int64_t nsteps) override {
+    if (net.ext_mode != FS_EXT_REPLAY) throw std::invalid_argument("engine not created in replay mode");
+    const size_t bytes = (size_t)nsteps * 5 * net.words * 4;
+    b_ext.alloc(bytes);
+    HIPCHECK(hipMemcpy(b_ext.p, bits, bytes, hipMemcpyHostToDevice));
+    net.ext_bits = b_ext.as<uint32_t>();
+    net.ext_first = first;
+    net.ext_steps = nsteps;
+    graph_dirty = true;
+  }
+
+  void run(int64_t nsteps, int64_t *off, uint32_t *ids, int64_t cap) override {
+    if (net.ext_mode == FS_EXT_REPLAY) {
+      if (!net.ext_bits || step < net.ext_first || step + nsteps > net.ext_first + net.ext_steps)
+        throw std::invalid_argument("replayed external trains do not cover the requested steps");
+    }
+    int64_t written = 0;
+    if (off) off[0] = 0;
+    t_upd = t_del = 0;
+    n_timed = 0;
+    HIPCHECK(hipMemsetAsync(b_counters.p, 0, 32, stream));
+    int64_t done = 0;
+    while (done < nsteps) {
+      const int32_t K = (int32_t)std::min<int64_t>(sched.steps_per_graph, nsteps - done);
+      hipLaunchKernelGGL(fsk::k_batch_begin, dim3(1), dim3(1), 0, stream, b_t0.as<int64_t>(), step,
+                         b_total.as<uint32_t>(), b_cum.as<uint32_t>());
+      HIPCHECK(hipGetLastError());
+      launch_step(0, timing);
+      if (K > 1) {
+        if (sched.use_graph) {
+          if (graph_dirty || graph_k != K) build_graph(K);
+          HIPCHECK(hipGraphLaunch(gexec, stream));
+        } else {
+          for (int32_t k = 1; k < K; ++k) launch_step(k, false);
+        }
+      }
+      HIPCHECK(hipStreamSynchronize(stream));
+      if (timing) {
+        float a = 0, b = 0;
+        HIPCHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
+        HIPCHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
+        t_upd += a;
+        t_del += b;
+        n_timed++;
+      }
+      uint32_t ovf = 0;
+      HIPCHECK(hipMemcpy(&ovf, b_overflow.p, 4, hipMemcpyDeviceToHost));
+      if (ovf) throw std::length_error("device spike record overflow; lower steps_per_graph");
+      if (off || ids) {
+        cum_h.resize(K + 1);
+        HIPCHECK(hipMemcpy(cum_h.data(), b_cum.p, (size_t)(K + 1) * 4, hipMemcpyDeviceToHost));
+        const uint32_t tot = cum_h[K];
+        if (ids) {
+          if (written + (int64_t)tot > cap) throw std::length_error("spike id buffer too small");
+          if (tot) HIPCHECK(hipMemcpy(ids + written, b_rec_ids.p, (size_t)tot * 4, hipMemcpyDeviceToHost));
+          for (int32_t k = 0; k < K; ++k)
+            std::sort(ids + written + cum_h[k], ids + written + cum_h[k + 1]);
+        }
+        if (off)
+          for (int32_t k = 0; k < K; ++k) off[done + k + 1] = written + cum_h[k + 1];
+        written += tot;
+      }
+      step += K;
+      done += K;
+    }
+    HIPCHECK(hipMemcpy(ctr_last, b_counters.p, 32, hipMemcpyDeviceToHost));
+  }
+
+  void get_state(int32_t f, double *out) override {
+    const size_t n = (size_t)N;
+    std::vector<T> tmp(n);
+    auto d2h = [&](const void *p) {
+      HIPCHECK(hipMemcpy(tmp.data(), p, n * sizeof(T), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n; ++i) out[i] = (double)tmp[i];
+    };
+    switch (f) {
+      case 0: d2h(b_V.p); break;
+      case 1: case 2: {
+        std::vector<uint32_t> c(n);
+        HIPCHECK(hipMemcpy(c.data(), b_cs.p, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+          if (f == 1) out[i] = (double)(c[i] & fsk::CNT_MASK);
+          else {
+            uint32_t s = 0;  // reference NState bits (code/Inc.h:225-231)
+            if (c[i] & fsk::F_FIRE) s |= 1;
+            if (c[i] & fsk::F_SPIKE) s |= 2;
+            if (c[i] & fsk::F_OVF) s |= 16;
+            if (c[i] & fsk::F_UDF) s |= 32;
+            out[i] = (double)s;
+          }
+        }
+        break;
+      }
+      case 3: case 4: case 6: case 7: {
+        const int rid = f - 3;
+        const int li = rid < 2 ? rid : rid - 1;
+        hipLaunchKernelGGL((fsk::k_read_g<T>), dim3((N + 255) / 256), dim3(256), 0, stream, net,
+                           step - 1, li, b_tmp.as<T>());
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(stream));
+        d2h(b_tmp.p);
+        break;
+      }
+      case 5: d2h(b_gn.p); break;
+      case 8: d2h(b_spre.p); break;
+      case 9: d2h(b_xn.p); break;
+      case 10: {
+        const int64_t s = ((step - 1) % net.DN + net.DN) % net.DN;
+        d2h(b_shist.as<T>() + (size_t)s * n);
+        break;
+      }
+      default: throw std::invalid_argument("unknown state field");
+    }
+  }
+
+  void pop_counts(uint64_t *out) override {
+    HIPCHECK(hipMemcpy(out, b_popcnt.p, (size_t)P * 8, hipMemcpyDeviceToHost));
+  }
+  void times(double *u, double *p, double *g, int64_t *n) override {
+    if (u) *u = n_timed ? t_upd / n_timed : 0.0;
+    if (p) *p = n_timed ? t_del / n_timed : 0.0;
+    if (g) *g = 0.0;
+    if (n) *n = n_timed;
+  }
+  void counters(int64_t *s, int64_t *e, int64_t *t) override {
+    if (s) *s = (int64_t)ctr_last[0];
+    if (e) *e = (int64_t)ctr_last[1];
+    if (t) *t = (int64_t)ctr_last[2];
+  }
+};
+
+}  // namespace
+
+struct fs_handle {
+  std::unique_ptr<EngineBase> e;
+};
+
+#define ENGINE_CALL(...)                                               \
+  try {                                                                 \
+    __VA_ARGS__;                                                             \
+    return FS_OK;                                                       \
+  } catch (const HipError &ex) {                                        \
+    return fs::set_error(FS_ERR_HIP, ex.what());                        \
+  } catch (const std::length_error &ex) {                               \
+    return fs::set_error(FS_ERR_CAPACITY, ex.what());                   \
+  } catch (const std::bad_alloc &) {                                    \
+    return fs::set_error(FS_ERR_CAPACITY, "host allocation failed");    \
+  } catch (const std::exception &ex) {                                  \
+    return fs::set_error(FS_ERR_ARG, ex.what());                        \
+  }
+
+extern "C" {
+
+int fs_create(const fs_net_desc *d, fs_handle **out) {
+  if (!d || !out) return fs::set_error(FS_ERR_ARG, "null argument");
+  if (!d->neuron_pop || !d->pops || (d->n_syn > 0 && (!d->pre || !d->post || !d->w || !d->receptor)))
+    return fs::set_error(FS_ERR_ARG, "incomplete network description");
+  if (!(d->dt > 0.0f)) return fs::set_error(FS_ERR_ARG, "dt must be positive");
+  ENGINE_CALL({
+    auto h = std::make_unique<fs_handle>();
+    if (d->precision == FS_FP64) h->e.reset(new Engine<double>(*d));
+    else if (d->precision == FS_FP32) h->e.reset(new Engine<float>(*d));
+    else throw std::invalid_argument("precision must be FS_FP32 or FS_FP64");
+    *out = h.release();
+  })
+}
+
+int fs_destroy(fs_handle *h) {
+  delete h;
+  return FS_OK;
+}
+
+int fs_reset(fs_handle *h, uint64_t seed) {
+  if (!h) return fs::set_error(FS_ERR_ARG, "null handle");
+  ENGINE_CALL(h->e->reset(seed))
+}
+
+int fs_set_ext_rate(fs_handle *h, int32_t pop, int32_t receptor, float hz) {
+  if (!h) return fs::set_error(FS_ERR_ARG, "null handle");
+  if (receptor < 0 || receptor > FS_GCL) return fs::set_error(FS_ERR_ARG, "receptor out of range");
+  ENGINE_CALL({
+    fs_pop_params p = h->e->pop(pop);
+    static const uint32_t bit[5] = {FS_ACT_AMPA, FS_ACT_GABA, FS_ACT_NMDA, FS_ACT_ACH, FS_ACT_GCL};
+    if (hz == 0.0f) p.act &= ~bit[receptor];
+    else {
+      p.act |= bit[receptor];
+      p.pb[receptor] = (float)(hz * h->e->dt * 1e-3f);
+    }
+    h->e->set_pop(pop, p);
+  })
+}
+
+int fs_set_injection(fs_handle *h, int32_t pop, float mean, float sd) {
+  if (!h) return fs::set_error(FS_ERR_ARG, "null handle");
+  ENGINE_CALL({
+    fs_pop_params p = h->e->pop(pop);
+    p.inj_mean = mean;
+    p.inj_std = sd * (1.0f + (float)std::log((1.0f - std::exp(-0.1f / p.taum)) / (1.0f - p.df)));
+    if (mean == 0.0f && sd == 0.0f) p.act &= ~(uint32_t)FS_ACT_INJ;
+    else p.act |= FS_ACT_INJ;
+    h->e->set_pop(pop, p);
+  })
+}
+
+int fs_set_pop_params(fs_handle *h, int32_t pop, const fs_pop_params *p) {
+  if (!h || !p) return fs::set_error(FS_ERR_ARG, "null argument");
+  ENGINE_CALL(h->e->set_pop(pop, *p))
+}
+
+int fs_load_ext_trains(fs_handle *h, const uint32_t *bits, int64_t first, int64_t n) {
+  if (!h || !bits || n <= 0 || first < 0) return fs::set_error(FS_ERR_ARG, "bad argument");
+  ENGINE_CALL(h->e->load_ext(bits, first, n))
+}
+
+int fs_run(fs_handle *h, int64_t n, int64_t *off, uint32_t *ids, int64_t cap) {
+  if (!h || n < 0) return fs::set_error(FS_ERR_ARG, "bad argument");
+  ENGINE_CALL(h->e->run(n, off, ids, cap))
+}
+
+int fs_step_index(fs_handle *h, int64_t *s) {
+  if (!h || !s) return fs::set_error(FS_ERR_ARG, "null argument");
+  *s = h->e->step;
+  return FS_OK;
+}
+
+int fs_get_state(fs_handle *h, int32_t f, double *out) {
+  if (!h || !out) return fs::set_error(FS_ERR_ARG, "null argument");
+  ENGINE_CALL(h->e->get_state(f, out))
+}
+
+int fs_pop_spike_counts(fs_handle *h, uint64_t *out) {
+  if (!h || !out) return fs::set_error(FS_ERR_ARG, "null argument");
+  ENGINE_CALL(h->e->pop_counts(out))
+}
+
+int fs_get_schedule(fs_handle *h, fs_schedule *s) {
+  if (!h || !s) return fs::set_error(FS_ERR_ARG, "null argument");
+  *s = h->e->sched;
+  return FS_OK;
+}
+
+int fs_set_schedule(fs_handle *h, const fs_schedule *s) {
+  if (!h || !s) return fs::set_error(FS_ERR_ARG, "null argument");
+  ENGINE_CALL({
+    const fs_schedule old = h->e->sched;
+    h->e->sched = *s;
+    try {
+      static_cast<void>(0);
+      if (auto *e32 = dynamic_cast<Engine<float> *>(h->e.get())) e32->apply_schedule();
+      else if (auto *e64 = dynamic_cast<Engine<double> *>(h->e.get())) e64->apply_schedule();
+    } catch (...) {
+      h->e->sched = old;
+      throw;
+    }
+  })
+}
+
+int fs_enable_timing(fs_handle *h, int32_t on) {
+  if (!h) return fs::set_error(FS_ERR_ARG, "null handle");
+  h->e->timing = on != 0;
+  return FS_OK;
+}
+
+int fs_kernel_times(fs_handle *h, double *u, double *p, double *g, int64_t *n) {
+  if (!h) return fs::set_error(FS_ERR_ARG, "null handle");
+  h->e->times(u, p, g, n);
+  return FS_OK;
+}
+
+int fs_run_counters(fs_handle *h, int64_t *s, int64_t *e, int64_t *t) {
+  if (!h) return fs::set_error(FS_ERR_ARG, "null handle");
+  h->e->counters(s, e, t);
+  return FS_OK;
+}
+
+}  // extern "C"

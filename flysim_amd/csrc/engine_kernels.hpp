@@ -1,0 +1,379 @@
+// Device kernels of the Flysim engine (gfx950, wave64).
+//
+// One simulated step t is two launches on the engine's stream:
+//   k_update  : per neuron, elementwise over SoA state -- ring-slot intake of
+//               the linear conductances, external Poisson gates (counter RNG
+//               or replayed bits), conductance LIF update with NMDA Mg block
+//               and the refractory state machine, presynaptic gate update,
+//               and wave-ballot compaction of the step's spike list;
+//   k_deliver : CSR push of the compacted spike list into per-delay ring
+//               buffers (one wave per spike, float/double atomics) and, in
+//               the same grid, the NMDA pull-gather over CSR-by-post rows
+//               (a group of L lanes per row with a shuffle reduction).
+// The equations restate code/EqsGnl2.cpp:152-337 (VmEul, firing, BgInit) and
+// :404-502 / :866-1002 (gates and sums).  For the linear receptors the
+// per-synapse gates of the reference are replaced by the exact push identity
+//   G_p(t) = df_p G_p(t-1) + sum_{j spiking at t-d} g_jp (1 + (1-df_p) s_j(t-d-1))
+// where s_j is the presynaptic gate shared by all of j's synapses; NMDA keeps
+// the per-step gather G_N(p) = sum_j g_jp s_j(t-d_jp) because its gate
+// saturates (the (1-s) term) and has no push form.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsk {
+
+// state word: refractory count in bits 0..23, flags above
+constexpr uint32_t CNT_MASK = 0x00FFFFFFu;
+constexpr uint32_t F_FIRE = 1u << 24, F_SPIKE = 1u << 25, F_OVF = 1u << 28, F_UDF = 1u << 29;
+constexpr uint32_t ACT_AMPA = 1, ACT_GABA = 2, ACT_NMDA = 4, ACT_ACH = 32, ACT_GCL = 64,
+                   ACT_INJ = 128;
+// linear receptor slots li = 0..3 map to receptor ids A=0, G=1, H=3, L=4
+__host__ __device__ constexpr int lin_rid(int li) { return li < 2 ? li : li + 1; }
+__host__ __device__ constexpr uint32_t lin_act(int li) {
+  return li == 0 ? ACT_AMPA : li == 1 ? ACT_GABA : li == 2 ? ACT_ACH : ACT_GCL;
+}
+
+// Per-population parameters in the precision of the run.
+template <typename T>
+struct DevPop {
+  T vth, leak_drive, cm, vreset, df, ef, dvmax, mg;
+  T vrev[5];
+  T dfst[5];            // postsynaptic gate decay by receptor id
+  T dfxt_n, efas_n;     // postsynaptic NMDA (external gate) x decay, efst*as
+  T gext[5];
+  T pre_df;             // presynaptic linear gate decay (targets' dfst)
+  T pre_dfxt, pre_dfst, pre_efas;  // presynaptic NMDA gate (targets' params)
+  T inj_mean, inj_std;
+  float pb[5];
+  int32_t rfped, spidy;
+  uint32_t act;
+  uint32_t has_lin_out, has_nmda_out;
+  uint32_t pad;
+};
+
+template <typename T>
+struct Net {
+  int32_t N;
+  int32_t D;            // ring depth (linear receptors)
+  int32_t DN;           // NMDA gate history depth
+  int32_t lin_mask;     // bit li set: linear receptor slot li has synapses
+  int32_t ext_mask;     // bit rid set: some population has external drive on rid
+  int32_t has_nmda_syn;
+  int32_t ext_mode;     // 0 philox, 1 replay
+  int32_t words;        // ceil(N/32)
+  const uint32_t *npop;
+  const DevPop<T> *pops;
+  T *V;
+  uint32_t *cs;
+  T *G;                 // [4][N] running linear conductances
+  T *ring;              // [4][D][N]
+  T *spre;              // [N] linear presynaptic gate
+  T *xn;                // [N] NMDA presynaptic x
+  T *shist;             // [DN][N] NMDA presynaptic s history
+  T *gn;                // [N] gathered NMDA conductance of the last step
+  T *elin;              // [4][N] external linear gates
+  T *enx, *ens;         // [N] external NMDA gate
+  // push CSR (linear)
+  const uint64_t *rp_p;
+  const uint32_t *meta_p;   // col | delay<<24 | li<<30
+  const float *w_p;
+  // gather CSR (NMDA)
+  const uint64_t *rp_n;
+  const uint32_t *meta_n;   // pre | delay<<24
+  const float *w_n;
+  // replayed external bits
+  const uint32_t *ext_bits;
+  int64_t ext_first, ext_steps;
+  uint64_t seed;
+  // spike record of the current batch
+  uint32_t *rec_ids;
+  T *rec_a;
+  uint32_t *total;          // [1] spikes recorded in this batch
+  uint32_t *cum;            // [K+1] per-step offsets
+  uint32_t *overflow;       // [1]
+  uint64_t rec_cap;
+  const int64_t *t0;        // [1] first step of the batch
+  unsigned long long *pop_cnt;  // [P]
+  unsigned long long *counters; // [4]: spikes, syn events, nmda terms, reserved
+  int32_t gather_lanes;
+  int32_t push_waves;       // waves of k_deliver dedicated to the push
+};
+
+// ------------------------------------------------------------------ RNG
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based.
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+    const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+__device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// ------------------------------------------------------------------ update
+template <typename T, bool REPLAY>
+__global__ void __launch_bounds__(1024) k_update(Net<T> net, int32_t k) {
+  const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = n < net.N;
+  const int64_t t = *net.t0 + k;
+  bool spk = false;
+  T a_push = T(0);
+  if (active) {
+    const int32_t N = net.N;
+    const DevPop<T> &P = net.pops[net.npop[n]];
+    uint32_t cs = net.cs[n];
+    uint32_t cnt = cs & CNT_MASK;
+    T v = net.V[n];
+    // (1) linear conductances of the previous Spike phase: G = df*G + slot
+    T g[5] = {T(0), T(0), T(0), T(0), T(0)};
+    const int32_t slot = (int32_t)((t + net.D - 1) % net.D);
+#pragma unroll
+    for (int li = 0; li < 4; ++li) {
+      if (!(net.lin_mask & (1 << li))) continue;
+      const int rid = lin_rid(li);
+      T *rs = net.ring + ((size_t)li * net.D + slot) * N + n;
+      const T inc = *rs;
+      *rs = T(0);
+      T *gp = net.G + (size_t)li * N + n;
+      const T gv = P.dfst[rid] * *gp + inc;
+      *gp = gv;
+      g[rid] = gv;
+    }
+    if (net.has_nmda_syn) g[2] = net.gn[n];
+    // (2) external Poisson gates, order A, G, ACh, GluCl, NMDA (BgInit)
+    const uint32_t act = P.act;
+    uint32_t rnd[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+    if ((!REPLAY && net.ext_mask) || (act & ACT_INJ)) {
+      uint32_t c[4] = {(uint32_t)n, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0u};
+      philox10(c, (uint32_t)net.seed, (uint32_t)(net.seed >> 32));
+      rnd[0] = c[0]; rnd[1] = c[1]; rnd[2] = c[2]; rnd[3] = c[3];
+      if ((net.ext_mask & (1 << 4)) || (act & ACT_INJ)) {
+        uint32_t e[4] = {(uint32_t)n, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 1u};
+        philox10(e, (uint32_t)net.seed, (uint32_t)(net.seed >> 32));
+        rnd[4] = e[0]; rnd[5] = e[1]; rnd[6] = e[2]; rnd[7] = e[3];
+      }
+    }
+    auto draw = [&](int rid) -> bool {
+      if (REPLAY) {
+        const int64_t s = t - net.ext_first;
+        if (s < 0 || s >= net.ext_steps) return false;
+        return (net.ext_bits[((size_t)s * 5 + rid) * net.words + (n >> 5)] >> (n & 31)) & 1u;
+      }
+      return P.pb[rid] > u24(rnd[rid]);
+    };
+#pragma unroll
+    for (int li = 0; li < 4; ++li) {
+      const int rid = lin_rid(li);
+      if (!(net.ext_mask & (1 << rid)) || !(act & lin_act(li))) continue;
+      T *ep = net.elin + (size_t)li * N + n;
+      T e = *ep;
+      e = draw(rid) ? e + T(1) : e * P.dfst[rid];
+      *ep = e;
+      g[rid] = g[rid] + P.gext[rid] * e;
+    }
+    if ((net.ext_mask & (1 << 2)) && (act & ACT_NMDA)) {
+      T x = net.enx[n], s = net.ens[n];
+      x = draw(2) ? x + T(1) : x * P.dfxt_n;
+      s = P.dfst[2] * s + P.efas_n * x * (T(1) - s);
+      net.enx[n] = x;
+      net.ens[n] = s;
+      g[2] = g[2] + P.gext[2] * s;
+    }
+    T isyn = T(0);
+    if (act & ACT_INJ) {
+      // Box-Muller on the second Philox block (statistically the reference's
+      // Gaussian injection; the reference draws from a sequential ziggurat)
+      const float u1 = ((float)(rnd[6] >> 8) + 1.0f) * (1.0f / 16777216.0f);
+      const float u2 = u24(rnd[7]);
+      const float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+      isyn = P.inj_mean + P.inj_std * T(z);
+    }
+    // (3) membrane (VmEul / firing)
+    auto firing = [&]() {
+      if ((int32_t)cnt < P.rfped) {
+        v = cnt == 0 ? T(0) : P.vreset;
+        if ((int32_t)cnt == P.spidy) cs |= F_SPIKE;
+        else cs &= ~F_SPIKE;
+        cnt++;
+      } else {
+        cnt = 0;
+        cs &= ~F_FIRE;
+      }
+    };
+    if (cs & F_FIRE) {
+      firing();
+    } else {
+      const T i0 = g[0] * (P.vrev[0] - v);
+      const T i1 = g[1] * (P.vrev[1] - v);
+      T ex;
+      if constexpr (sizeof(T) == 4) ex = expf(T(-0.062f) * v);
+      else ex = exp(T(-0.062f) * v);
+      const T i2 = g[2] * (P.vrev[2] - v) / (T(1) + P.mg * ex / T(3.57f));
+      const T i3 = g[3] * (P.vrev[3] - v);
+      const T i4 = g[4] * (P.vrev[4] - v);
+      T I = i0 + i1 + i2 + i3 + i4 + isyn * T(1e3f) + P.leak_drive;
+      I = T(1e-3f) * I / P.cm;
+      cs &= ~(F_OVF | F_UDF);
+      if (I > P.dvmax) { I = P.dvmax; cs |= F_OVF; }
+      else if (I < -P.dvmax) { I = -P.dvmax; cs |= F_UDF; }
+      v = P.df * v + P.ef * I;
+      if (v > P.vth) { cs |= F_FIRE; firing(); }
+    }
+    spk = (cs & F_SPIKE) != 0;
+    // (4) presynaptic gates (the reference's per-synapse gate rule, shared)
+    if (P.has_lin_out) {
+      const T s = net.spre[n];
+      a_push = T(1) + (T(1) - P.pre_df) * s;
+      net.spre[n] = spk ? s + T(1) : s * P.pre_df;
+    }
+    if (P.has_nmda_out) {
+      T x = net.xn[n];
+      const int32_t sp = (int32_t)((t + net.DN - 1) % net.DN);
+      const int32_t sc = (int32_t)(t % net.DN);
+      const T s0 = net.shist[(size_t)sp * N + n];
+      x = spk ? x + T(1) : x * P.pre_dfxt;
+      net.xn[n] = x;
+      net.shist[(size_t)sc * N + n] = P.pre_dfst * s0 + P.pre_efas * x * (T(1) - s0);
+    }
+    net.V[n] = v;
+    net.cs[n] = (cs & ~CNT_MASK) | (cnt & CNT_MASK);
+  }
+  // (5) wave-ballot compaction of the spike list
+  const uint64_t mask = __ballot(spk);
+  if (mask) {
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t cntw = (uint32_t)__popcll(mask);
+    uint32_t base = 0;
+    if (lane == leader) {
+      base = atomicAdd(net.total, cntw);
+      atomicAdd(&net.counters[0], (unsigned long long)cntw);
+    }
+    base = __shfl(base, leader);
+    if (spk) {
+      const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+      if (pos < net.rec_cap) {
+        net.rec_ids[pos] = (uint32_t)n;
+        net.rec_a[pos] = a_push;
+      } else {
+        atomicOr(net.overflow, 1u);
+      }
+      // per-population spike count, aggregated when the wave shares one population
+      const uint32_t p = net.npop[n];
+      const uint32_t p0 = __shfl(p, leader);
+      const bool same = __ballot(spk && p != p0) == 0ull;
+      if (same) {
+        if (lane == leader) atomicAdd(&net.pop_cnt[p0], (unsigned long long)cntw);
+      } else {
+        atomicAdd(&net.pop_cnt[p], 1ull);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ delivery
+template <typename T>
+__device__ __forceinline__ void atomic_add_t(T *p, T v) {
+  unsafeAtomicAdd(p, v);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_deliver(Net<T> net, int32_t k) {
+  const int64_t t = *net.t0 + k;
+  const int32_t N = net.N;
+  const int wave = (int)(threadIdx.x >> 6);
+  const int lane = (int)(threadIdx.x & 63);
+  const int waves_per_block = (int)(blockDim.x >> 6);
+  const int gwave = (int)blockIdx.x * waves_per_block + wave;
+  const int total_waves = (int)gridDim.x * waves_per_block;
+  unsigned long long my_events = 0, my_terms = 0;
+  const uint32_t end_raw = *net.total;
+  if (gwave < net.push_waves) {
+    // ---- push: one wave per spike of step t
+    const uint32_t start = net.cum[k];
+    const uint32_t end = end_raw < net.rec_cap ? end_raw : (uint32_t)net.rec_cap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) net.cum[k + 1] = end;
+    for (uint32_t i = start + (uint32_t)gwave; i < end; i += (uint32_t)net.push_waves) {
+      const uint32_t j = net.rec_ids[i];
+      const T a = net.rec_a[i];
+      const uint64_t r0 = net.rp_p[j], r1 = net.rp_p[j + 1];
+      if (lane == 0) my_events += r1 - r0;
+      for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
+        const uint32_t m = net.meta_p[e];
+        const uint32_t col = m & 0x00FFFFFFu;
+        const uint32_t d = (m >> 24) & 63u;
+        const uint32_t li = m >> 30;
+        const int32_t slot = (int32_t)((t + (int64_t)d) % net.D);
+        atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(net.w_p[e]) * a);
+      }
+    }
+  } else if (net.has_nmda_syn) {
+    // ---- NMDA gather over CSR-by-post, L lanes per row
+    const int L = net.gather_lanes;
+    const int gw = gwave - net.push_waves;
+    const int gwaves = total_waves - net.push_waves;
+    const int groups_per_wave = 64 / L;
+    const int sub = lane / L, sl = lane % L;
+    const int64_t stride = (int64_t)gwaves * groups_per_wave;
+    for (int64_t p = (int64_t)gw * groups_per_wave + sub; p - sub < N; p += stride) {
+      T acc = T(0);
+      if (p < N) {
+        const uint64_t r0 = net.rp_n[p], r1 = net.rp_n[p + 1];
+        if (sl == 0) my_terms += r1 - r0;
+        for (uint64_t e = r0 + (uint64_t)sl; e < r1; e += (uint64_t)L) {
+          const uint32_t m = net.meta_n[e];
+          const uint32_t j = m & 0x00FFFFFFu;
+          const uint32_t d = m >> 24;
+          const int32_t hs = (int32_t)(((t - (int64_t)d) % net.DN + net.DN) % net.DN);
+          acc += T(net.w_n[e]) * net.shist[(size_t)hs * N + j];
+        }
+      }
+      for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+      if (p < N && sl == 0) net.gn[p] = acc;
+    }
+  }
+  // block-level reduction of the event counters: one atomic per block
+  __shared__ unsigned long long s_ev[2];
+  if (threadIdx.x == 0) { s_ev[0] = 0; s_ev[1] = 0; }
+  __syncthreads();
+  if (my_events) atomicAdd(&s_ev[0], my_events);
+  if (my_terms) atomicAdd(&s_ev[1], my_terms);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_ev[0]) atomicAdd(&net.counters[1], s_ev[0]);
+    if (s_ev[1]) atomicAdd(&net.counters[2], s_ev[1]);
+  }
+}
+
+// batch prologue: first step of the batch, record buffer reset
+__global__ void k_batch_begin(int64_t *t0, int64_t value, uint32_t *total, uint32_t *cum) {
+  *t0 = value;
+  *total = 0u;
+  cum[0] = 0u;
+}
+
+template <typename T>
+__global__ void k_fill(T *p, T v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// reference-equivalent synaptic conductance after the Spike phase of the
+// last step (G_r(t) = df G_r(t-1) + pending slot t), for state readback
+template <typename T>
+__global__ void k_read_g(Net<T> net, int64_t t, int li, T *out) {
+  const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= net.N) return;
+  const DevPop<T> &P = net.pops[net.npop[n]];
+  const int32_t slot = (int32_t)(((t % net.D) + net.D) % net.D);
+  out[n] = P.dfst[lin_rid(li)] * net.G[(size_t)li * net.N + n] +
+           net.ring[((size_t)li * net.D + slot) * net.N + n];
+}
+
+}  // namespace fsk

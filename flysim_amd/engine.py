@@ -1,0 +1,269 @@
+"""Python front end of the Flysim core: the drop-in configuration loader and
+the GPU engine, both through the C ABI of include/flysim.h."""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _capi as A
+from .workloads import Network, Population
+
+STATE_FIELDS = {"V": 0, "count": 1, "nstate": 2, "gA": 3, "gG": 4, "gN": 5, "gH": 6, "gL": 7,
+                "s_lin": 8, "x_nmda": 9, "s_nmda": 10}
+
+
+def pop_conf_of(p: Population) -> A.PopConf:
+    """Conf-level values of one population (what network.conf states)."""
+    c = A.PopConf()
+    c.cm, c.taum, c.vth, c.el, c.vreset = p.cm, p.taum, p.vth, p.rest, p.reset
+    c.refractory_ms, c.spike_delay_ms = p.refractory_ms, p.spike_delay_ms
+    for r in p.receptors:
+        k = r.kind
+        c.declared[k] = 1
+        c.tau[k], c.tau_xt[k], c.vrev[k] = r.tau, r.tau_xt, r.rev
+        c.freq_ext[k], c.ext_eff[k], c.ext_con[k] = r.freq_ext, r.mean_ext_eff, r.mean_ext_con
+    return c
+
+
+def derive_params(p: Population, dt: float = 0.1, act: int = A.DEFAULT_ACT) -> A.PopParams:
+    """Derived per-population parameters, computed by the library exactly as
+    the reference's SetPp computes them (float arithmetic)."""
+    out = A.PopParams()
+    A.check(A.lib().fs_derive_params(C.byref(pop_conf_of(p)), C.c_float(dt), act, C.byref(out)))
+    return out
+
+
+class NetArrays:
+    """Owns the numpy arrays an fs_net_desc points into."""
+
+    def __init__(self, neuron_pop, pops, pre, post, w, receptor, delay, dt):
+        self.neuron_pop = np.ascontiguousarray(neuron_pop, dtype=np.uint32)
+        self.pops = (A.PopParams * len(pops))(*pops)
+        self.pre = np.ascontiguousarray(pre, dtype=np.uint32)
+        self.post = np.ascontiguousarray(post, dtype=np.uint32)
+        self.w = np.ascontiguousarray(w, dtype=np.float32)
+        self.receptor = np.ascontiguousarray(receptor, dtype=np.uint8)
+        self.delay = None if delay is None else np.ascontiguousarray(delay, dtype=np.uint8)
+        self.dt = float(dt)
+
+    @property
+    def n(self) -> int:
+        return int(self.neuron_pop.shape[0])
+
+    def desc(self, precision: int, ext_mode: int, record: bool = True) -> A.NetDesc:
+        d = A.NetDesc()
+        d.n_neuron = self.n
+        d.n_pop = len(self.pops)
+        d.neuron_pop = A.ptr(self.neuron_pop, C.c_uint32)
+        d.pops = C.cast(self.pops, C.POINTER(A.PopParams))
+        d.n_syn = int(self.pre.shape[0])
+        d.pre = A.ptr(self.pre, C.c_uint32)
+        d.post = A.ptr(self.post, C.c_uint32)
+        d.w = A.ptr(self.w, C.c_float)
+        d.receptor = A.ptr(self.receptor, C.c_uint8)
+        d.delay = A.ptr(self.delay, C.c_uint8) if self.delay is not None else None
+        d.dt = self.dt
+        d.precision = precision
+        d.ext_mode = ext_mode
+        d.record_spikes = 1 if record else 0
+        return d
+
+
+def arrays_from_network(net: Network, act: int = A.DEFAULT_ACT) -> NetArrays:
+    pops = [derive_params(p, net.dt, act) for p in net.pops]
+    off = net.pop_offsets()
+    npop = np.repeat(np.arange(len(net.pops), dtype=np.uint32), np.diff(off))
+    return NetArrays(npop, pops, net.pre, net.post, net.w, net.receptor, net.delay, net.dt)
+
+
+class Conf:
+    """A network.conf / network.pro pair loaded and built by the library
+    (parser and builder of the drop-in API)."""
+
+    def __init__(self, conf: str, pro: str, dt: float = 0.1, threads: int = 1, repeats: int = 1):
+        o = A.CliOpts()
+        o.dt, o.threads, o.repeats = dt, threads, repeats
+        h = C.c_void_p()
+        A.check(A.lib().fs_conf_load(conf.encode(), pro.encode(), C.byref(o), C.byref(h)))
+        self._h = h
+        s = np.zeros(7, dtype=np.int64)
+        A.check(A.lib().fs_conf_sizes(self._h, s.ctypes.data))
+        (self.n_neuron, self.n_pop, self.n_syn, self.total_synapses, self.n_events,
+         self.n_outputs, self.n_repeats) = [int(x) for x in s]
+
+    def close(self):
+        if self._h:
+            A.lib().fs_conf_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def seed(self, repeat: int = 0) -> int:
+        v = C.c_uint32()
+        A.check(A.lib().fs_conf_seed(self._h, repeat, C.byref(v)))
+        return int(v.value)
+
+    def desc(self) -> A.NetDesc:
+        d = A.NetDesc()
+        A.check(A.lib().fs_conf_desc(self._h, C.byref(d)))
+        return d
+
+    def ext_trains(self, n_steps: int, repeat: int = 0) -> np.ndarray:
+        words = (self.n_neuron + 31) // 32
+        bits = np.zeros(n_steps * 5 * words, dtype=np.uint32)
+        A.check(A.lib().fs_conf_ext_trains(self._h, repeat, n_steps, bits.ctypes.data))
+        return bits
+
+    def ref_layout(self) -> Dict[str, np.ndarray]:
+        L = A.RefLayout()
+        A.check(A.lib().fs_conf_ref_layout(self._h, C.byref(L)))
+
+        def arr(p, n, dt):
+            if n == 0:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
+
+        n_term = int(L.row_beg[L.n_row]) if L.n_row else 0
+        pc = [L.pop_conf[i] for i in range(self.n_pop)]
+        return {
+            "gate_pre": arr(L.gate_pre, L.n_gate, np.uint32),
+            "gate_post0": arr(L.gate_post0, L.n_gate, np.uint32),
+            "gate_rec": arr(L.gate_rec, L.n_gate, np.uint8),
+            "gate_group": arr(L.gate_group, L.n_gate, np.uint8),
+            "gate_g": arr(L.gate_g, L.n_gate, np.float32),
+            "row_post": arr(L.row_post, L.n_row, np.uint32),
+            "row_rec": arr(L.row_rec, L.n_row, np.uint8),
+            "row_beg": arr(L.row_beg, L.n_row + 1, np.uint32),
+            "term": arr(L.term, n_term, np.uint32),
+            "neuron_pop": arr(L.neuron_pop, self.n_neuron, np.uint32),
+            "pop_conf": pc,
+            "rfped": arr(L.pop_rfped, self.n_pop, np.int32),
+            "spidy": arr(L.pop_spidy, self.n_pop, np.int32),
+            "evt_time_s": arr(L.evt_time_s, L.n_evt, np.float32),
+            "evt_var1": arr(L.evt_var1, L.n_evt, np.float32),
+            "evt_var2": arr(L.evt_var2, L.n_evt, np.float32),
+            "evt_type": arr(L.evt_type, L.n_evt, np.int32),
+            "evt_pop": arr(L.evt_pop, L.n_evt, np.int32),
+            "evt_sub": arr(L.evt_sub, L.n_evt, np.int32),
+            "act_ctr": int(L.act_ctr),
+        }
+
+
+class Engine:
+    """One network (or a batch of independent instances) resident on the GPU."""
+
+    def __init__(self, net, precision: int = A.FS_FP32, ext_mode: int = A.FS_EXT_PHILOX,
+                 record: bool = True):
+        self._keep = net
+        if isinstance(net, Conf):
+            d = net.desc()
+            d.precision, d.ext_mode, d.record_spikes = precision, ext_mode, int(record)
+            self.n = net.n_neuron
+            self.n_pop = net.n_pop
+        else:
+            d = net.desc(precision, ext_mode, record)
+            self.n = net.n
+            self.n_pop = len(net.pops)
+        self.dt = float(d.dt)
+        h = C.c_void_p()
+        A.check(A.lib().fs_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "_h", None):
+            A.lib().fs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self, seed: int = 0) -> None:
+        A.check(A.lib().fs_reset(self._h, int(seed)))
+
+    def load_ext_trains(self, bits: np.ndarray, first_step: int, n_steps: int) -> None:
+        self._bits = np.ascontiguousarray(bits, dtype=np.uint32)
+        A.check(A.lib().fs_load_ext_trains(self._h, self._bits.ctypes.data, first_step, n_steps))
+
+    def run(self, n_steps: int, record: bool = True) -> Tuple[np.ndarray, np.ndarray]:
+        """Run n steps; returns (offsets[n+1], ids) of the per-step spike lists."""
+        if not record:
+            A.check(A.lib().fs_run(self._h, n_steps, None, None, 0))
+            return np.zeros(0, np.int64), np.zeros(0, np.uint32)
+        off = np.zeros(n_steps + 1, dtype=np.int64)
+        cap = max(1024, self.n * min(n_steps, 2000))
+        while True:
+            ids = np.zeros(cap, dtype=np.uint32)
+            s0 = self.step
+            rc = A.lib().fs_run(self._h, n_steps, off.ctypes.data, ids.ctypes.data, cap)
+            if rc == -3 and self.step == s0 and cap < self.n * n_steps:
+                cap = self.n * n_steps
+                continue
+            A.check(rc)
+            return off, ids[: off[-1]].copy()
+
+    def raster(self, n_steps: int) -> np.ndarray:
+        """Run n steps and return an (k, 2) int64 array of (step, nid)."""
+        s0 = self.step
+        off, ids = self.run(n_steps)
+        steps = np.repeat(np.arange(s0, s0 + n_steps, dtype=np.int64), np.diff(off))
+        return np.stack([steps, ids.astype(np.int64)], axis=1)
+
+    @property
+    def step(self) -> int:
+        v = C.c_int64()
+        A.check(A.lib().fs_step_index(self._h, C.byref(v)))
+        return int(v.value)
+
+    def state(self, field: str) -> np.ndarray:
+        out = np.zeros(self.n, dtype=np.float64)
+        A.check(A.lib().fs_get_state(self._h, STATE_FIELDS[field], out.ctypes.data))
+        return out
+
+    def pop_counts(self) -> np.ndarray:
+        out = np.zeros(self.n_pop, dtype=np.uint64)
+        A.check(A.lib().fs_pop_spike_counts(self._h, out.ctypes.data))
+        return out
+
+    def set_ext_rate(self, pop: int, receptor: int, hz: float) -> None:
+        A.check(A.lib().fs_set_ext_rate(self._h, pop, receptor, C.c_float(hz)))
+
+    def set_injection(self, pop: int, mean: float, std: float) -> None:
+        A.check(A.lib().fs_set_injection(self._h, pop, C.c_float(mean), C.c_float(std)))
+
+    def set_pop_params(self, pop: int, p: A.PopParams) -> None:
+        A.check(A.lib().fs_set_pop_params(self._h, pop, C.byref(p)))
+
+    @property
+    def schedule(self) -> Dict[str, int]:
+        s = A.Schedule()
+        A.check(A.lib().fs_get_schedule(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in A.Schedule._fields_}
+
+    def set_schedule(self, **kw) -> None:
+        cur = self.schedule
+        cur.update(kw)
+        s = A.Schedule(**cur)
+        A.check(A.lib().fs_set_schedule(self._h, C.byref(s)))
+
+    def enable_timing(self, on: bool = True) -> None:
+        A.check(A.lib().fs_enable_timing(self._h, int(on)))
+
+    def kernel_times(self) -> Dict[str, float]:
+        u, p, g, n = C.c_double(), C.c_double(), C.c_double(), C.c_int64()
+        A.check(A.lib().fs_kernel_times(self._h, C.byref(u), C.byref(p), C.byref(g), C.byref(n)))
+        return {"update_ms": u.value, "deliver_ms": p.value, "n_timed": n.value}
+
+    def counters(self) -> Dict[str, int]:
+        s, e, t = C.c_int64(), C.c_int64(), C.c_int64()
+        A.check(A.lib().fs_run_counters(self._h, C.byref(s), C.byref(e), C.byref(t)))
+        return {"spikes": s.value, "syn_events": e.value, "nmda_terms": t.value}

@@ -1,0 +1,241 @@
+"""Synthetic workloads for the Flysim simulation core.
+
+Each generator returns a :class:`Network` (population parameter table plus an
+explicit synapse list) built with numpy's PCG64 from a stated seed.  The same
+network can be written in the reference's drop-in text format
+(``network.conf`` / ``network.pro``, grammar of code/parser.cpp:297-907 and
+code/parser.cpp:1032-1407) whenever the reference format can express it:
+
+* the 1k E/I network is written as 1,000 single-neuron populations with one
+  ``TargetPopulation`` block per synapse and ``Connectivity=1`` so that every
+  link takes the reference's single-synapse path
+  (code/NetSimGnl2_BulPp.cpp:571-695) with a per-synapse efficacy and no
+  ``std::shuffle`` draw;
+* the 20k fly-like net and the 1M net carry per-synapse delays, which the
+  reference format cannot express; they are handed to the engine directly.
+
+Units follow the reference (code/NetSimGnl2_info.cpp:119-127): nF, nS, ms, mV.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+# Receptor ids used throughout the package (order of the reference's
+# RVs/RPs members, code/neuron_affi.h:637-656).
+AMPA, GABA, NMDA, ACH, GCL = 0, 1, 2, 3, 4
+RECEPTOR_NAMES = {AMPA: "AMPA", GABA: "GABA", NMDA: "NMDA", ACH: "Ach", GCL: "GluCl"}
+
+
+@dataclasses.dataclass
+class Receptor:
+    """One ``Receptor:`` block of a population (code/parser.cpp:731-801)."""
+    kind: int
+    tau: float = 5.0          # ms (default, code/parser.cpp:188)
+    tau_xt: float = 2.0       # ms (default, code/parser.cpp:189)
+    rev: float = 0.0          # mV (default 0 when declared, code/parser.cpp:190)
+    freq_ext: float = 0.0     # Hz
+    mean_ext_eff: float = 2.1  # nS (default, code/parser.cpp:192)
+    mean_ext_con: float = 1.0  # (default, code/parser.cpp:193)
+
+
+@dataclasses.dataclass
+class Population:
+    """One ``NeuralPopulation`` block (code/parser.cpp:346-467)."""
+    name: str
+    n: int
+    cm: float = 1.0           # nF   (default, code/parser.cpp:212)
+    taum: float = 2.0         # ms   (default, code/parser.cpp:208)
+    vth: float = -50.0        # mV
+    rest: float = -70.0       # mV
+    reset: float = -50.0      # mV   (VRESET, code/Inc.h:26)
+    refractory_ms: float = 1.8
+    spike_delay_ms: float = 0.0
+    self_connection: bool = False
+    receptors: List[Receptor] = dataclasses.field(default_factory=list)
+
+
+@dataclasses.dataclass
+class Network:
+    """A network as population ranges plus an explicit synapse list.
+
+    ``pre``/``post`` are global neuron ids; populations own contiguous id
+    ranges in declaration order (code/NetSimGnl2_BulPp.cpp:40-47).
+    ``delay`` is an extra per-synapse transmission delay in steps on top of the
+    reference's per-population ``SpikeDly``; 0 reproduces the reference.
+    """
+    name: str
+    pops: List[Population]
+    pre: np.ndarray
+    post: np.ndarray
+    w: np.ndarray          # float32 efficacy in nS
+    receptor: np.ndarray   # uint8 receptor id
+    delay: np.ndarray      # uint8 extra delay in steps
+    dt: float = 0.1
+    t_end_ms: float = 1000.0
+    seed: int = 1
+    notes: Dict[str, object] = dataclasses.field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(sum(p.n for p in self.pops))
+
+    @property
+    def n_syn(self) -> int:
+        return int(self.pre.shape[0])
+
+    def pop_offsets(self) -> np.ndarray:
+        off = np.zeros(len(self.pops) + 1, dtype=np.int64)
+        off[1:] = np.cumsum([p.n for p in self.pops])
+        return off
+
+
+def _f32(x) -> np.ndarray:
+    return np.asarray(x, dtype=np.float32)
+
+
+# ---------------------------------------------------------------------------
+# 1k E/I network (800 E + 200 I), FP64 parity workload
+# ---------------------------------------------------------------------------
+C1_G_EXT = 1.6          # nS per external AMPA event (MeanExtEff, MeanExtCon=1)
+C1_NMDA_RATIO = 0.1     # g_NMDA = ratio * g_AMPA of the same E->X pair
+
+
+def c1_network(seed: int = 1, n_exc: int = 800, n_inh: int = 200, p: float = 0.1,
+               g_lo: float = 0.5, g_hi: float = 1.5, f_ext: float = 2400.0,
+               g_ext: float = C1_G_EXT, nmda_ratio: float = C1_NMDA_RATIO,
+               t_end_ms: float = 1000.0) -> Network:
+    """Plain E/I LIF network: 800 E + 200 I single-neuron populations.
+
+    Cm 0.5 nF, gL 25 nS (Taum 20 ms), Vrest -70, Vth -50, Vreset -55 mV,
+    refractory 2 ms, SpikeDly 1 ms.  AMPA tau 2 ms / Erev 0; NMDA tau 100 ms,
+    tauXT 2 ms / Erev 0 (Mg 1); GABA tau 5 ms / Erev -70 mV.  Bernoulli(p)
+    pairs without self-connections, g ~ U(g_lo, g_hi) nS per pair; E->X pairs
+    carry AMPA (g) and NMDA (nmda_ratio*g); I->X pairs carry GABA (g).
+    External Poisson AMPA at f_ext Hz with g_ext nS per event.
+    """
+    rng = np.random.default_rng(seed)
+    n = n_exc + n_inh
+    conn = rng.random((n, n)) < p
+    np.fill_diagonal(conn, False)
+    g = _f32(rng.uniform(g_lo, g_hi, size=(n, n)))
+    pops = []
+    for i in range(n):
+        exc = i < n_exc
+        pops.append(Population(
+            name=("E%d" % i) if exc else ("I%d" % (i - n_exc)), n=1, cm=0.5,
+            taum=20.0, vth=-50.0, rest=-70.0, reset=-55.0, refractory_ms=2.0,
+            spike_delay_ms=1.0, self_connection=False,
+            receptors=[Receptor(AMPA, tau=2.0, rev=0.0, freq_ext=f_ext,
+                                mean_ext_eff=g_ext, mean_ext_con=1.0),
+                       Receptor(NMDA, tau=100.0, tau_xt=2.0, rev=0.0),
+                       Receptor(GABA, tau=5.0, rev=-70.0)]))
+    pre_l, post_l, w_l, r_l = [], [], [], []
+    for j in range(n):
+        tg = np.nonzero(conn[j])[0]
+        if j < n_exc:
+            pre_l += [np.full(tg.size, j), np.full(tg.size, j)]
+            post_l += [tg, tg]
+            w_l += [g[j, tg], _f32(g[j, tg] * np.float32(nmda_ratio))]
+            r_l += [np.full(tg.size, AMPA), np.full(tg.size, NMDA)]
+        else:
+            pre_l.append(np.full(tg.size, j))
+            post_l.append(tg)
+            w_l.append(g[j, tg])
+            r_l.append(np.full(tg.size, GABA))
+    pre = np.concatenate(pre_l).astype(np.uint32)
+    post = np.concatenate(post_l).astype(np.uint32)
+    w = _f32(np.concatenate(w_l))
+    rec = np.concatenate(r_l).astype(np.uint8)
+    return Network(name="ei1k", pops=pops, pre=pre, post=post, w=w, receptor=rec,
+                   delay=np.zeros(pre.shape[0], np.uint8), dt=0.1, t_end_ms=t_end_ms,
+                   seed=seed, notes={"g_ext_nS": g_ext, "nmda_ratio": nmda_ratio,
+                                     "p": p, "f_ext_Hz": f_ext})
+
+
+# ---------------------------------------------------------------------------
+# Reference text format writers (drop-in API)
+# ---------------------------------------------------------------------------
+def _fmt(x: float) -> str:
+    """A decimal that the reference's ``stof`` reads back as the same float32."""
+    return "%.9g" % float(np.float32(x))
+
+
+def write_conf(net: Network, path: str) -> None:
+    """Write ``network.conf`` in the reference grammar (code/parser.cpp:297-907).
+
+    Only networks whose populations are single neurons (or whose synapse list
+    is expressible as population links with Connectivity=1 and one target) can
+    be written: every synapse becomes one ``TargetPopulation`` block.
+    """
+    if np.any(net.delay != 0):
+        raise ValueError("per-synapse delays are not expressible in network.conf")
+    off = net.pop_offsets()
+    if not all(p.n == 1 for p in net.pops):
+        raise ValueError("write_conf supports single-neuron populations only")
+    order = np.lexsort((np.arange(net.n_syn), net.pre))
+    by_pre: Dict[int, List[int]] = {}
+    for s in order:
+        by_pre.setdefault(int(net.pre[s]), []).append(int(s))
+    out = []
+    for i, pop in enumerate(net.pops):
+        out.append("NeuralPopulation: %s\n" % pop.name)
+        out.append("N=%d\nC=%s\nTaum=%s\nThreshold=%s\nRestPot=%s\nResetPot=%s\n"
+                   "RefractoryPeriod=%s\nSpikeDly=%s\n" % (
+                       pop.n, _fmt(pop.cm), _fmt(pop.taum), _fmt(pop.vth),
+                       _fmt(pop.rest), _fmt(pop.reset), _fmt(pop.refractory_ms),
+                       _fmt(pop.spike_delay_ms)))
+        if pop.self_connection:
+            out.append("SelfConnection=true\n")
+        for r in pop.receptors:
+            out.append("Receptor:%s\nTau=%s\nTauXT=%s\nRevPot=%s\nFreqExt=%s\n"
+                       "MeanExtEff=%s\nMeanExtCon=%s\nEndReceptor\n" % (
+                           RECEPTOR_NAMES[r.kind], _fmt(r.tau), _fmt(r.tau_xt),
+                           _fmt(r.rev), _fmt(r.freq_ext), _fmt(r.mean_ext_eff),
+                           _fmt(r.mean_ext_con)))
+        for s in by_pre.get(int(off[i]), []):
+            out.append("TargetPopulation: %s\nTargetReceptor=%s\nMeanEff=%s\n"
+                       "EndTargetPopulation\n" % (
+                           net.pops[int(net.post[s])].name,
+                           RECEPTOR_NAMES[int(net.receptor[s])], _fmt(net.w[s])))
+        out.append("EndNeuralPopulation\n\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
+def write_pro(path: str, t_end_ms: float, seeds: Sequence[int] = (),
+              spike_file: Optional[str] = "Spikes.dat",
+              rate_file: Optional[str] = None, rate_window_ms: int = 50,
+              rate_print_ms: int = 10, events: Sequence[str] = ()) -> None:
+    """Write ``network.pro`` (code/parser.cpp:911-1407): seeds, events, outputs."""
+    out = []
+    if seeds:
+        out.append("DefineMacro\nRandomSeeds: %s EndRandomSeeds\nEndDefineMacro\n\n"
+                   % " ".join(str(int(s)) for s in seeds))
+    out.extend(events)
+    out.append("EventTime %s\nType=EndTrial\nLabel=End_of_the_trial\nEndEvent\n\n"
+               % _fmt(t_end_ms))
+    out.append("OutControl\n")
+    if spike_file:
+        out.append("FileName:%s\nType=Spike\npopulation:AllPopulation\nEndOutputFile\n"
+                   % spike_file)
+    if rate_file:
+        out.append("FileName:%s\nType=FiringRate\nFiringRateWindow=%d\nPrintStep=%d\n"
+                   "population:AllPopulation\nEndOutputFile\n"
+                   % (rate_file, rate_window_ms, rate_print_ms))
+    out.append("EndOutControl\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
+def read_spikes(path: str) -> np.ndarray:
+    """Read a reference ``Spike`` file ("%.7g time_s nid" lines) as (step?, nid).
+
+    Returns an (n, 2) array of [time_s as float64, nid]."""
+    if not os.path.exists(path) or os.path.getsize(path) == 0:
+        return np.zeros((0, 2))
+    return np.loadtxt(path, ndmin=2)

@@ -1,0 +1,220 @@
+/*
+ * flysim.h -- C ABI of the MI355X-native Flysim simulation core.
+ *
+ * Plain pointers and sizes only; every function returns an int status
+ * (FS_OK = 0, negative on error) and fs_last_error() gives the message of the
+ * last failing call on the calling thread.  No exceptions cross this ABI.
+ *
+ * The reference has no interface between its parser, network builder and
+ * solver: they are one object (NetSimGnl2 : NetSim, EqsGnl2,
+ * code/NetSim.h:367), and the solver is driven by direct calls and public
+ * arrays.  Each entry point below names the reference code it replaces.
+ */
+#ifndef FLYSIM_H
+#define FLYSIM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FS_OK = 0,
+  FS_ERR_ARG = -1,       /* bad argument / unsupported network */
+  FS_ERR_HIP = -2,       /* HIP runtime error */
+  FS_ERR_CAPACITY = -3,  /* caller buffer or device record buffer too small */
+  FS_ERR_IO = -4,        /* file could not be read or written */
+  FS_ERR_PARSE = -5,     /* malformed network.conf / network.pro */
+  FS_ERR_STATE = -6      /* call out of sequence */
+};
+
+/* receptor ids (member order of the reference's RVs, code/neuron_affi.h:648) */
+enum { FS_AMPA = 0, FS_GABA = 1, FS_NMDA = 2, FS_ACH = 3, FS_GCL = 4, FS_NREC = 5 };
+
+/* activity bits, same values as the reference's ActCtr (code/Inc.h:253-280) */
+enum {
+  FS_ACT_AMPA = 1, FS_ACT_GABA = 2, FS_ACT_NMDA = 4, FS_ACT_GAP = 8, FS_ACT_ACH = 32,
+  FS_ACT_GCL = 64, FS_ACT_INJ = 128, FS_ACT_LIF = 256, FS_ACT_LTP = 8192,
+  FS_ACT_STD = 16384, FS_ACT_STP = 32768
+};
+
+enum { FS_FP32 = 0, FS_FP64 = 1 };
+enum { FS_EXT_PHILOX = 0, FS_EXT_REPLAY = 1 };
+
+/* Derived per-population parameters, the values the reference stores per
+ * neuron after SetPp (code/NetSimGnl2_BulPp.cpp:159-278); fs_derive_params
+ * computes them from conf-level values exactly as the reference does. */
+typedef struct fs_pop_params {
+  float vth, el, gl, cm, vreset, df, ef, dvmax;
+  int32_t rfped, spidy;
+  float vrev[5];          /* by receptor id */
+  float dfst[5];          /* gate decay per step by receptor id (NMDA: s) */
+  float dfxt_n, efst_n, as_n, mg_n;
+  float pb[5];            /* external Bernoulli probability per step */
+  float gext[5];          /* external efficacy (nS) */
+  uint32_t act;           /* FS_ACT_* bits of this population */
+  float inj_mean, inj_std;/* Gaussian current injection (nA) */
+  float taum;             /* ms */
+} fs_pop_params;
+
+/* Conf-level population values (NetConf + NeuRep, code/parser.h:9-66). */
+typedef struct fs_pop_conf {
+  float cm, taum, vth, el, vreset;
+  float refractory_ms, spike_delay_ms;
+  int32_t declared[5];    /* receptor declared, by receptor id */
+  float tau[5], tau_xt[5], vrev[5], freq_ext[5], ext_eff[5], ext_con[5];
+} fs_pop_conf;
+
+/* Network handed to fs_create.  Neurons may be ordered arbitrarily; synapses
+ * are a COO list in any order.  Independent network instances (parameter
+ * sweeps) are simply disjoint neuron ranges of one description. */
+typedef struct fs_net_desc {
+  int32_t n_neuron, n_pop;
+  const uint32_t *neuron_pop;     /* [n_neuron] */
+  const fs_pop_params *pops;      /* [n_pop] */
+  int64_t n_syn;
+  const uint32_t *pre, *post;     /* [n_syn] */
+  const float *w;                 /* [n_syn] efficacy, nS */
+  const uint8_t *receptor;        /* [n_syn] FS_AMPA..FS_GCL */
+  const uint8_t *delay;           /* [n_syn] extra delay in steps, NULL = 0 */
+  float dt;                       /* ms */
+  int32_t precision;              /* FS_FP32 | FS_FP64 */
+  int32_t ext_mode;               /* FS_EXT_PHILOX | FS_EXT_REPLAY */
+  int32_t record_spikes;          /* 1: keep per-step spike lists for readback */
+} fs_net_desc;
+
+typedef struct fs_handle fs_handle;
+
+/* Replaces the parameter derivation of NetSimGnl2::SetPp
+ * (code/NetSimGnl2_BulPp.cpp:159-278); float arithmetic as the reference. */
+int fs_derive_params(const fs_pop_conf *conf, float dt, uint32_t act, fs_pop_params *out);
+
+/* Replaces NetSimGnl2::SetPp + PostSetPp (code/NetSimGnl2_BulPp.cpp:27-727):
+ * copies the description, builds CSR-by-pre (linear receptors, push) and
+ * CSR-by-post (NMDA, gather) on the device.  Uses device 0 (the current HIP
+ * device of the calling thread). */
+int fs_create(const fs_net_desc *desc, fs_handle **out);
+
+/* Replaces EqsGnl2::VarInit + the reseeding of NetSimGnl2::Run
+ * (code/EqsGnl2.cpp:339-401, code/NetSimGnl2.cpp:277-323). */
+int fs_reset(fs_handle *h, uint64_t seed);
+
+/* ChangeExtFreq semantics of NetSimGnl2::DoEvt (code/NetSimGnl2.cpp:60-119):
+ * hz == 0 clears the receptor's activity bit (no draws, external gate frozen),
+ * otherwise sets it and pb = (float)(hz*dt*1e-3f). */
+int fs_set_ext_rate(fs_handle *h, int32_t pop, int32_t receptor, float hz);
+
+/* ChangeMembraneNoise / CurrentInjec semantics (code/NetSimGnl2.cpp:120-133). */
+int fs_set_injection(fs_handle *h, int32_t pop, float mean, float std_dev);
+
+/* Overwrite one population's derived parameters (sweeps, gains). */
+int fs_set_pop_params(fs_handle *h, int32_t pop, const fs_pop_params *p);
+
+/* Replay mode: external gate outcomes, one bit per (step, receptor, neuron):
+ * bits[((step - first_step)*5 + receptor)*ceil(N/32) + n/32] >> (n%32) & 1,
+ * covering steps [first_step, first_step + n_steps).  Replaces the mt19937
+ * draws of EqsGnl2::BgInit (code/EqsGnl2.cpp:284-332). */
+int fs_load_ext_trains(fs_handle *h, const uint32_t *bits, int64_t first_step, int64_t n_steps);
+
+/* Replaces NetSimGnl2::SimOne's loop body for n_steps steps
+ * (code/NetSimGnl2.cpp:231-264: Activation + Spike; DoEvt and FileOut stay
+ * with the caller).  Spikes (SPIKE_F) of step s are returned in
+ * ids[offsets[s-s0] .. offsets[s-s0+1]) in ascending neuron order when
+ * offsets/ids are given (offsets has n_steps+1 entries). */
+int fs_run(fs_handle *h, int64_t n_steps, int64_t *offsets, uint32_t *ids, int64_t ids_cap);
+
+/* Number of the next step to run. */
+int fs_step_index(fs_handle *h, int64_t *step);
+
+/* State readback (the reference's public arrays NetVBse/NetVLIF/RVs):
+ * 0 V, 1 refractory count, 2 NState flags, 3..7 synaptic conductance of the
+ * last Spike phase by receptor A,G,N,H,L, 8 linear presynaptic gate,
+ * 9 NMDA presynaptic x, 10 NMDA presynaptic s.  Written as double[N]. */
+int fs_get_state(fs_handle *h, int32_t field, double *out);
+
+/* Per-population spike counts accumulated since fs_reset (uint64[n_pop]). */
+int fs_pop_spike_counts(fs_handle *h, uint64_t *out);
+
+/* Launch schedule (autotuner output): block size of the neuron update,
+ * lanes per row of the NMDA gather, steps per captured graph. */
+typedef struct fs_schedule {
+  int32_t update_block, gather_lanes, push_block, steps_per_graph, use_graph;
+} fs_schedule;
+int fs_get_schedule(fs_handle *h, fs_schedule *s);
+int fs_set_schedule(fs_handle *h, const fs_schedule *s);
+
+/* Kernel timing with HIP events on the engine's stream over the last fs_run
+ * (enabled by fs_enable_timing): average duration in ms of the update,
+ * push and gather kernels, and the number of timed launches. */
+int fs_enable_timing(fs_handle *h, int32_t on);
+int fs_kernel_times(fs_handle *h, double *update_ms, double *push_ms, double *gather_ms,
+                    int64_t *n_timed);
+
+/* Algorithmic counters of the last fs_run: spikes, synaptic events (pushes),
+ * NMDA terms gathered. */
+int fs_run_counters(fs_handle *h, int64_t *spikes, int64_t *syn_events, int64_t *nmda_terms);
+
+int fs_destroy(fs_handle *h);
+const char *fs_last_error(void);
+
+/* ---------------------------------------------------------------------
+ * Drop-in host side: network.conf / network.pro.
+ * ------------------------------------------------------------------- */
+typedef struct fs_conf fs_conf;
+
+typedef struct fs_cli_opts {
+  float dt;               /* -dt (default 0.1) */
+  int32_t threads;        /* -t  (only consumes draws, as in the reference) */
+  int32_t repeats;        /* -rp */
+  int32_t stp, std_, ltp; /* -STP / -STD / -LTP */
+  int32_t udf_seed;       /* -udfsed given */
+  uint32_t udf_seed_value;
+  const char *out_mem, *out_spi, *out_fr;   /* -om / -os / -or, NULL = unset */
+} fs_cli_opts;
+
+/* Replaces Parser::CSta/parse/Comm/DefMacro/parse(pro) + NetSimGnl2::SetPp
+ * (code/main.cpp:233-248). */
+int fs_conf_load(const char *conf_path, const char *pro_path, const fs_cli_opts *opts,
+                 fs_conf **out);
+int fs_conf_free(fs_conf *c);
+/* Engine description of the loaded network (pointers stay owned by c). */
+int fs_conf_desc(fs_conf *c, fs_net_desc *out);
+/* Sizes: neurons, populations, synapses, total synapses as the reference
+ * reports them, events, outputs, repeats. */
+int fs_conf_sizes(fs_conf *c, int64_t out[7]);
+/* Seed of repeat r (code/NetSimGnl2_BulPp.cpp:83-127). */
+int fs_conf_seed(fs_conf *c, int32_t repeat, uint32_t *seed);
+/* External gate outcomes of the reference's mt19937 stream for steps
+ * [0, n_steps) of one repeat, in the fs_load_ext_trains layout
+ * (code/EqsGnl2.cpp:284-332 with the seeding of code/NetSimGnl2.cpp:313-330
+ * and the ActCtr changes of DoEvt). */
+int fs_conf_ext_trains(fs_conf *c, int32_t repeat, int64_t n_steps, uint32_t *bits);
+
+/* Reference internal layout (gates and pull-sum rows in execution order),
+ * for CPU restatements of the reference's summation order. */
+typedef struct fs_ref_layout {
+  int64_t n_gate, n_row;
+  const uint32_t *gate_pre, *gate_post0;
+  const uint8_t *gate_rec, *gate_group;
+  const float *gate_g;
+  const uint32_t *row_post, *row_beg, *term;
+  const uint8_t *row_rec;
+  const fs_pop_conf *pop_conf;    /* [n_pop] */
+  const int32_t *pop_rfped, *pop_spidy;  /* [n_pop] refractory / spike delay in steps */
+  const uint32_t *neuron_pop;     /* [n_neuron] */
+  int64_t n_evt;
+  const float *evt_time_s, *evt_var1, *evt_var2;
+  const int32_t *evt_type, *evt_pop, *evt_sub;
+  uint32_t act_ctr;
+} fs_ref_layout;
+int fs_conf_ref_layout(fs_conf *c, fs_ref_layout *out);
+
+/* The drop-in command line (code/main.cpp:26-256): parses argv like the
+ * reference, runs every repeat on the GPU and writes the Spike / FiringRate
+ * / MemPot output files byte-compatibly. */
+int fs_cli_main(int argc, const char *const *argv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLYSIM_H */
