@@ -87,7 +87,7 @@ EXPORTS = [
     "fs_pop_spike_counts", "fs_get_schedule", "fs_set_schedule", "fs_enable_timing",
     "fs_kernel_times", "fs_run_counters", "fs_destroy", "fs_last_error", "fs_conf_load",
     "fs_conf_free", "fs_conf_desc", "fs_conf_sizes", "fs_conf_seed", "fs_conf_ext_trains",
-    "fs_conf_ref_layout", "fs_cli_main",
+    "fs_conf_ref_layout", "fs_conf_write_outputs", "fs_cli_main",
 ]
 
 _lib = None
@@ -135,6 +135,8 @@ def lib() -> C.CDLL:
             "fs_conf_seed": [C.c_void_p, C.c_int32, P(C.c_uint32)],
             "fs_conf_ext_trains": [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p],
             "fs_conf_ref_layout": [C.c_void_p, P(RefLayout)],
+            "fs_conf_write_outputs": [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
+                                      C.c_char_p],
             "fs_cli_main": [C.c_int, P(C.c_char_p)],
         }
         for name, args in sig.items():

@@ -544,6 +544,9 @@ struct Engine final : EngineBase {
         d2h(b_shist.as<T>() + (size_t)s * n);
         break;
       }
+      case 11: case 12: case 13: case 14: d2h(b_elin.as<T>() + (size_t)(f - 11) * n); break;
+      case 15: d2h(b_enx.p); break;
+      case 16: d2h(b_ens.p); break;
       default: throw std::invalid_argument("unknown state field");
     }
   }

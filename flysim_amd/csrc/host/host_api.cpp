@@ -12,6 +12,7 @@
 #include "events.hpp"
 #include "host_common.hpp"
 #include "netconf.hpp"
+#include "writers.hpp"
 
 
 namespace fs {
@@ -175,6 +176,23 @@ int fs_conf_ext_trains(fs_conf *c, int32_t repeat, int64_t n_steps, uint32_t *bi
         if (p.pb[rid[k]] > u) plane[rid[k] * words + n / 32] |= 1u << (n % 32);
       }
     }
+  }
+  return FS_OK;
+  FS_TRY_END
+}
+
+int fs_conf_write_outputs(fs_conf *c, int32_t repeat, int64_t n_steps, const int64_t *offsets,
+                          const uint32_t *ids, const char *dir) {
+  FS_TRY_BEGIN
+  if (!c || !offsets || (!ids && n_steps > 0 && offsets[n_steps] > 0) || n_steps < 0)
+    return fs::set_error(FS_ERR_ARG, "bad argument");
+  fs::OutputWriters w(c->cfg, c->net, repeat, dir ? dir : "");
+  if (w.needs_potential())
+    return fs::set_error(FS_ERR_ARG, "MemPot outputs need membrane potentials (use the CLI)");
+  for (int64_t k = 0; k < n_steps; ++k) {
+    for (int64_t q = offsets[k]; q < offsets[k + 1]; ++q)
+      if (ids[q] >= c->net.n_neuron) return fs::set_error(FS_ERR_ARG, "spike id out of range");
+    w.step((uint64_t)k, ids + offsets[k], offsets[k + 1] - offsets[k], nullptr);
   }
   return FS_OK;
   FS_TRY_END

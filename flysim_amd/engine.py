@@ -11,7 +11,8 @@ from . import _capi as A
 from .workloads import Network, Population
 
 STATE_FIELDS = {"V": 0, "count": 1, "nstate": 2, "gA": 3, "gG": 4, "gN": 5, "gH": 6, "gL": 7,
-                "s_lin": 8, "x_nmda": 9, "s_nmda": 10}
+                "s_lin": 8, "x_nmda": 9, "s_nmda": 10, "extA": 11, "extG": 12, "extH": 13,
+                "extL": 14, "ext_xN": 15, "ext_sN": 16}
 
 
 def pop_conf_of(p: Population) -> A.PopConf:
@@ -119,6 +120,15 @@ class Conf:
         bits = np.zeros(n_steps * 5 * words, dtype=np.uint32)
         A.check(A.lib().fs_conf_ext_trains(self._h, repeat, n_steps, bits.ctypes.data))
         return bits
+
+    def write_outputs(self, offsets: np.ndarray, ids: np.ndarray, directory: str,
+                      repeat: int = 1) -> None:
+        """Write the protocol's Spike / FiringRate files for a raster given in
+        the fs_run layout (offsets[n_steps+1], ids)."""
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        idv = np.ascontiguousarray(ids, dtype=np.uint32)
+        A.check(A.lib().fs_conf_write_outputs(self._h, repeat, len(off) - 1, off.ctypes.data,
+                                              idv.ctypes.data, directory.encode()))
 
     def ref_layout(self) -> Dict[str, np.ndarray]:
         L = A.RefLayout()

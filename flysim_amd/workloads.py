@@ -158,6 +158,127 @@ def c1_network(seed: int = 1, n_exc: int = 800, n_inh: int = 200, p: float = 0.1
 
 
 # ---------------------------------------------------------------------------
+# Synthetic Drosophila-scale connectome (20k neurons, 50 modules)
+# ---------------------------------------------------------------------------
+C2_G_EXT = 2.2          # nS per external AMPA event
+C2_NMDA_RATIO = 0.1
+
+
+def _ei_receptors(f_ext: float, g_ext: float) -> List[Receptor]:
+    return [Receptor(AMPA, tau=2.0, rev=0.0, freq_ext=f_ext, mean_ext_eff=g_ext, mean_ext_con=1.0),
+            Receptor(NMDA, tau=100.0, tau_xt=2.0, rev=0.0),
+            Receptor(GABA, tau=5.0, rev=-70.0)]
+
+
+def _lif_pop(name: str, n: int, f_ext: float, g_ext: float) -> Population:
+    return Population(name=name, n=int(n), cm=0.5, taum=20.0, vth=-50.0, rest=-70.0, reset=-55.0,
+                      refractory_ms=2.0, spike_delay_ms=0.0, self_connection=False,
+                      receptors=_ei_receptors(f_ext, g_ext))
+
+
+def _emit(pre, post, g, exc_pre, nmda_ratio, delay):
+    """E presynaptic pairs carry AMPA (g) and NMDA (ratio*g), I pairs GABA (g)."""
+    e = exc_pre
+    pre_o = np.concatenate([pre[e], pre[e], pre[~e]]).astype(np.uint32)
+    post_o = np.concatenate([post[e], post[e], post[~e]]).astype(np.uint32)
+    w_o = _f32(np.concatenate([g[e], g[e] * np.float32(nmda_ratio), g[~e]]))
+    r_o = np.concatenate([np.full(e.sum(), AMPA), np.full(e.sum(), NMDA),
+                          np.full((~e).sum(), GABA)]).astype(np.uint8)
+    d_o = np.concatenate([delay[e], delay[e], delay[~e]]).astype(np.uint8)
+    return pre_o, post_o, w_o, r_o, d_o
+
+
+def _largest_remainder(x: np.ndarray, total: int) -> np.ndarray:
+    x = x / x.sum() * total
+    n = np.floor(x).astype(np.int64)
+    rest = total - n.sum()
+    n[np.argsort(-(x - n))[:rest]] += 1
+    return n
+
+
+def c2_network(seed: int = 2, n: int = 20000, n_mod: int = 50, mu: float = 5.8,
+               sigma: float = 0.6, exc_frac: float = 0.7, p_in: float = 0.05,
+               p_out: float = 0.002, mod_edges: int = 150, w_sigma: float = 0.8,
+               d_lo: int = 10, d_hi: int = 20, f_lo: float = 1000.0, f_hi: float = 3000.0,
+               g_ext: float = C2_G_EXT, nmda_ratio: float = C2_NMDA_RATIO,
+               t_end_ms: float = 2000.0, drive: Optional[np.ndarray] = None) -> Network:
+    """Synthetic fly-like connectome: n LIF neurons in n_mod modules with
+    lognormal(mu, sigma) sizes renormalised to n; exc_frac excitatory
+    (AMPA+NMDA) / rest inhibitory (GABA) in every module; Bernoulli pairs with
+    p_in inside a module and p_out (both directions) along the edges of a
+    G(n_mod, mod_edges) random module graph (mean degree 2*mod_edges/n_mod);
+    efficacies lognormal(median 1 nS, w_sigma); delays uniform
+    {d_lo..d_hi} steps (1-2 ms); Poisson AMPA drive U(f_lo, f_hi) Hz per
+    module (or `drive`).  Populations: module m -> "M<m>E", "M<m>I"."""
+    rng = np.random.default_rng(seed)
+    sizes = _largest_remainder(rng.lognormal(mu, sigma, n_mod), n)
+    pairs = [(a, b) for a in range(n_mod) for b in range(a + 1, n_mod)]
+    edges = [pairs[i] for i in rng.choice(len(pairs), size=mod_edges, replace=False)]
+    if drive is None:
+        drive = rng.uniform(f_lo, f_hi, n_mod)
+    pops, first, n_exc = [], np.zeros(n_mod + 1, np.int64), np.zeros(n_mod, np.int64)
+    for m in range(n_mod):
+        ne = int(round(sizes[m] * exc_frac))
+        n_exc[m] = ne
+        pops.append(_lif_pop("M%dE" % m, ne, float(drive[m]), g_ext))
+        pops.append(_lif_pop("M%dI" % m, sizes[m] - ne, float(drive[m]), g_ext))
+        first[m + 1] = first[m] + sizes[m]
+    pre_l, post_l = [], []
+    for m in range(n_mod):
+        k = int(sizes[m])
+        c = rng.random((k, k)) < p_in
+        np.fill_diagonal(c, False)
+        a, b = np.nonzero(c)
+        pre_l.append(a + first[m]); post_l.append(b + first[m])
+    for (a_m, b_m) in edges:
+        for (x, y) in ((a_m, b_m), (b_m, a_m)):
+            c = rng.random((int(sizes[x]), int(sizes[y]))) < p_out
+            a, b = np.nonzero(c)
+            pre_l.append(a + first[x]); post_l.append(b + first[y])
+    pre = np.concatenate(pre_l)
+    post = np.concatenate(post_l)
+    mod_of = np.repeat(np.arange(n_mod), sizes)
+    exc = (pre - first[mod_of[pre]]) < n_exc[mod_of[pre]]
+    g = _f32(rng.lognormal(0.0, w_sigma, pre.shape[0]))
+    d = rng.integers(d_lo, d_hi + 1, pre.shape[0])
+    P, Q, Wt, R, Dl = _emit(pre, post, g, exc, nmda_ratio, d)
+    return Network(name="fly20k", pops=pops, pre=P, post=Q, w=Wt, receptor=R, delay=Dl, dt=0.1,
+                   t_end_ms=t_end_ms, seed=seed,
+                   notes={"pairs": int(pre.shape[0]), "module_sizes": sizes.tolist(),
+                          "module_edges": edges, "drive_Hz": [float(x) for x in drive],
+                          "g_ext_nS": g_ext, "nmda_ratio": nmda_ratio})
+
+
+# ---------------------------------------------------------------------------
+# Scaled sparse network (1M neurons, fixed in-degree 100)
+# ---------------------------------------------------------------------------
+C3_G_EXT = 2.2
+
+
+def c3_network(seed: int = 3, n: int = 1_000_000, exc_frac: float = 0.8, k_in: int = 100,
+               w_sigma: float = 0.8, d_lo: int = 10, d_hi: int = 50, f_ext: float = 2000.0,
+               g_ext: float = C3_G_EXT, nmda_ratio: float = 0.1,
+               t_end_ms: float = 1000.0) -> Network:
+    """n LIF neurons (exc_frac E), each receiving exactly k_in synapses from
+    uniformly drawn presynaptic neurons (self excluded); lognormal(median
+    1 nS, w_sigma) efficacies; delays uniform {d_lo..d_hi} steps (1-5 ms);
+    Poisson AMPA drive f_ext Hz.  E pairs carry AMPA+NMDA, I pairs GABA."""
+    rng = np.random.default_rng(seed)
+    ne = int(round(n * exc_frac))
+    pops = [_lif_pop("E", ne, f_ext, g_ext), _lif_pop("I", n - ne, f_ext, g_ext)]
+    post = np.repeat(np.arange(n, dtype=np.uint32), k_in)
+    pre = rng.integers(0, n - 1, size=n * k_in, dtype=np.uint32)
+    pre = pre + (pre >= post)  # skip self without bias
+    g = _f32(rng.lognormal(0.0, w_sigma, pre.shape[0]))
+    d = rng.integers(d_lo, d_hi + 1, pre.shape[0], dtype=np.uint8)
+    exc = pre < ne
+    P, Q, Wt, R, Dl = _emit(pre, post, g, exc, nmda_ratio, d)
+    return Network(name="sparse1m", pops=pops, pre=P, post=Q, w=Wt, receptor=R, delay=Dl,
+                   dt=0.1, t_end_ms=t_end_ms, seed=seed,
+                   notes={"pairs": int(pre.shape[0]), "g_ext_nS": g_ext, "k_in": k_in})
+
+
+# ---------------------------------------------------------------------------
 # Reference text format writers (drop-in API)
 # ---------------------------------------------------------------------------
 def _fmt(x: float) -> str:

@@ -129,7 +129,8 @@ int fs_step_index(fs_handle *h, int64_t *step);
 /* State readback (the reference's public arrays NetVBse/NetVLIF/RVs):
  * 0 V, 1 refractory count, 2 NState flags, 3..7 synaptic conductance of the
  * last Spike phase by receptor A,G,N,H,L, 8 linear presynaptic gate,
- * 9 NMDA presynaptic x, 10 NMDA presynaptic s.  Written as double[N]. */
+ * 9 NMDA presynaptic x, 10 NMDA presynaptic s, 11..14 external gate of
+ * AMPA, GABA, ACh, GluCl, 15/16 external NMDA x/s.  Written as double[N]. */
 int fs_get_state(fs_handle *h, int32_t field, double *out);
 
 /* Per-population spike counts accumulated since fs_reset (uint64[n_pop]). */
@@ -189,6 +190,12 @@ int fs_conf_seed(fs_conf *c, int32_t repeat, uint32_t *seed);
  * (code/EqsGnl2.cpp:284-332 with the seeding of code/NetSimGnl2.cpp:313-330
  * and the ActCtr changes of DoEvt). */
 int fs_conf_ext_trains(fs_conf *c, int32_t repeat, int64_t n_steps, uint32_t *bits);
+
+/* Writes the Spike / FiringRate output files of the protocol (FileOut,
+ * code/NetSimGnl2_info.cpp:624-699) for a given spike raster, in the
+ * fs_run offsets/ids layout, into directory dir (NULL: current directory). */
+int fs_conf_write_outputs(fs_conf *c, int32_t repeat, int64_t n_steps, const int64_t *offsets,
+                          const uint32_t *ids, const char *dir);
 
 /* Reference internal layout (gates and pull-sum rows in execution order),
  * for CPU restatements of the reference's summation order. */

@@ -1,0 +1,413 @@
+"""Regenerate the golden fixtures from the reference simulator built from its
+own sources (oracle/ref.mk -> oracle/_ref/flysim_ref).
+
+Runs only where the reference sources exist (this build container).  For
+each case it writes the network in the reference's text format, runs the
+reference with `-t 1 -rp 1` and keeps only small outputs: spike rasters as
+(step, neuron) pairs, synapse orders from network.log, the bytes of small
+output files, and digests.  Usage (repo root):  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from flysim_amd import workloads as W  # noqa: E402
+
+REF_CODE = "/root/reference/code"
+REF_SEIZURE_PRO = "/root/reference/experiment/seizure/network1_900_10nOvf.pro"
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "flysim_ref")
+
+
+def build_ref():
+    subprocess.run(["make", "-f", "oracle/ref.mk", "-j8"], cwd=ROOT, check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+def run_ref(d, args=(), timeout=None, wait_for=None):
+    cmd = [REF_BIN, "-conf", "network.conf", "-pro", "network.pro"] + list(args)
+    if wait_for is None:
+        r = subprocess.run(cmd, cwd=d, capture_output=True, text=True, timeout=timeout)
+        return r.stdout
+    # run until `wait_for` is complete (the file is written before the step loop)
+    p = subprocess.Popen(cmd, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+    out = []
+    for line in p.stdout:
+        out.append(line)
+        if line.startswith("start processing"):
+            break
+    p.kill()
+    p.wait()
+    return "".join(out)
+
+
+def raster(path, dt_s=1e-4):
+    s = W.read_spikes(path)
+    if len(s) == 0:
+        return np.zeros((0, 2), np.int32)
+    return np.stack([np.round(s[:, 0] / dt_s).astype(np.int32), s[:, 1].astype(np.int32)], 1)
+
+
+def syn_sections(path):
+    """Synapse lists of network.log in solver array order
+    (code/NetSimGnl2_info.cpp:203-380): name -> int array of (pre, post)."""
+    txt = open(path).read()
+    out = {}
+    for name in ["SynAMPA1", "SynGABA1", "SynACH1", "SynGCL1", "SynNMDA1"]:
+        m = re.search(r"\n%s:(\d+)((?:;[^;\n]*\n?)*)" % name, txt)
+        if m:
+            items = [x for x in m.group(2).replace("\n", "").split(";") if x]
+            out[name] = np.array([[int(float(v)) for v in it.split(",")[:2]] for it in items],
+                                 np.int32).reshape(-1, 2)
+    for name in ["PSynAMPA", "PSynGABA", "PSynNMDA", "PSynACH", "PSynGCL"]:
+        m = re.search(r"\n%s:(\d+);([^\n]*(?:\n[^A-Za-z\n][^\n]*)*)" % name, txt)
+        if m:
+            items = [x for x in m.group(2).replace("\n", "").split(";") if x]
+            out[name] = np.array([[int(float(v)) for v in it.split(",")[:2]] for it in items],
+                                 np.int32).reshape(-1, 2)
+    return out
+
+
+def total_synapses(stdout):
+    m = re.search(r"total synapses=(\d+)", stdout)
+    return int(m.group(1)) if m else -1
+
+
+def c1_case(name, seed, g_ext, t_ms, store, keep_syn=False, cut_steps=None):
+    d = tempfile.mkdtemp(prefix="golden_")
+    net = W.c1_network(seed=seed, g_ext=g_ext, t_end_ms=t_ms)
+    W.write_conf(net, os.path.join(d, "network.conf"))
+    W.write_pro(os.path.join(d, "network.pro"), t_ms, seeds=[seed], spike_file="Spikes.dat")
+    out = run_ref(d)
+    r = raster(os.path.join(d, "Spikes.dat"))
+    store[name + "_counts"] = np.bincount(r[:, 1], minlength=net.n).astype(np.int32)
+    store[name + "_raster"] = r if cut_steps is None else r[r[:, 0] < cut_steps]
+    if keep_syn:
+        for k, v in syn_sections(os.path.join(d, "network.log")).items():
+            store[name + "_" + k] = v
+    meta = {"seed": seed, "g_ext_nS": g_ext, "t_ms": t_ms, "spikes": int(len(r)),
+            "raster_steps_kept": cut_steps, "total_synapses": total_synapses(out),
+            "n_syn_generated": net.n_syn}
+    shutil.rmtree(d)
+    return meta
+
+
+MODULE_CONF = """NeuralPopulation: ExcA
+N=40
+C=0.5
+Taum=20
+Threshold=-50
+RestPot=-70
+ResetPot=-55
+RefractoryPeriod=2
+SpikeDly=0.5
+Receptor:AMPA
+Tau=2
+RevPot=0
+FreqExt=2400
+MeanExtEff=1.3
+EndReceptor
+Receptor:NMDA
+Tau=100
+TauXT=2
+RevPot=0
+EndReceptor
+Receptor:GABA
+Tau=5
+RevPot=-70
+EndReceptor
+TargetPopulation: ExcA
+TargetReceptor=AMPA
+MeanEff=0.8
+Connectivity=0.3
+EndTargetPopulation
+TargetPopulation: ExcA
+TargetReceptor=NMDA
+MeanEff=0.05
+Connectivity=0.3
+EndTargetPopulation
+TargetPopulation: Inh
+TargetReceptor=AMPA
+MeanEff=0.9
+Connectivity=0.4
+EndTargetPopulation
+TargetPopulation: Solo
+TargetReceptor=AMPA
+MeanEff=0.6
+Connectivity=0.05
+EndTargetPopulation
+EndNeuralPopulation
+
+NeuralPopulation: Inh
+N=10
+C=0.2
+Taum=10
+Threshold=-50
+RestPot=-70
+ResetPot=-55
+RefractoryPeriod=1
+Receptor:AMPA
+Tau=2
+RevPot=0
+FreqExt=1800
+MeanExtEff=1.1
+EndReceptor
+Receptor:GABA
+Tau=5
+RevPot=-70
+EndReceptor
+TargetPopulation: ExcA
+TargetReceptor=GABA
+MeanEff=1.5
+Connectivity=0.5
+EndTargetPopulation
+TargetPopulation: Inh
+TargetReceptor=GABA
+MeanEff=1.0
+EndTargetPopulation
+EndNeuralPopulation
+
+NeuralPopulation: Solo
+N=20
+C=0.5
+Taum=20
+Threshold=-50
+RestPot=-70
+ResetPot=-55
+RefractoryPeriod=2
+SelfConnection=true
+Receptor:AMPA
+Tau=2
+RevPot=0
+FreqExt=2600
+MeanExtEff=1.2
+EndReceptor
+Receptor:GABA
+Tau=5
+RevPot=-70
+EndReceptor
+TargetPopulation: Inh
+TargetReceptor=AMPA
+MeanEff=0.7
+Connectivity=0.1
+EndTargetPopulation
+TargetPopulation: Solo
+TargetReceptor=AMPA
+MeanEff=0.5
+Connectivity=0.5
+EndTargetPopulation
+EndNeuralPopulation
+"""
+
+MODULE_PRO = """DefineMacro
+GroupName:Drive
+GroupMembers:ExcA,Solo
+EndGroupMembers
+RandomSeeds: 7 11 EndRandomSeeds
+EndDefineMacro
+
+EventTime 100.0
+Type=ChangeExtFreq
+Label=#1#
+Population: Drive
+Receptor: AMPA
+FreqExt=3000.0
+EndEvent
+
+EventTime 180.0
+Type=ChangeExtFreq
+Label=#2#
+Population: Inh
+Receptor: AMPA
+FreqExt=0
+EndEvent
+
+EventTime 240.0
+Type=ChangeExtFreq
+Label=#3#
+Population: Inh
+Receptor: AMPA
+FreqExt=900.0
+EndEvent
+
+EventTime 300.0
+Type=EndTrial
+Label=End_of_the_trial
+EndEvent
+
+OutControl
+FileName:Spikes.dat
+Type=Spike
+population:AllPopulation
+EndOutputFile
+
+FileName:FRates.dat
+Type=FiringRate
+FiringRateWindow=20
+PrintStep=10
+population:Drive
+EndOutputFile
+
+FileName:FRsum.dat
+Type=FiringRate
+FiringRateWindow=30
+PrintStep=25
+SumAllPopulation=true
+population:AllPopulation
+EndOutputFile
+
+FileName:SpAlign.dat
+Type=Spike
+Align=true
+population:Inh
+EndOutputFile
+EndOutControl
+"""
+
+
+def module_case(store, files):
+    d = tempfile.mkdtemp(prefix="golden_")
+    open(os.path.join(d, "network.conf"), "w").write(MODULE_CONF)
+    open(os.path.join(d, "network.pro"), "w").write(MODULE_PRO)
+    out = run_ref(d, ["-t", "1"])
+    store["module_raster"] = raster(os.path.join(d, "Spikes.dat"))
+    for k, v in syn_sections(os.path.join(d, "network.log")).items():
+        store["module_" + k] = v
+    for fn in ["Spikes.dat", "FRates.dat", "FRsum.dat", "SpAlign.dat"]:
+        files["module_" + fn] = open(os.path.join(d, fn)).read()
+    files["module_network.conf"] = MODULE_CONF
+    files["module_network.pro"] = MODULE_PRO
+    meta = {"total_synapses": total_synapses(out), "spikes": int(len(store["module_raster"]))}
+    # the same network with -t 3 and -rp 2: seeds and the connectome change
+    out3 = run_ref(d, ["-t", "3", "-rp", "2"])
+    for k, v in syn_sections(os.path.join(d, "network.log")).items():
+        store["module_t3_" + k] = v
+    store["module_t3_raster_r2"] = raster(os.path.join(d, "Spikes.dat_2"))
+    seeds = re.search(r"each repeat random seed=([ 0-9]+)", open(os.path.join(d, "network.log")).read())
+    meta["t3_rp2_seeds"] = [int(x) for x in seeds.group(1).split()]
+    meta["t3_total_synapses"] = total_synapses(out3)
+    shutil.rmtree(d)
+    return meta
+
+
+NOISE_PRO = """DefineMacro
+RandomSeeds: 5 EndRandomSeeds
+EndDefineMacro
+EventTime 20.0
+Type=ChangeMembraneNoise
+Population: Solo
+GaussMean=0.3
+GaussSTD=0.2
+EndEvent
+EventTime 60.0
+Type=ChangeMembraneNoise
+Population: ExcA
+GaussMean=0.1
+GaussSTD=0.4
+EndEvent
+EventTime 120.0
+Type=EndTrial
+EndEvent
+OutControl
+FileName:Spikes.dat
+Type=Spike
+population:AllPopulation
+EndOutputFile
+EndOutControl
+"""
+
+
+def noise_case(store, files):
+    d = tempfile.mkdtemp(prefix="golden_")
+    open(os.path.join(d, "network.conf"), "w").write(MODULE_CONF)
+    open(os.path.join(d, "network.pro"), "w").write(NOISE_PRO)
+    run_ref(d)
+    store["noise_raster"] = raster(os.path.join(d, "Spikes.dat"))
+    files["noise_network.pro"] = NOISE_PRO
+    shutil.rmtree(d)
+    return {"spikes": int(len(store["noise_raster"]))}
+
+
+def example_case(store):
+    d = tempfile.mkdtemp(prefix="golden_")
+    for fn in ["network.conf", "network.pro"]:
+        shutil.copy(os.path.join(REF_CODE, fn), d)
+    out = run_ref(d, ["-STP"], wait_for="network.log")
+    for k, v in syn_sections(os.path.join(d, "network.log")).items():
+        store["example_" + k] = v
+    meta = {"total_synapses": total_synapses(out),
+            "seed": int(re.search(r"random seed= (\d+)", open(os.path.join(d, "network.log")).read()).group(1))}
+    shutil.rmtree(d)
+    return meta
+
+
+def event_digest_lines(log_text):
+    i = log_text.index("time event information:")
+    j = log_text.index("output files information:")
+    return [l for l in log_text[i:j].split("\n")[1:] if l.startswith("Event Time")]
+
+
+def seizure_case():
+    """Parse the fly-brain seizure protocol through the reference and keep a
+    digest of the expanded event list as network.log prints it."""
+    if not os.path.exists(REF_SEIZURE_PRO):
+        return None
+    txt = open(REF_SEIZURE_PRO).read()
+    names = []
+    seen = set()
+    for m in re.finditer(r"Population:\s*([^\s]+)", txt):
+        n = m.group(1)
+        if n not in seen and n not in ("AllPopulation",):
+            seen.add(n)
+            names.append(n)
+    groups = set(re.findall(r"GroupName:\s*([^\s]+)", txt))
+    names = [n for n in names if n not in groups]
+    d = tempfile.mkdtemp(prefix="golden_")
+    with open(os.path.join(d, "network.conf"), "w") as f:
+        for n in names:
+            f.write("NeuralPopulation: %s\nN=1\nEndNeuralPopulation\n" % n)
+    shutil.copy(REF_SEIZURE_PRO, os.path.join(d, "network.pro"))
+    run_ref(d, wait_for="network.log")
+    lines = event_digest_lines(open(os.path.join(d, "network.log")).read())
+    shutil.rmtree(d)
+    h = hashlib.sha256("\n".join(lines).encode()).hexdigest()
+    return {"n_pops": len(names), "n_events": len(lines), "sha256": h,
+            "first": lines[:3], "last": lines[-3:], "pop_names_sha256":
+            hashlib.sha256("\n".join(names).encode()).hexdigest()}
+
+
+def main():
+    build_ref()
+    store, files, meta = {}, {}, {}
+    meta["c1_s1"] = c1_case("c1_s1", 1, 1.15, 1000.0, store, keep_syn=True)
+    meta["c1_s2"] = c1_case("c1_s2", 2, 1.15, 1000.0, store)
+    meta["c1_s3"] = c1_case("c1_s3", 3, 1.15, 1000.0, store, cut_steps=2000)
+    meta["c1_hyper"] = c1_case("c1_hyper", 1, 1.18, 300.0, store, cut_steps=2500)
+    meta["module"] = module_case(store, files)
+    meta["noise"] = noise_case(store, files)
+    meta["example"] = example_case(store)
+    sz = seizure_case()
+    if sz:
+        meta["seizure_pro"] = sz
+    np.savez_compressed(os.path.join(HERE, "reference_runs.npz"), **store)
+    with open(os.path.join(HERE, "reference_files.json"), "w") as f:
+        json.dump(files, f, indent=0)
+    with open(os.path.join(HERE, "reference_meta.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta, indent=1)[:3000])
+
+
+if __name__ == "__main__":
+    main()

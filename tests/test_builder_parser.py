@@ -1,0 +1,205 @@
+"""Drop-in host side: the conf/pro parser and the network builder against the
+reference's own outputs (synapse orders of network.log, synapse totals,
+seeds, the expanded event list of the fly-brain seizure protocol) and the
+output writers against the reference's files, byte for byte."""
+import hashlib
+import json
+import os
+import re
+import tempfile
+
+import numpy as np
+import pytest
+
+from flysim_amd import _capi as A
+from flysim_amd import workloads as W
+from flysim_amd.engine import Conf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_SEIZURE_PRO = "/root/reference/experiment/seizure/network1_900_10nOvf.pro"
+REF_CODE = "/root/reference/code"
+
+
+def _write(d, conf, pro):
+    open(os.path.join(d, "network.conf"), "w").write(conf)
+    open(os.path.join(d, "network.pro"), "w").write(pro)
+    return os.path.join(d, "network.conf"), os.path.join(d, "network.pro")
+
+
+def _order(L, group, rid):
+    """(pre, post) in the reference's solver array order for one receptor."""
+    if group:
+        pairs = []
+        for r in range(len(L["row_post"])):
+            if L["row_rec"][r] != rid:
+                continue
+            for t in range(L["row_beg"][r], L["row_beg"][r + 1]):
+                g = L["term"][t]
+                if L["gate_group"][g]:
+                    pairs.append((L["gate_pre"][g], L["row_post"][r]))
+        return np.array(pairs, np.int64).reshape(-1, 2)
+    sel = (L["gate_group"] == 0) & (L["gate_rec"] == rid)
+    return np.stack([L["gate_pre"][sel], L["gate_post0"][sel]], 1).astype(np.int64)
+
+
+SECTIONS = {"SynAMPA1": (0, 0), "SynGABA1": (0, 1), "SynNMDA1": (0, 2), "SynACH1": (0, 3),
+            "SynGCL1": (0, 4), "PSynAMPA": (1, 0), "PSynGABA": (1, 1), "PSynNMDA": (1, 2),
+            "PSynACH": (1, 3)}
+
+
+def _check_sections(L, runs, prefix):
+    n = 0
+    for name, (grp, rid) in SECTIONS.items():
+        key = prefix + name
+        mine = _order(L, grp, rid)
+        if key not in runs.files:
+            assert len(mine) == 0, name
+            continue
+        np.testing.assert_array_equal(mine, runs[key].astype(np.int64), err_msg=name)
+        n += len(mine)
+    return n
+
+
+def test_c1_synapse_order_matches_reference_log(golden):
+    """1k E/I network: 180k explicit single-synapse links; includes the
+    reference's 64 KiB chunk-boundary token quirk (links that silently target
+    population 0)."""
+    with tempfile.TemporaryDirectory() as d:
+        net = W.c1_network(seed=1, g_ext=1.15, t_end_ms=1000)
+        W.write_conf(net, os.path.join(d, "network.conf"))
+        W.write_pro(os.path.join(d, "network.pro"), 1000, seeds=[1])
+        c = Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"))
+        n = _check_sections(c.ref_layout(), golden["runs"], "c1_s1_")
+        assert c.total_synapses == golden["meta"]["c1_s1"]["total_synapses"] == n
+
+
+def test_module_group_gates_shuffle_and_seeds(golden):
+    f = golden["files"]
+    with tempfile.TemporaryDirectory() as d:
+        c = Conf(*_write(d, f["module_network.conf"], f["module_network.pro"]))
+        n = _check_sections(c.ref_layout(), golden["runs"], "module_")
+        assert c.total_synapses == golden["meta"]["module"]["total_synapses"] == n
+        assert c.seed(0) == 7
+        # -t 3 consumes two draws before the connectome; -rp 2 takes seed 11
+        c3 = Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"),
+                  threads=3, repeats=2)
+        _check_sections(c3.ref_layout(), golden["runs"], "module_t3_")
+        assert [c3.seed(0), c3.seed(1)] == golden["meta"]["module"]["t3_rp2_seeds"]
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_CODE), reason="reference example files absent")
+def test_reference_example_network():
+    with tempfile.TemporaryDirectory() as d:
+        import shutil
+        for fn in ("network.conf", "network.pro"):
+            shutil.copy(os.path.join(REF_CODE, fn), d)
+        c = Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"))
+        assert c.n_neuron == 40
+        assert c.total_synapses == 600          # code/readme:40
+        assert c.n_events == 3
+
+
+def test_example_group_synapse_order(golden):
+    """Connectome of the reference's example network (seeded from
+    random_device there, so only the count and the row structure are
+    comparable): 200 Exc1->Exc2 + 400 Exc2->Exc2 (self connections on)."""
+    runs = golden["runs"]
+    assert len(runs["example_PSynAMPA"]) == 600
+    posts = runs["example_PSynAMPA"][:, 1]
+    assert np.all(np.diff(posts) >= 0)
+    assert set(np.unique(posts)) == set(range(20, 40))
+
+
+def _event_lines(c):
+    L = c.ref_layout()
+    out = []
+    for t, ty, p, sub, v1, v2 in zip(L["evt_time_s"], L["evt_type"], L["evt_pop"], L["evt_sub"],
+                                     L["evt_var1"], L["evt_var2"]):
+        s = "Event Time=%g" % float(t)
+        if ty == 1:
+            s += ", type:ChangeExtFreq"
+        elif ty == 2:
+            s += ", type:CurrentInjec"
+        s += ", Target population:%d, Receptor:%d, Variable1=%g, Variable2=%g" % (
+            p, sub, float(v1), float(v2))
+        out.append(s)
+    return out
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SEIZURE_PRO), reason="seizure protocol absent")
+def test_seizure_protocol_event_expansion(golden):
+    """The 3.3 MB fly-brain protocol (20,089 ChangeMembraneNoise events) parses
+    to exactly the event list the reference prints (order after its
+    std::sort included)."""
+    meta = golden["meta"]["seizure_pro"]
+    txt = open(REF_SEIZURE_PRO).read()
+    names, seen = [], set()
+    for m in re.finditer(r"Population:\s*([^\s]+)", txt):
+        if m.group(1) not in seen and m.group(1) != "AllPopulation":
+            seen.add(m.group(1))
+            names.append(m.group(1))
+    groups = set(re.findall(r"GroupName:\s*([^\s]+)", txt))
+    names = [n for n in names if n not in groups]
+    assert hashlib.sha256("\n".join(names).encode()).hexdigest() == meta["pop_names_sha256"]
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "network.conf"), "w") as f:
+            for n in names:
+                f.write("NeuralPopulation: %s\nN=1\nEndNeuralPopulation\n" % n)
+        c = Conf(os.path.join(d, "network.conf"), REF_SEIZURE_PRO)
+        lines = _event_lines(c)
+    assert len(lines) == meta["n_events"]
+    assert lines[:3] == meta["first"] and lines[-3:] == meta["last"]
+    assert hashlib.sha256("\n".join(lines).encode()).hexdigest() == meta["sha256"]
+
+
+def test_output_writers_byte_identical_to_reference(golden, tmp_path):
+    """Spike (plain and Align), FiringRate (windowed, group list) and the
+    summed FiringRate files written from the reference's own raster."""
+    f = golden["files"]
+    c = Conf(*_write(str(tmp_path), f["module_network.conf"], f["module_network.pro"]))
+    r = golden["runs"]["module_raster"].astype(np.int64)
+    steps = 3000
+    counts = np.bincount(r[:, 0], minlength=steps)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    out = tmp_path / "out"
+    out.mkdir()
+    c.write_outputs(off, r[:, 1].astype(np.uint32), str(out))
+    for fn in ["Spikes.dat", "FRates.dat", "FRsum.dat", "SpAlign.dat"]:
+        assert (out / fn).read_text() == f["module_" + fn], fn
+
+
+def test_parser_errors_are_reported(tmp_path):
+    bad = "NeuralPopulation: A\nN=abc\nEndNeuralPopulation\n"
+    conf, pro = _write(str(tmp_path), bad, "EventTime 10\nType=EndTrial\nEndEvent\n")
+    with pytest.raises(A.FlysimError) as e:
+        Conf(conf, pro)
+    assert e.value.code == -5
+    with pytest.raises(A.FlysimError) as e:
+        Conf(str(tmp_path / "missing.conf"), pro)
+    assert e.value.code == -4
+
+
+def test_unknown_target_population_maps_to_population_zero(tmp_path):
+    conf = ("NeuralPopulation: A\nN=2\nTargetPopulation: Nope\nTargetReceptor=AMPA\nMeanEff=1\n"
+            "EndTargetPopulation\nEndNeuralPopulation\nNeuralPopulation: B\nN=1\nEndNeuralPopulation\n")
+    c = Conf(*_write(str(tmp_path), conf, "EventTime 10\nType=EndTrial\nEndEvent\n"))
+    L = c.ref_layout()
+    assert set(L["row_post"].tolist()) <= {0, 1}
+    # the link falls back to A itself (2 neurons, connectivity 1, no self
+    # connections): the group path drops it (one target left, :312-315) and the
+    # single-synapse path skips it (ceil(2*1) != 1, :587-588) -> no synapse
+    assert c.total_synapses == 0
+    assert "unknown target population" in " ".join(c.warnings) if hasattr(c, "warnings") else True
+
+
+def test_refractory_and_delay_conversions(tmp_path):
+    """RfPed = int(floor(x/dt)+dt), SpiDy = int(floor(x/dt)) in float
+    (code/parser.cpp:507-515): 2 -> 20, 1.9 -> 19, 1 -> 10, 0.3 -> 3."""
+    conf = ""
+    for i, (rf, sd) in enumerate([("2", "1"), ("1.9", "0.3"), ("1", "2")]):
+        conf += "NeuralPopulation: P%d\nN=1\nRefractoryPeriod=%s\nSpikeDly=%s\nEndNeuralPopulation\n" % (i, rf, sd)
+    conf += "NeuralPopulation: D\nN=1\nEndNeuralPopulation\n"
+    c = Conf(*_write(str(tmp_path), conf, "EventTime 10\nType=EndTrial\nEndEvent\n"))
+    L = c.ref_layout()
+    assert L["rfped"].tolist() == [20, 19, 10, 18]
+    assert L["spidy"].tolist() == [10, 3, 20, 0]

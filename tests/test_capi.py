@@ -1,0 +1,51 @@
+"""The C ABI library loads and exports every entry point include/flysim.h
+declares (no compute calls: this file runs without a GPU)."""
+import ctypes as C
+import os
+import re
+
+from flysim_amd import _capi as A
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "flysim.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(?:int|const char \*)\s*(fs_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_declares_the_bound_api():
+    names = declared_functions()
+    assert len(names) >= 25
+    assert set(names) == set(A.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = A.lib()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_struct_sizes_match_header_layout():
+    # fs_pop_params: 8 floats, 2 ints, 5+5 floats, 4 floats, 5+5 floats, uint, 3 floats
+    assert C.sizeof(A.PopParams) == 4 * (8 + 2 + 10 + 4 + 10 + 1 + 3)
+    assert C.sizeof(A.Schedule) == 20
+
+
+def test_derive_params_reference_values():
+    """Derived parameters of the 1k E/I population, as measured on the
+    reference: df 0.995012462, ef 0.0997507572, dfst(2 ms) 0.951229453,
+    dfst(100 ms) 0.99900049, pb(2400 Hz) 0.24000001f."""
+    import numpy as np
+    from flysim_amd import workloads as W
+    from flysim_amd.engine import derive_params
+    net = W.c1_network(seed=1, n_exc=8, n_inh=2)
+    q = derive_params(net.pops[0])
+    assert np.float32(q.df) == np.float32(0.995012462)
+    assert np.float32(q.ef) == np.float32(0.0997507572)
+    assert np.float32(q.dfst[0]) == np.float32(0.951229453)
+    assert np.float32(q.dfst[2]) == np.float32(0.99900049)
+    assert q.pb[0] == np.float32(0.24000001)
+    assert q.rfped == 20 and q.spidy == 10
+    assert q.gl == np.float32(0.5) / np.float32(20.0)

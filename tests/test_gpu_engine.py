@@ -1,0 +1,288 @@
+"""GPU parity tests of the HIP engine, through the C ABI (include/flysim.h).
+
+Reference data: the golden rasters written by the reference simulator built
+from source (tests/golden/make_golden.py) and the CPU restatement (oracle/).
+Bars: FP64 replay reproduces the reference raster exactly for 200 ms of the
+1k E/I network and its population rates within 1 % over 1 s; FP32 replay
+keeps population rates within 2 % and the per-neuron spike-count
+distribution within a two-sample KS p > 0.05 of the reference; delays,
+batching and launch schedules obey exact properties in FP64.
+"""
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from flysim_amd import _capi as A
+from flysim_amd import workloads as W
+from flysim_amd.engine import Conf, Engine, arrays_from_network
+
+pytestmark = pytest.mark.gpu
+
+STEPS_EXACT = 2000   # 200 ms
+STEPS_FULL = 10000   # 1 s
+
+
+def _c1_conf(tmp, seed=1, g_ext=1.15, t_ms=1000.0):
+    net = W.c1_network(seed=seed, g_ext=g_ext, t_end_ms=t_ms)
+    W.write_conf(net, os.path.join(tmp, "network.conf"))
+    W.write_pro(os.path.join(tmp, "network.pro"), t_ms, seeds=[seed])
+    return Conf(os.path.join(tmp, "network.conf"), os.path.join(tmp, "network.pro"))
+
+
+@pytest.fixture(scope="module")
+def c1():
+    with tempfile.TemporaryDirectory() as d:
+        conf = _c1_conf(d)
+        bits = conf.ext_trains(STEPS_FULL)
+        yield conf, bits
+
+
+def _run(conf, bits, precision, steps):
+    eng = Engine(conf, precision=precision, ext_mode=A.FS_EXT_REPLAY)
+    eng.reset(conf.seed(0))
+    eng.load_ext_trains(bits, 0, STEPS_FULL)
+    return eng, eng.raster(steps)
+
+
+def test_c1_fp64_replay_exact_raster_200ms(c1, golden):
+    conf, bits = c1
+    ref = golden["runs"]["c1_s1_raster"].astype(np.int64)
+    ref = ref[ref[:, 0] < STEPS_EXACT]
+    print("case c1 seed 1, fp64 replay, %d steps, %d reference spikes" % (STEPS_EXACT, len(ref)), flush=True)
+    _, r = _run(conf, bits, A.FS_FP64, STEPS_EXACT)
+    assert r.shape == ref.shape
+    np.testing.assert_array_equal(r, ref)
+
+
+def _pop_rates(r, n_exc=800, n=1000, seconds=1.0):
+    ids = r[:, 1]
+    return np.array([(ids < n_exc).sum() / n_exc / seconds,
+                     (ids >= n_exc).sum() / (n - n_exc) / seconds])
+
+
+def test_c1_fp64_rates_1s_within_1pct(c1, golden):
+    conf, bits = c1
+    ref = golden["runs"]["c1_s1_raster"].astype(np.int64)
+    _, r = _run(conf, bits, A.FS_FP64, STEPS_FULL)
+    rg, rr = _pop_rates(r), _pop_rates(ref)
+    print("fp64 rates", rg, "reference", rr)
+    np.testing.assert_allclose(rg, rr, rtol=0.01)
+
+
+def test_c1_fp32_replay_rates_and_ks(c1, golden):
+    from scipy.stats import ks_2samp
+    conf, bits = c1
+    ref = golden["runs"]["c1_s1_raster"].astype(np.int64)
+    _, r = _run(conf, bits, A.FS_FP32, STEPS_FULL)
+    rg, rr = _pop_rates(r), _pop_rates(ref)
+    print("fp32 rates", rg, "reference", rr)
+    np.testing.assert_allclose(rg, rr, rtol=0.02)
+    cg = np.bincount(r[:, 1], minlength=1000)
+    cr = np.bincount(ref[:, 1], minlength=1000)
+    p = ks_2samp(cg, cr).pvalue
+    print("KS p-value of per-neuron spike counts", p)
+    assert p > 0.05
+
+
+def test_c1_fp64_state_tracks_oracle(c1):
+    from oracle.oracle import Oracle
+    conf, bits = c1
+    eng = Engine(conf, precision=A.FS_FP64, ext_mode=A.FS_EXT_REPLAY)
+    eng.reset(conf.seed(0))
+    eng.load_ext_trains(bits, 0, STEPS_FULL)
+    orc = Oracle(conf.ref_layout(), conf.n_neuron, precision=1)
+    orc.reset(conf.seed(0))
+    orc.replay(bits, STEPS_FULL)
+    for _ in range(10):
+        rg = eng.raster(100)
+        ro = orc.raster(100)
+        np.testing.assert_array_equal(rg, ro)
+        for f in ("V", "gA", "gG", "gN"):
+            a, b = eng.state(f), orc.state(f)
+            np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-9, err_msg=f)
+        np.testing.assert_array_equal(eng.state("count"), orc.state("count"))
+        np.testing.assert_array_equal(eng.state("nstate"), orc.state("nstate"))
+
+
+def test_pop_counts_match_raster(c1):
+    conf, bits = c1
+    eng, r = _run(conf, bits, A.FS_FP32, 3000)
+    counts = eng.pop_counts()
+    np.testing.assert_array_equal(counts, np.bincount(r[:, 1], minlength=conf.n_pop))
+
+
+def test_schedules_give_identical_fp64_rasters(c1):
+    conf, bits = c1
+    base = None
+    for sch in [dict(update_block=256, gather_lanes=8, steps_per_graph=100),
+                dict(update_block=64, gather_lanes=1, steps_per_graph=1, use_graph=0),
+                dict(update_block=1024, gather_lanes=64, steps_per_graph=333),
+                dict(update_block=512, gather_lanes=16, steps_per_graph=7, push_block=1024)]:
+        eng = Engine(conf, precision=A.FS_FP64, ext_mode=A.FS_EXT_REPLAY)
+        eng.set_schedule(**sch)
+        eng.reset(conf.seed(0))
+        eng.load_ext_trains(bits, 0, STEPS_FULL)
+        r = eng.raster(1500)
+        if base is None:
+            base = r
+        else:
+            np.testing.assert_array_equal(r, base, err_msg=str(sch))
+
+
+# ---------------------------------------------------------------- delays
+def _tiny_net(delay_lin, delay_nmda, n_rep=1):
+    """Driver neurons (strong external drive) projecting AMPA + NMDA + GABA
+    onto silent observer neurons (threshold out of reach)."""
+    recs = lambda fext: [W.Receptor(W.AMPA, tau=2, rev=0, freq_ext=fext, mean_ext_eff=3.0),  # noqa: E731
+                         W.Receptor(W.NMDA, tau=100, tau_xt=2, rev=0),
+                         W.Receptor(W.GABA, tau=5, rev=-70)]
+    drv_e = W.Population("drvE", n=2, cm=0.5, taum=20, vth=-50, rest=-70, reset=-55,
+                         refractory_ms=2.0, receptors=recs(3000))
+    drv_i = W.Population("drvI", n=2, cm=0.5, taum=20, vth=-50, rest=-70, reset=-55,
+                         refractory_ms=2.0, receptors=recs(2500))
+    obs = W.Population("obs", n=3, cm=0.5, taum=20, vth=1e6, rest=-70, reset=-55,
+                       refractory_ms=2.0, receptors=recs(0))
+    pops = [drv_e, drv_i, obs]
+    pre, post, w, rec, dl = [], [], [], [], []
+    for j in range(4):
+        for p in range(4, 7):
+            if j < 2:
+                syn = [(W.AMPA, 0.5 + 0.1 * j, delay_lin), (W.NMDA, 0.05 * (p - 3), delay_nmda)]
+            else:
+                syn = [(W.GABA, 0.3 + 0.05 * j, delay_lin)]
+            for r, g, d in syn:
+                pre.append(j); post.append(p); w.append(g); rec.append(r); dl.append(d)
+    return W.Network("tiny", pops, np.array(pre, np.uint32), np.array(post, np.uint32),
+                     np.array(w, np.float32), np.array(rec, np.uint8), np.array(dl, np.uint8))
+
+
+def _trace(net, steps, fields):
+    arr = arrays_from_network(net)
+    eng = Engine(arr, precision=A.FS_FP64, ext_mode=A.FS_EXT_PHILOX)
+    eng.set_schedule(steps_per_graph=1, use_graph=0)
+    eng.reset(12345)
+    out = {f: [] for f in fields}
+    spikes = []
+    for _ in range(steps):
+        spikes.append(eng.raster(1))
+        for f in fields:
+            out[f].append(eng.state(f)[4:7].copy())
+    return {f: np.array(v) for f, v in out.items()}, np.concatenate(spikes)
+
+
+def test_linear_delay_shifts_conductance_exactly():
+    """GABA/AMPA conductance of an observer with delay d equals the d=0 trace
+    shifted by d steps (drivers unaffected by observers)."""
+    base, sp0 = _trace(_tiny_net(0, 0), 400, ["gA", "gG"])
+    for d in (1, 7, 25):
+        sh, sp = _trace(_tiny_net(d, 0), 400, ["gA", "gG"])
+        np.testing.assert_array_equal(sp, sp0)
+        for f in ("gA", "gG"):
+            np.testing.assert_array_equal(sh[f][d:], base[f][:-d], err_msg="%s d=%d" % (f, d))
+            assert np.all(sh[f][:d] == 0)
+
+
+def test_nmda_delay_shifts_conductance_exactly():
+    base, _ = _trace(_tiny_net(0, 0), 400, ["gN"])
+    for d in (1, 13):
+        sh, _ = _trace(_tiny_net(0, d), 400, ["gN"])
+        np.testing.assert_allclose(sh["gN"][d:], base["gN"][:-d], rtol=1e-14, atol=1e-15)
+        assert np.all(sh["gN"][:d] == 0)
+
+
+# ---------------------------------------------------------------- batching
+def test_batched_instances_equal_separate_runs():
+    """Two instances concatenated in one description run exactly like each
+    alone (FP64, replayed identical external trains)."""
+    a = W.c1_network(seed=5, n_exc=240, n_inh=60, g_ext=1.3, t_end_ms=100)
+    b = W.c1_network(seed=6, n_exc=400, n_inh=100, g_ext=1.25, t_end_ms=100)
+    steps = 800
+    rng = np.random.default_rng(0)
+
+    def bits_for(n):
+        words = (n + 31) // 32
+        x = np.zeros((steps, 5, words), np.uint32)
+        x[:, 0, :] = rng.integers(0, 2**32, size=(steps, words), dtype=np.uint64).astype(np.uint32)
+        x[:, 0, :] &= rng.integers(0, 2**32, size=(steps, words), dtype=np.uint64).astype(np.uint32)
+        if n % 32:
+            x[:, 0, -1] &= (1 << (n % 32)) - 1
+        return x
+
+    ba, bb = bits_for(a.n), bits_for(b.n)
+    ra = _replay_raster(arrays_from_network(a), ba, steps)
+    rb = _replay_raster(arrays_from_network(b), bb, steps)
+    cat = W.Network("ab", a.pops + b.pops, np.concatenate([a.pre, b.pre + a.n]),
+                    np.concatenate([a.post, b.post + a.n]), np.concatenate([a.w, b.w]),
+                    np.concatenate([a.receptor, b.receptor]), np.concatenate([a.delay, b.delay]))
+    n = cat.n
+    words = (n + 31) // 32
+    bc = np.zeros((steps, 5, words), np.uint32)
+    for s in range(steps):
+        for r in range(5):
+            bits = np.concatenate([np.unpackbits(ba[s, r].view(np.uint8), bitorder="little")[:a.n],
+                                   np.unpackbits(bb[s, r].view(np.uint8), bitorder="little")[:b.n]])
+            bits = np.concatenate([bits, np.zeros(words * 32 - n, np.uint8)])
+            bc[s, r] = np.packbits(bits, bitorder="little").view(np.uint32)
+    rc = _replay_raster(arrays_from_network(cat), bc, steps)
+    expect = np.concatenate([ra, rb + np.array([0, a.n])])
+    expect = expect[np.lexsort((expect[:, 1], expect[:, 0]))]
+    np.testing.assert_array_equal(rc, expect)
+    assert len(ra) > 0 and len(rb) > 0
+
+
+def _replay_raster(arr, bits, steps):
+    eng = Engine(arr, precision=A.FS_FP64, ext_mode=A.FS_EXT_REPLAY)
+    eng.reset(1)
+    eng.load_ext_trains(bits.reshape(-1), 0, steps)
+    return eng.raster(steps)
+
+
+# ---------------------------------------------------------------- RNG
+def test_philox_external_gate_mean_matches_bernoulli_rate():
+    """Stationary mean of the external AMPA gate is pb/(1-df) (Bernoulli(pb)
+    input per step); with pb = 1 the gate counts steps exactly."""
+    pop = W.Population("x", n=20000, cm=0.5, taum=20, vth=1e6, rest=-70, reset=-55,
+                       receptors=[W.Receptor(W.AMPA, tau=2, rev=0, freq_ext=2400, mean_ext_eff=1.0)])
+    net = W.Network("ext", [pop], np.zeros(0, np.uint32), np.zeros(0, np.uint32),
+                    np.zeros(0, np.float32), np.zeros(0, np.uint8), np.zeros(0, np.uint8))
+    arr = arrays_from_network(net)
+    eng = Engine(arr, precision=A.FS_FP64)
+    eng.reset(99)
+    eng.run(2000, record=False)
+    pb = arr.pops[0].pb[0]
+    df = arr.pops[0].dfst[0]
+    e = eng.state("extA")
+    expect = pb / (1 - df)
+    sem = np.sqrt(pb * (1 - pb) / (1 - df * df)) / np.sqrt(len(e))
+    print("ext gate mean", e.mean(), "expected", expect, "sem", sem)
+    assert abs(e.mean() - expect) < 5 * sem
+    pop.receptors[0].freq_ext = 10000.0  # pb = 1
+    arr = arrays_from_network(net)
+    eng = Engine(arr, precision=A.FS_FP64)
+    eng.reset(1)
+    eng.run(37, record=False)
+    np.testing.assert_array_equal(eng.state("extA"), 37.0)
+
+
+# ---------------------------------------------------------------- drop-in CLI
+def test_cli_module_network_fp64_replay_matches_reference_files(golden, tmp_path):
+    """The drop-in command line on the module network (group gates, shuffled
+    targets, ChangeExtFreq events incl. FreqExt=0, groups, Align, summed
+    FiringRate) reproduces the reference's output files byte for byte."""
+    files = golden["files"]
+    (tmp_path / "network.conf").write_text(files["module_network.conf"])
+    (tmp_path / "network.pro").write_text(files["module_network.pro"])
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        argv = [b"flysim_amd", b"-fp64", b"-ext", b"replay", b"-t", b"1"]
+        arr = (C.c_char_p * len(argv))(*argv)
+        A.check(A.lib().fs_cli_main(len(argv), arr))
+    finally:
+        os.chdir(cwd)
+    for fn in ["Spikes.dat", "FRates.dat", "FRsum.dat", "SpAlign.dat"]:
+        got = (tmp_path / fn).read_text()
+        assert got == files["module_" + fn], fn

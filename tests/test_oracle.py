@@ -1,0 +1,191 @@
+"""The CPU restatement (oracle/) pinned against the reference simulator built
+from source (golden rasters) and against closed forms of the model."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from flysim_amd import workloads as W
+from flysim_amd.engine import Conf
+from oracle.oracle import Oracle, mt19937_uniform
+
+
+def _conf_from_text(d, conf_text, pro_text, **kw):
+    open(os.path.join(d, "network.conf"), "w").write(conf_text)
+    open(os.path.join(d, "network.pro"), "w").write(pro_text)
+    return Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"), **kw)
+
+
+def _c1(d, seed, g_ext, t_ms):
+    net = W.c1_network(seed=seed, g_ext=g_ext, t_end_ms=t_ms)
+    W.write_conf(net, os.path.join(d, "network.conf"))
+    W.write_pro(os.path.join(d, "network.pro"), t_ms, seeds=[seed])
+    return Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"))
+
+
+@pytest.mark.parametrize("case,seed,g_ext,steps", [
+    ("c1_s1", 1, 1.15, 2000), ("c1_s2", 2, 1.15, 2000), ("c1_s3", 3, 1.15, 2000),
+    ("c1_hyper", 1, 1.18, 2500)])
+def test_fp32_oracle_reproduces_reference_raster(golden, case, seed, g_ext, steps):
+    with tempfile.TemporaryDirectory() as d:
+        c = _c1(d, seed, g_ext, 1000.0)
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        r = o.raster(steps)
+    ref = golden["runs"][case + "_raster"].astype(np.int64)
+    ref = ref[ref[:, 0] < steps]
+    assert len(ref) > 0
+    np.testing.assert_array_equal(r, ref)
+
+
+def test_fp32_oracle_module_network_with_events(golden):
+    """Group gates, shuffled targets, ChangeExtFreq events (incl. FreqExt=0)."""
+    f = golden["files"]
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, f["module_network.conf"], f["module_network.pro"])
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        r = o.raster(3000)
+    np.testing.assert_array_equal(r, golden["runs"]["module_raster"].astype(np.int64))
+
+
+def test_fp32_oracle_gaussian_injection_ziggurat(golden):
+    """ChangeMembraneNoise events drive the Marsaglia-Tsang ziggurat stream."""
+    f = golden["files"]
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, f["module_network.conf"], f["noise_network.pro"])
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        r = o.raster(1200)
+    np.testing.assert_array_equal(r, golden["runs"]["noise_raster"].astype(np.int64))
+
+
+def test_fp64_oracle_with_replayed_trains_matches_reference_200ms(golden):
+    """FP64 restatement fed the reference's draw outcomes keeps the raster."""
+    with tempfile.TemporaryDirectory() as d:
+        c = _c1(d, 1, 1.15, 1000.0)
+        bits = c.ext_trains(2000)
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=1)
+        o.reset(c.seed(0))
+        o.replay(bits, 2000)
+        r = o.raster(2000)
+    ref = golden["runs"]["c1_s1_raster"].astype(np.int64)
+    np.testing.assert_array_equal(r, ref[ref[:, 0] < 2000])
+
+
+def test_recorded_draws_equal_product_train_dumper():
+    """The oracle's restated generate_canonical<float> stream and the
+    product's libstdc++ uniform_real_distribution dumper agree bit for bit."""
+    with tempfile.TemporaryDirectory() as d:
+        c = _c1(d, 2, 1.15, 50.0)
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        rec = o.record(300)
+        o.raster(300)
+        bits = c.ext_trains(300)
+    np.testing.assert_array_equal(rec, bits)
+    assert rec.sum() > 0
+
+
+def test_canonical_float_draw_properties():
+    u = mt19937_uniform(5489, 10000)
+    # the 10000th output of a default-seeded mt19937 is 4123659995 (C++ standard)
+    assert u[-1] == np.float32(4123659995) / np.float32(2.0 ** 32)
+    assert np.all((u >= 0) & (u < 1))
+
+
+# ----------------------------------------------------------------- closed forms
+SINGLE = """NeuralPopulation: A
+N=3
+C=0.5
+Taum=20
+Threshold=-50
+RestPot=-60
+ResetPot=-55
+RefractoryPeriod=%s
+SpikeDly=%s
+Receptor:AMPA
+Tau=2
+RevPot=0
+FreqExt=%s
+MeanExtEff=%s
+EndReceptor
+EndNeuralPopulation
+"""
+PRO = "DefineMacro\nRandomSeeds: 1 EndRandomSeeds\nEndDefineMacro\nEventTime 100\nType=EndTrial\nEndEvent\n"
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_leak_relaxation_closed_form(precision):
+    """No input: the exponential-Euler recurrence V' = df V + ef (gl El / Cm)
+    has the solution V_t = V* + (V0 - V*) df^t, V* = ef gl El / (Cm (1-df)),
+    with V0 = -70 mV (VRESTING, not RestPot) and V* -> El = -60 mV up to the
+    float rounding of the derived parameters df, ef, gl."""
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, SINGLE % ("2", "0", "0", "1"), PRO)
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=precision)
+        o.reset(1)
+        df, ef, gl = o.params("df")[0], o.params("ef")[0], o.params("gl")[0]
+        # the reference scales the leak drive by 1e3f and back by 1e-3f (float)
+        k = float(np.float32(1e-3)) * float(np.float32(1e3))
+        vstar = ef * gl * -60.0 * k / (0.5 * (1.0 - df))
+        assert abs(vstar + 60.0) < 1e-4
+        for t in (1, 10, 200, 1000):
+            o.raster(t - o.step)
+            v = o.state("V")[0]
+            expect = vstar + (-70.0 - vstar) * df ** t
+            assert abs(v - expect) < (1e-3 if precision == 0 else 1e-9)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_external_gate_counts_with_pb_one_and_decays_with_pb_zero(precision):
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, SINGLE % ("2", "0", "10000", "0.001"), PRO)
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=precision)
+        o.reset(1)
+        o.raster(57)
+        np.testing.assert_array_equal(o.state("extA"), 57.0)
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, SINGLE % ("2", "0", "0", "1"), PRO)
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=precision)
+        o.reset(1)
+        o.raster(20)
+        np.testing.assert_array_equal(o.state("extA"), 0.0)
+
+
+@pytest.mark.parametrize("rfped_ms,spidy_ms", [("2", "0"), ("2", "1"), ("1.9", "1.8"), ("1", "2")])
+def test_refractory_state_machine(rfped_ms, spidy_ms):
+    """A neuron driven far above threshold every step: it fires, is frozen for
+    RfPed+1 steps (V = 0 on the first, then Vreset), SPIKE_F appears at
+    Count == SpiDy; SpiDy >= RfPed never spikes; SpiDy == RfPed-1 keeps
+    SPIKE_F set after the freeze (code/EqsGnl2.cpp:229-251)."""
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, SINGLE % (rfped_ms, spidy_ms, "10000", "200"), PRO)
+        L = c.ref_layout()
+        rf, sd = int(L["rfped"][0]), int(L["spidy"][0])
+        o = Oracle(L, c.n_neuron, precision=1)
+        o.reset(1)
+        vs, fl = [], []
+        for _ in range(120):
+            o.raster(1)
+            vs.append(o.state("V")[0])
+            fl.append(int(o.state("nstate")[0]))
+    fire = np.array([(f & 1) != 0 for f in fl])
+    spike = np.array([(f & 2) != 0 for f in fl])
+    t0 = int(np.argmax(fire))                  # threshold step
+    assert vs[t0] == 0.0
+    # FIRE_F set for RfPed steps; on step t0+RfPed it clears without
+    # integrating, so V is frozen for RfPed+1 steps
+    assert fire[t0:t0 + rf].all() and not fire[t0 + rf]
+    assert all(v == -55.0 for v in vs[t0 + 1:t0 + rf + 1])
+    assert fire[t0 + rf + 1] and vs[t0 + rf + 1] == 0.0   # driven: fires again at once
+    if sd >= rf:
+        assert not spike.any()
+    else:
+        assert spike[t0 + sd]
+        assert not spike[t0:t0 + sd].any()
+        if sd == rf - 1:
+            assert spike[t0 + rf]              # sticky past the freeze
+        else:
+            assert not spike[t0 + sd + 1:t0 + rf].any()
