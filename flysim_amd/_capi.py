@@ -56,7 +56,7 @@ class NetDesc(C.Structure):
 class Schedule(C.Structure):
     _fields_ = [("update_block", C.c_int32), ("gather_lanes", C.c_int32),
                 ("push_block", C.c_int32), ("steps_per_graph", C.c_int32),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("fused", C.c_int32)]
 
 
 class CliOpts(C.Structure):
@@ -126,7 +126,7 @@ def lib() -> C.CDLL:
             "fs_enable_timing": [C.c_void_p, C.c_int32],
             "fs_kernel_times": [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double),
                                 P(C.c_int64)],
-            "fs_run_counters": [C.c_void_p, P(C.c_int64), P(C.c_int64), P(C.c_int64)],
+            "fs_run_counters": [C.c_void_p, C.c_void_p],
             "fs_destroy": [C.c_void_p],
             "fs_conf_load": [C.c_char_p, C.c_char_p, P(CliOpts), P(C.c_void_p)],
             "fs_conf_free": [C.c_void_p],

@@ -59,7 +59,7 @@ struct EngineBase {
   virtual void get_state(int32_t f, double *out) = 0;
   virtual void pop_counts(uint64_t *out) = 0;
   virtual void times(double *u, double *p, double *g, int64_t *n) = 0;
-  virtual void counters(int64_t *s, int64_t *e, int64_t *t) = 0;
+  virtual void counters(int64_t *out) = 0;
   fs_schedule sched{};
   bool timing = false;
   int64_t step = 0;
@@ -78,7 +78,9 @@ struct Engine final : EngineBase {
   // device buffers
   DevBuf b_npop, b_pops, b_V, b_cs, b_G, b_ring, b_spre, b_xn, b_shist, b_gn, b_elin, b_enx,
       b_ens, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
-      b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp;
+      b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_done;
+  bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
+  bool fused() const { return fused_ok && sched.fused; }
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -161,7 +163,7 @@ struct Engine final : EngineBase {
       if (d.neuron_pop[i] >= (uint32_t)P) throw std::invalid_argument("neuron_pop out of range");
     // ---------------- synapses: classify, validate, CSR by pre / by post
     std::vector<uint64_t> cnt_p(N + 1, 0), cnt_n(N + 1, 0);
-    int32_t maxd_p = 0, maxd_n = 0;
+    int32_t maxd_p = 0, maxd_n = 0, mind_all = 1 << 30;
     int32_t lin_mask = 0;
     // presynaptic gate parameters per population (targets' receptor params)
     std::vector<int> pre_set(P, 0), pre_set_n(P, 0);
@@ -175,6 +177,7 @@ struct Engine final : EngineBase {
       if (r > FS_GCL) throw std::invalid_argument("synapse receptor id out of range");
       const uint32_t pp = d.neuron_pop[a];
       const fs_pop_params &q = hpop[d.neuron_pop[b]];
+      mind_all = std::min(mind_all, dl);
       if (r == FS_NMDA) {
         if (dl > 255) throw std::invalid_argument("NMDA delay above 255 steps");
         maxd_n = std::max(maxd_n, dl);
@@ -242,7 +245,9 @@ struct Engine final : EngineBase {
     const size_t n = (size_t)N;
     net.N = N;
     net.D = maxd_p + 1;
-    net.DN = maxd_n + 1;
+    // the fused step reads gate history while writing the current slot
+    net.DN = maxd_n + 2;
+    fused_ok = d.n_syn > 0 && mind_all >= 1;
     net.lin_mask = lin_mask;
     net.has_nmda_syn = syn_nmda > 0;
     net.ext_mode = d.ext_mode;
@@ -280,8 +285,16 @@ struct Engine final : EngineBase {
       HIPCHECK(hipMemcpy(b_meta_n.p, meta_n.data(), syn_nmda * 4, hipMemcpyHostToDevice));
       HIPCHECK(hipMemcpy(b_w_n.p, w_n.data(), syn_nmda * 4, hipMemcpyHostToDevice));
     }
+    {
+      std::vector<uint32_t> nout(n, 0u);
+      for (int64_t s = 0; s < d.n_syn; ++s)
+        if (d.receptor[s] == FS_NMDA) nout[d.pre[s]]++;
+      b_nmda_out.alloc(n * 4);
+      HIPCHECK(hipMemcpy(b_nmda_out.p, nout.data(), n * 4, hipMemcpyHostToDevice));
+    }
     b_total.alloc(4);
     b_overflow.alloc(4);
+    b_done.alloc(4);
     b_t0.alloc(8);
     b_popcnt.alloc((size_t)P * 8);
     b_counters.alloc(4 * 8);
@@ -305,8 +318,10 @@ struct Engine final : EngineBase {
     net.rp_n = b_rp_n.as<uint64_t>();
     net.meta_n = b_meta_n.as<uint32_t>();
     net.w_n = b_w_n.as<float>();
+    net.nmda_out = b_nmda_out.as<uint32_t>();
     net.total = b_total.as<uint32_t>();
     net.overflow = b_overflow.as<uint32_t>();
+    net.done = b_done.as<uint32_t>();
     net.t0 = b_t0.as<int64_t>();
     net.pop_cnt = b_popcnt.as<unsigned long long>();
     net.counters = b_counters.as<unsigned long long>();
@@ -319,6 +334,7 @@ struct Engine final : EngineBase {
     sched.push_block = 256;
     sched.steps_per_graph = 100;
     sched.use_graph = 1;
+    sched.fused = 1;
     apply_schedule();
     reset(0);
   }
@@ -372,6 +388,18 @@ struct Engine final : EngineBase {
     const int dgrid = deliver_grid(&pw);
     net.push_waves = pw;
     if (timed) HIPCHECK(hipEventRecord(ev[0], stream));
+    if (fused()) {
+      if (net.ext_mode == FS_EXT_REPLAY)
+        hipLaunchKernelGGL((fsk::k_step<T, true>), dim3(update_grid()), dim3(sched.update_block), 0, stream, net, k);
+      else
+        hipLaunchKernelGGL((fsk::k_step<T, false>), dim3(update_grid()), dim3(sched.update_block), 0, stream, net, k);
+      HIPCHECK(hipGetLastError());
+      if (timed) {
+        HIPCHECK(hipEventRecord(ev[1], stream));
+        HIPCHECK(hipEventRecord(ev[2], stream));
+      }
+      return;
+    }
     if (net.ext_mode == FS_EXT_REPLAY)
       hipLaunchKernelGGL((fsk::k_update<T, true>), dim3(update_grid()), dim3(sched.update_block), 0, stream, net, k);
     else
@@ -383,11 +411,22 @@ struct Engine final : EngineBase {
     if (timed) HIPCHECK(hipEventRecord(ev[2], stream));
   }
 
+  // fused path: the spikes of the batch's last step are pushed by this launch
+  void launch_tail(int32_t k) {
+    int pw;
+    (void)deliver_grid(&pw);
+    const int wpb = sched.push_block / 64;
+    hipLaunchKernelGGL((fsk::k_push_only<T>), dim3((pw + wpb - 1) / wpb), dim3(sched.push_block), 0,
+                       stream, net, k);
+    HIPCHECK(hipGetLastError());
+  }
+
   void build_graph(int32_t K) {
     if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
     if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
     HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     for (int32_t k = 1; k < K; ++k) launch_step(k, false);
+    if (fused()) launch_tail(K - 1);
     HIPCHECK(hipStreamEndCapture(stream, &graph));
     HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
     graph_k = K;
@@ -415,6 +454,7 @@ struct Engine final : EngineBase {
     HIPCHECK(hipMemsetAsync(b_popcnt.p, 0, (size_t)P * 8, stream));
     HIPCHECK(hipMemsetAsync(b_counters.p, 0, 32, stream));
     HIPCHECK(hipMemsetAsync(b_overflow.p, 0, 4, stream));
+    HIPCHECK(hipMemsetAsync(b_done.p, 0, 4, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     net.seed = seed;
     graph_dirty = true;  // the seed is a kernel argument
@@ -467,7 +507,10 @@ struct Engine final : EngineBase {
           HIPCHECK(hipGraphLaunch(gexec, stream));
         } else {
           for (int32_t k = 1; k < K; ++k) launch_step(k, false);
+          if (fused()) launch_tail(K - 1);
         }
+      } else if (fused()) {
+        launch_tail(0);
       }
       HIPCHECK(hipStreamSynchronize(stream));
       if (timing) {
@@ -536,7 +579,15 @@ struct Engine final : EngineBase {
         d2h(b_tmp.p);
         break;
       }
-      case 5: d2h(b_gn.p); break;
+      case 5:
+        if (fused() && net.has_nmda_syn) {  // the fused step keeps gN in registers
+          hipLaunchKernelGGL((fsk::k_gather_only<T>), dim3((N + 3) / 4), dim3(256), 0, stream, net,
+                             step - 1);
+          HIPCHECK(hipGetLastError());
+          HIPCHECK(hipStreamSynchronize(stream));
+        }
+        d2h(b_gn.p);
+        break;
       case 8: d2h(b_spre.p); break;
       case 9: d2h(b_xn.p); break;
       case 10: {
@@ -560,10 +611,8 @@ struct Engine final : EngineBase {
     if (g) *g = 0.0;
     if (n) *n = n_timed;
   }
-  void counters(int64_t *s, int64_t *e, int64_t *t) override {
-    if (s) *s = (int64_t)ctr_last[0];
-    if (e) *e = (int64_t)ctr_last[1];
-    if (t) *t = (int64_t)ctr_last[2];
+  void counters(int64_t *out) override {
+    for (int i = 0; i < 4; ++i) out[i] = (int64_t)ctr_last[i];
   }
 };
 
@@ -705,9 +754,9 @@ int fs_kernel_times(fs_handle *h, double *u, double *p, double *g, int64_t *n) {
   return FS_OK;
 }
 
-int fs_run_counters(fs_handle *h, int64_t *s, int64_t *e, int64_t *t) {
-  if (!h) return fs::set_error(FS_ERR_ARG, "null handle");
-  h->e->counters(s, e, t);
+int fs_run_counters(fs_handle *h, int64_t out[4]) {
+  if (!h || !out) return fs::set_error(FS_ERR_ARG, "null argument");
+  h->e->counters(out);
   return FS_OK;
 }
 

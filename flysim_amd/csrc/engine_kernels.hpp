@@ -82,6 +82,7 @@ struct Net {
   const uint64_t *rp_n;
   const uint32_t *meta_n;   // pre | delay<<24
   const float *w_n;
+  const uint32_t *nmda_out; // [N] NMDA out-degree (event accounting)
   // replayed external bits
   const uint32_t *ext_bits;
   int64_t ext_first, ext_steps;
@@ -92,10 +93,12 @@ struct Net {
   uint32_t *total;          // [1] spikes recorded in this batch
   uint32_t *cum;            // [K+1] per-step offsets
   uint32_t *overflow;       // [1]
+  uint32_t *done;           // [1] blocks finished in the current fused step
   uint64_t rec_cap;
   const int64_t *t0;        // [1] first step of the batch
   unsigned long long *pop_cnt;  // [P]
-  unsigned long long *counters; // [4]: spikes, syn events, nmda terms, reserved
+  unsigned long long *counters; // [4]: spikes, linear pushes, NMDA out-synapses of spikers,
+                                //      NMDA terms gathered
   int32_t gather_lanes;
   int32_t push_waves;       // waves of k_deliver dedicated to the push
 };
@@ -116,11 +119,14 @@ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1
 __device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
 
 // ------------------------------------------------------------------ update
+// One neuron's step t (the reference's Activation) followed by the wave's
+// spike compaction; every lane of the wave must reach it (ballots).  With
+// gn_given the NMDA conductance comes from the caller (fused step) instead of
+// the gn array written by the separate gather.
 template <typename T, bool REPLAY>
-__global__ void __launch_bounds__(1024) k_update(Net<T> net, int32_t k) {
-  const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int64_t t,
+                                              bool gn_given, T gn_in) {
   const bool active = n < net.N;
-  const int64_t t = *net.t0 + k;
   bool spk = false;
   T a_push = T(0);
   if (active) {
@@ -144,7 +150,7 @@ __global__ void __launch_bounds__(1024) k_update(Net<T> net, int32_t k) {
       *gp = gv;
       g[rid] = gv;
     }
-    if (net.has_nmda_syn) g[2] = net.gn[n];
+    if (net.has_nmda_syn) g[2] = gn_given ? gn_in : net.gn[n];
     // (2) external Poisson gates, order A, G, ACh, GluCl, NMDA (BgInit)
     const uint32_t act = P.act;
     uint32_t rnd[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
@@ -276,14 +282,147 @@ __global__ void __launch_bounds__(1024) k_update(Net<T> net, int32_t k) {
   }
 }
 
+template <typename T, bool REPLAY>
+__global__ void __launch_bounds__(1024) k_update(Net<T> net, int32_t k) {
+  update_neuron<T, REPLAY>(net, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x), *net.t0 + k,
+                           false, T(0));
+}
+
 // ------------------------------------------------------------------ delivery
 template <typename T>
 __device__ __forceinline__ void atomic_add_t(T *p, T v) {
   unsafeAtomicAdd(p, v);
 }
 
+// Push the spikes recorded at [start, end) (emitted at step ts) into the
+// ring slots of their synapses' delays; one wave per spike, strided over
+// `nwaves` waves.  Lane 0 of each wave accumulates the event counters.
 template <typename T>
-__global__ void __launch_bounds__(256) k_deliver(Net<T> net, int32_t k) {
+__device__ __forceinline__ void push_spikes(const Net<T> &net, uint32_t start, uint32_t end,
+                                            int64_t ts, int gwave, int nwaves, int lane,
+                                            unsigned long long &events,
+                                            unsigned long long &nout) {
+  const int32_t N = net.N;
+  for (uint32_t i = start + (uint32_t)gwave; i < end; i += (uint32_t)nwaves) {
+    const uint32_t j = net.rec_ids[i];
+    const T a = net.rec_a[i];
+    const uint64_t r0 = net.rp_p[j], r1 = net.rp_p[j + 1];
+    if (lane == 0) { events += r1 - r0; nout += net.nmda_out[j]; }
+    for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
+      const uint32_t m = net.meta_p[e];
+      const uint32_t col = m & 0x00FFFFFFu;
+      const uint32_t d = (m >> 24) & 63u;
+      const uint32_t li = m >> 30;
+      const int32_t slot = (int32_t)((ts + (int64_t)d) % net.D);
+      atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(net.w_p[e]) * a);
+    }
+  }
+}
+
+// NMDA conductance of one row: sum_j w * s_j(ts - d) over CSR-by-post row p,
+// with L lanes of a group striding the row and a shuffle reduction (result
+// valid in every lane of the group).  All lanes of the group must call it.
+template <typename T>
+__device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int64_t ts, int L, int sl,
+                                        unsigned long long &terms) {
+  T acc = T(0);
+  if (p < net.N) {
+    const uint64_t r0 = net.rp_n[p], r1 = net.rp_n[p + 1];
+    if (sl == 0) terms += r1 - r0;
+    for (uint64_t e = r0 + (uint64_t)sl; e < r1; e += (uint64_t)L) {
+      const uint32_t m = net.meta_n[e];
+      const uint32_t j = m & 0x00FFFFFFu;
+      const uint32_t d = m >> 24;
+      const int32_t hs = (int32_t)(((ts - (int64_t)d) % net.DN + net.DN) % net.DN);
+      acc += T(net.w_n[e]) * net.shist[(size_t)hs * net.N + j];
+    }
+  }
+  for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+  return acc;
+}
+
+// One fused launch per step, valid when every synapse has delay >= 1 step:
+// (a) push the spikes of step t-1 (they land in ring slots >= t), (b) gather
+// the NMDA conductance of step t-1 for the wave's 64 neurons (gates of steps
+// <= t-2), (c) update the neurons of step t and record their spikes, (d) the
+// last block to finish publishes the step's spike-list end offset.
+template <typename T, bool REPLAY>
+__global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
+  const int64_t t = *net.t0 + k;
+  const int wave = (int)(threadIdx.x >> 6);
+  const int lane = (int)(threadIdx.x & 63);
+  const int wpb = (int)(blockDim.x >> 6);
+  const int gwave = (int)blockIdx.x * wpb + wave;
+  const int nwaves = (int)gridDim.x * wpb;
+  unsigned long long my_events = 0, my_nout = 0, my_terms = 0;
+  if (k >= 1) push_spikes(net, net.cum[k - 1], net.cum[k], t - 1, gwave, nwaves, lane,
+                          my_events, my_nout);
+  const int32_t n = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  T gn = T(0);
+  if (net.has_nmda_syn) {
+    __shared__ T s_gn[16][64];
+    const int L = net.gather_lanes;
+    const int G = 64 / L;
+    const int g = lane / L, sl = lane % L;
+    const int64_t row0 = (int64_t)gwave * 64;
+    for (int r = 0; r < L; ++r) {
+      const T v = gather_row(net, row0 + r * G + g, t - 1, L, sl, my_terms);
+      if (sl == 0) s_gn[wave][r * G + g] = v;
+    }
+    __syncthreads();
+    gn = s_gn[wave][lane];
+  }
+  update_neuron<T, REPLAY>(net, n, t, true, gn);
+  __shared__ unsigned long long s_ev[3];
+  __shared__ bool s_last;
+  if (threadIdx.x == 0) { s_ev[0] = 0; s_ev[1] = 0; s_ev[2] = 0; }
+  __syncthreads();
+  if (my_events) atomicAdd(&s_ev[0], my_events);
+  if (my_nout) atomicAdd(&s_ev[1], my_nout);
+  if (my_terms) atomicAdd(&s_ev[2], my_terms);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_ev[0]) atomicAdd(&net.counters[1], s_ev[0]);
+    if (s_ev[1]) atomicAdd(&net.counters[2], s_ev[1]);
+    if (s_ev[2]) atomicAdd(&net.counters[3], s_ev[2]);
+    const unsigned prev = atomicAdd(net.done, 1u);
+    s_last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    const uint32_t tot = atomicAdd(net.total, 0u);
+    net.cum[k + 1] = tot < net.rec_cap ? tot : (uint32_t)net.rec_cap;
+    atomicExch(net.done, 0u);
+  }
+}
+
+// Push of the last step of a fused batch (its spikes are in
+// [cum[k], cum[k+1])).
+template <typename T>
+__global__ void __launch_bounds__(1024) k_push_only(Net<T> net, int32_t k) {
+  const int64_t t = *net.t0 + k;
+  const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int wpb = (int)(blockDim.x >> 6);
+  unsigned long long ev = 0, no = 0;
+  push_spikes(net, net.cum[k], net.cum[k + 1], t, (int)blockIdx.x * wpb + wave,
+              (int)gridDim.x * wpb, lane, ev, no);
+  if (ev) atomicAdd(&net.counters[1], ev);
+  if (no) atomicAdd(&net.counters[2], no);
+}
+
+// NMDA gather of step ts into gn (state readback of the fused path).
+template <typename T>
+__global__ void __launch_bounds__(1024) k_gather_only(Net<T> net, int64_t ts) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  unsigned long long terms = 0;
+  const T v = gather_row(net, p, ts, 64, lane, terms);
+  if (lane == 0 && p < net.N) net.gn[p] = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
   const int64_t t = *net.t0 + k;
   const int32_t N = net.N;
   const int wave = (int)(threadIdx.x >> 6);
@@ -291,7 +430,7 @@ __global__ void __launch_bounds__(256) k_deliver(Net<T> net, int32_t k) {
   const int waves_per_block = (int)(blockDim.x >> 6);
   const int gwave = (int)blockIdx.x * waves_per_block + wave;
   const int total_waves = (int)gridDim.x * waves_per_block;
-  unsigned long long my_events = 0, my_terms = 0;
+  unsigned long long my_events = 0, my_nout = 0, my_terms = 0;
   const uint32_t end_raw = *net.total;
   if (gwave < net.push_waves) {
     // ---- push: one wave per spike of step t
@@ -302,7 +441,7 @@ __global__ void __launch_bounds__(256) k_deliver(Net<T> net, int32_t k) {
       const uint32_t j = net.rec_ids[i];
       const T a = net.rec_a[i];
       const uint64_t r0 = net.rp_p[j], r1 = net.rp_p[j + 1];
-      if (lane == 0) my_events += r1 - r0;
+      if (lane == 0) { my_events += r1 - r0; my_nout += net.nmda_out[j]; }
       for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
         const uint32_t m = net.meta_p[e];
         const uint32_t col = m & 0x00FFFFFFu;
@@ -338,15 +477,17 @@ __global__ void __launch_bounds__(256) k_deliver(Net<T> net, int32_t k) {
     }
   }
   // block-level reduction of the event counters: one atomic per block
-  __shared__ unsigned long long s_ev[2];
-  if (threadIdx.x == 0) { s_ev[0] = 0; s_ev[1] = 0; }
+  __shared__ unsigned long long s_ev[3];
+  if (threadIdx.x == 0) { s_ev[0] = 0; s_ev[1] = 0; s_ev[2] = 0; }
   __syncthreads();
   if (my_events) atomicAdd(&s_ev[0], my_events);
-  if (my_terms) atomicAdd(&s_ev[1], my_terms);
+  if (my_nout) atomicAdd(&s_ev[1], my_nout);
+  if (my_terms) atomicAdd(&s_ev[2], my_terms);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (s_ev[0]) atomicAdd(&net.counters[1], s_ev[0]);
     if (s_ev[1]) atomicAdd(&net.counters[2], s_ev[1]);
+    if (s_ev[2]) atomicAdd(&net.counters[3], s_ev[2]);
   }
 }
 

@@ -274,6 +274,10 @@ class Engine:
         return {"update_ms": u.value, "deliver_ms": p.value, "n_timed": n.value}
 
     def counters(self) -> Dict[str, int]:
-        s, e, t = C.c_int64(), C.c_int64(), C.c_int64()
-        A.check(A.lib().fs_run_counters(self._h, C.byref(s), C.byref(e), C.byref(t)))
-        return {"spikes": s.value, "syn_events": e.value, "nmda_terms": t.value}
+        """Counters of the last run(): spikes, linear receptor-synapses
+        pushed, NMDA receptor-synapses of the spiking neurons, their sum
+        (synaptic events) and the NMDA terms gathered."""
+        out = np.zeros(4, np.int64)
+        A.check(A.lib().fs_run_counters(self._h, out.ctypes.data))
+        return {"spikes": int(out[0]), "lin_pushes": int(out[1]), "nmda_out": int(out[2]),
+                "syn_events": int(out[1] + out[2]), "nmda_terms": int(out[3])}

@@ -353,6 +353,57 @@ def write_pro(path: str, t_end_ms: float, seeds: Sequence[int] = (),
         f.write("".join(out))
 
 
+def write_module_conf(net: Network, path: str, spike_delay_ms: Optional[float] = None) -> Dict:
+    """Write a population-level ("module form") network.conf of `net`: one
+    TargetPopulation link per (pre population, post population, receptor)
+    with Connectivity = the pair density of that block and MeanEff = its mean
+    efficacy.  The reference realises it with a fixed out-degree
+    ceil(N_post * Connectivity) per presynaptic neuron (code/NetSimGnl2_BulPp.cpp:310)
+    and one efficacy per link, so it is a network of the same size and class,
+    not the same synapses; per-synapse delays become one SpikeDly per
+    population (mean delay when `spike_delay_ms` is None).  Used to time the
+    reference on the fly-like and 1M workloads."""
+    off = net.pop_offsets()
+    P = len(net.pops)
+    pre_pop = np.searchsorted(off, net.pre, side="right") - 1
+    post_pop = np.searchsorted(off, net.post, side="right") - 1
+    key = (pre_pop.astype(np.int64) * P + post_pop) * 8 + net.receptor
+    uniq, inv, cnt = np.unique(key, return_inverse=True, return_counts=True)
+    wsum = np.bincount(inv, weights=net.w.astype(np.float64))
+    links: Dict[int, list] = {}
+    for k, c, ws in zip(uniq, cnt, wsum):
+        rec = int(k % 8)
+        pp, qq = divmod(int(k // 8), P)
+        n_pre, n_post = net.pops[pp].n, net.pops[qq].n
+        conn = c / float(n_pre) / float(n_post)
+        links.setdefault(pp, []).append((qq, rec, conn, ws / c))
+    if spike_delay_ms is None:
+        spike_delay_ms = float(np.mean(net.delay)) * net.dt if net.n_syn else 0.0
+    out = []
+    for i, pop in enumerate(net.pops):
+        out.append("NeuralPopulation: %s\nN=%d\nC=%s\nTaum=%s\nThreshold=%s\nRestPot=%s\n"
+                   "ResetPot=%s\nRefractoryPeriod=%s\nSpikeDly=%s\n" % (
+                       pop.name, pop.n, _fmt(pop.cm), _fmt(pop.taum), _fmt(pop.vth),
+                       _fmt(pop.rest), _fmt(pop.reset), _fmt(pop.refractory_ms),
+                       _fmt(spike_delay_ms)))
+        for r in pop.receptors:
+            out.append("Receptor:%s\nTau=%s\nTauXT=%s\nRevPot=%s\nFreqExt=%s\n"
+                       "MeanExtEff=%s\nMeanExtCon=%s\nEndReceptor\n" % (
+                           RECEPTOR_NAMES[r.kind], _fmt(r.tau), _fmt(r.tau_xt), _fmt(r.rev),
+                           _fmt(r.freq_ext), _fmt(r.mean_ext_eff), _fmt(r.mean_ext_con)))
+        for (qq, rec, conn, g) in links.get(i, []):
+            out.append("TargetPopulation: %s\nTargetReceptor=%s\nMeanEff=%s\nConnectivity=%s\n"
+                       "EndTargetPopulation\n" % (net.pops[qq].name, RECEPTOR_NAMES[rec],
+                                                  _fmt(g), _fmt(conn)))
+        out.append("EndNeuralPopulation\n\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+    n_links = sum(len(v) for v in links.values())
+    syn = sum(int(np.ceil(net.pops[qq].n * np.float32(conn))) * net.pops[pp].n
+              for pp, v in links.items() for (qq, rec, conn, g) in v)
+    return {"links": n_links, "approx_synapses": int(syn), "spike_delay_ms": spike_delay_ms}
+
+
 def read_spikes(path: str) -> np.ndarray:
     """Read a reference ``Spike`` file ("%.7g time_s nid" lines) as (step?, nid).
 

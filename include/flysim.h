@@ -137,9 +137,11 @@ int fs_get_state(fs_handle *h, int32_t field, double *out);
 int fs_pop_spike_counts(fs_handle *h, uint64_t *out);
 
 /* Launch schedule (autotuner output): block size of the neuron update,
- * lanes per row of the NMDA gather, steps per captured graph. */
+ * lanes per row of the NMDA gather, block size of the delivery, steps per
+ * captured graph, graph on/off, and the one-launch-per-step fused path
+ * (used only when every synapse has a delay of at least one step). */
 typedef struct fs_schedule {
-  int32_t update_block, gather_lanes, push_block, steps_per_graph, use_graph;
+  int32_t update_block, gather_lanes, push_block, steps_per_graph, use_graph, fused;
 } fs_schedule;
 int fs_get_schedule(fs_handle *h, fs_schedule *s);
 int fs_set_schedule(fs_handle *h, const fs_schedule *s);
@@ -151,9 +153,10 @@ int fs_enable_timing(fs_handle *h, int32_t on);
 int fs_kernel_times(fs_handle *h, double *update_ms, double *push_ms, double *gather_ms,
                     int64_t *n_timed);
 
-/* Algorithmic counters of the last fs_run: spikes, synaptic events (pushes),
- * NMDA terms gathered. */
-int fs_run_counters(fs_handle *h, int64_t *spikes, int64_t *syn_events, int64_t *nmda_terms);
+/* Algorithmic counters of the last fs_run: out[0] spikes, out[1] linear
+ * receptor-synapses pushed, out[2] NMDA receptor-synapses of the spiking
+ * neurons (out[1]+out[2] = synaptic events), out[3] NMDA terms gathered. */
+int fs_run_counters(fs_handle *h, int64_t out[4]);
 
 int fs_destroy(fs_handle *h);
 const char *fs_last_error(void);
