@@ -78,7 +78,7 @@ struct Engine final : EngineBase {
   // device buffers
   DevBuf b_npop, b_pops, b_V, b_cs, b_G, b_ring, b_spre, b_xn, b_shist, b_gn, b_elin, b_enx,
       b_ens, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
-      b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_done;
+      b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase;
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
   bool fused() const { return fused_ok && sched.fused; }
   hipStream_t stream = nullptr;
@@ -294,10 +294,11 @@ struct Engine final : EngineBase {
     }
     b_total.alloc(4);
     b_overflow.alloc(4);
-    b_done.alloc(4);
+    b_lcnt.alloc(4 * 4);
+    b_lbase.alloc(4 * 4);
     b_t0.alloc(8);
     b_popcnt.alloc((size_t)P * 8);
-    b_counters.alloc(4 * 8);
+    b_counters.alloc(3 * 8);
     b_tmp.alloc(n * sizeof(T));
     net.npop = b_npop.as<uint32_t>();
     net.pops = b_pops.as<fsk::DevPop<T>>();
@@ -321,7 +322,8 @@ struct Engine final : EngineBase {
     net.nmda_out = b_nmda_out.as<uint32_t>();
     net.total = b_total.as<uint32_t>();
     net.overflow = b_overflow.as<uint32_t>();
-    net.done = b_done.as<uint32_t>();
+    net.lcnt = b_lcnt.as<uint32_t>();
+    net.lbase = b_lbase.as<uint32_t>();
     net.t0 = b_t0.as<int64_t>();
     net.pop_cnt = b_popcnt.as<unsigned long long>();
     net.counters = b_counters.as<unsigned long long>();
@@ -334,7 +336,7 @@ struct Engine final : EngineBase {
     sched.push_block = 256;
     sched.steps_per_graph = 100;
     sched.use_graph = 1;
-    sched.fused = 1;
+    sched.fused = 0;
     apply_schedule();
     reset(0);
   }
@@ -367,6 +369,11 @@ struct Engine final : EngineBase {
   }
 
   int update_grid() const { return (N + sched.update_block - 1) / sched.update_block; }
+  // fused step: a group of gather_lanes lanes per neuron
+  int step_grid() const {
+    const int64_t threads = (int64_t)N * sched.gather_lanes;
+    return (int)((threads + sched.update_block - 1) / sched.update_block);
+  }
   int deliver_grid(int *push_waves) const {
     const int wpb = sched.push_block / 64;
     // push waves: enough for the typical spike count of a step (spikes are
@@ -390,9 +397,9 @@ struct Engine final : EngineBase {
     if (timed) HIPCHECK(hipEventRecord(ev[0], stream));
     if (fused()) {
       if (net.ext_mode == FS_EXT_REPLAY)
-        hipLaunchKernelGGL((fsk::k_step<T, true>), dim3(update_grid()), dim3(sched.update_block), 0, stream, net, k);
+        hipLaunchKernelGGL((fsk::k_step<T, true>), dim3(step_grid()), dim3(sched.update_block), 0, stream, net, k);
       else
-        hipLaunchKernelGGL((fsk::k_step<T, false>), dim3(update_grid()), dim3(sched.update_block), 0, stream, net, k);
+        hipLaunchKernelGGL((fsk::k_step<T, false>), dim3(step_grid()), dim3(sched.update_block), 0, stream, net, k);
       HIPCHECK(hipGetLastError());
       if (timed) {
         HIPCHECK(hipEventRecord(ev[1], stream));
@@ -411,13 +418,14 @@ struct Engine final : EngineBase {
     if (timed) HIPCHECK(hipEventRecord(ev[2], stream));
   }
 
-  // fused path: the spikes of the batch's last step are pushed by this launch
-  void launch_tail(int32_t k) {
+  // fused path: the spikes of the last step of a batch of K launches are
+  // pushed by this launch
+  void launch_tail(int32_t K) {
     int pw;
     (void)deliver_grid(&pw);
     const int wpb = sched.push_block / 64;
-    hipLaunchKernelGGL((fsk::k_push_only<T>), dim3((pw + wpb - 1) / wpb), dim3(sched.push_block), 0,
-                       stream, net, k);
+    hipLaunchKernelGGL((fsk::k_push_tail<T>), dim3((pw + wpb - 1) / wpb), dim3(sched.push_block), 0,
+                       stream, net, K);
     HIPCHECK(hipGetLastError());
   }
 
@@ -426,7 +434,7 @@ struct Engine final : EngineBase {
     if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
     HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     for (int32_t k = 1; k < K; ++k) launch_step(k, false);
-    if (fused()) launch_tail(K - 1);
+    if (fused()) launch_tail(K);
     HIPCHECK(hipStreamEndCapture(stream, &graph));
     HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
     graph_k = K;
@@ -452,9 +460,10 @@ struct Engine final : EngineBase {
     fill(b_enx.p, n, T(0));
     fill(b_ens.p, n, T(0));
     HIPCHECK(hipMemsetAsync(b_popcnt.p, 0, (size_t)P * 8, stream));
-    HIPCHECK(hipMemsetAsync(b_counters.p, 0, 32, stream));
+    HIPCHECK(hipMemsetAsync(b_counters.p, 0, 24, stream));
     HIPCHECK(hipMemsetAsync(b_overflow.p, 0, 4, stream));
-    HIPCHECK(hipMemsetAsync(b_done.p, 0, 4, stream));
+    HIPCHECK(hipMemsetAsync(b_lcnt.p, 0, 16, stream));
+    HIPCHECK(hipMemsetAsync(b_lbase.p, 0, 16, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     net.seed = seed;
     graph_dirty = true;  // the seed is a kernel argument
@@ -493,12 +502,13 @@ struct Engine final : EngineBase {
     if (off) off[0] = 0;
     t_upd = t_del = 0;
     n_timed = 0;
-    HIPCHECK(hipMemsetAsync(b_counters.p, 0, 32, stream));
+    HIPCHECK(hipMemsetAsync(b_counters.p, 0, 24, stream));
     int64_t done = 0;
     while (done < nsteps) {
       const int32_t K = (int32_t)std::min<int64_t>(sched.steps_per_graph, nsteps - done);
       hipLaunchKernelGGL(fsk::k_batch_begin, dim3(1), dim3(1), 0, stream, b_t0.as<int64_t>(), step,
-                         b_total.as<uint32_t>(), b_cum.as<uint32_t>());
+                         b_total.as<uint32_t>(), b_cum.as<uint32_t>(), b_lcnt.as<uint32_t>(),
+                         b_lbase.as<uint32_t>());
       HIPCHECK(hipGetLastError());
       launch_step(0, timing);
       if (K > 1) {
@@ -507,10 +517,10 @@ struct Engine final : EngineBase {
           HIPCHECK(hipGraphLaunch(gexec, stream));
         } else {
           for (int32_t k = 1; k < K; ++k) launch_step(k, false);
-          if (fused()) launch_tail(K - 1);
+          if (fused()) launch_tail(K);
         }
       } else if (fused()) {
-        launch_tail(0);
+        launch_tail(1);
       }
       HIPCHECK(hipStreamSynchronize(stream));
       if (timing) {
@@ -541,7 +551,9 @@ struct Engine final : EngineBase {
       step += K;
       done += K;
     }
-    HIPCHECK(hipMemcpy(ctr_last, b_counters.p, 32, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(ctr_last, b_counters.p, 24, hipMemcpyDeviceToHost));
+    // every step gathers each NMDA synapse's term once
+    ctr_last[3] = (unsigned long long)syn_nmda * (unsigned long long)nsteps;
   }
 
   void get_state(int32_t f, double *out) override {

@@ -93,12 +93,12 @@ struct Net {
   uint32_t *total;          // [1] spikes recorded in this batch
   uint32_t *cum;            // [K+1] per-step offsets
   uint32_t *overflow;       // [1]
-  uint32_t *done;           // [1] blocks finished in the current fused step
+  uint32_t *lcnt;           // [3] fused path: spike count of the lists of steps k%3
+  uint32_t *lbase;          // [3] fused path: record offset of the list of step k%3
   uint64_t rec_cap;
   const int64_t *t0;        // [1] first step of the batch
   unsigned long long *pop_cnt;  // [P]
-  unsigned long long *counters; // [4]: spikes, linear pushes, NMDA out-synapses of spikers,
-                                //      NMDA terms gathered
+  unsigned long long *counters; // [3]: spikes, linear pushes, NMDA out-synapses of spikers
   int32_t gather_lanes;
   int32_t push_waves;       // waves of k_deliver dedicated to the push
 };
@@ -125,7 +125,8 @@ __device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * (1.0
 // the gn array written by the separate gather.
 template <typename T, bool REPLAY>
 __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int64_t t,
-                                              bool gn_given, T gn_in) {
+                                              bool gn_given, T gn_in, uint32_t *list_ctr,
+                                              uint32_t list_base) {
   const bool active = n < net.N;
   bool spk = false;
   T a_push = T(0);
@@ -257,7 +258,7 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
     const uint32_t cntw = (uint32_t)__popcll(mask);
     uint32_t base = 0;
     if (lane == leader) {
-      base = atomicAdd(net.total, cntw);
+      base = list_base + atomicAdd(list_ctr, cntw);
       atomicAdd(&net.counters[0], (unsigned long long)cntw);
     }
     base = __shfl(base, leader);
@@ -285,7 +286,7 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
 template <typename T, bool REPLAY>
 __global__ void __launch_bounds__(1024) k_update(Net<T> net, int32_t k) {
   update_neuron<T, REPLAY>(net, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x), *net.t0 + k,
-                           false, T(0));
+                           false, T(0), net.total, 0u);
 }
 
 // ------------------------------------------------------------------ delivery
@@ -319,16 +320,31 @@ __device__ __forceinline__ void push_spikes(const Net<T> &net, uint32_t start, u
   }
 }
 
+// Block-level reduction of the push counters (synaptic events, NMDA
+// out-synapses of the spikers): one global atomic per block that pushed.
+template <typename T>
+__device__ __forceinline__ void block_add_events(const Net<T> &net, unsigned long long events,
+                                                 unsigned long long nout) {
+  __shared__ unsigned long long s_ev[2];
+  if (threadIdx.x == 0) { s_ev[0] = 0; s_ev[1] = 0; }
+  __syncthreads();
+  if (events) atomicAdd(&s_ev[0], events);
+  if (nout) atomicAdd(&s_ev[1], nout);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_ev[0]) atomicAdd(&net.counters[1], s_ev[0]);
+    if (s_ev[1]) atomicAdd(&net.counters[2], s_ev[1]);
+  }
+}
+
 // NMDA conductance of one row: sum_j w * s_j(ts - d) over CSR-by-post row p,
 // with L lanes of a group striding the row and a shuffle reduction (result
 // valid in every lane of the group).  All lanes of the group must call it.
 template <typename T>
-__device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int64_t ts, int L, int sl,
-                                        unsigned long long &terms) {
+__device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int64_t ts, int L, int sl) {
   T acc = T(0);
   if (p < net.N) {
     const uint64_t r0 = net.rp_n[p], r1 = net.rp_n[p + 1];
-    if (sl == 0) terms += r1 - r0;
     for (uint64_t e = r0 + (uint64_t)sl; e < r1; e += (uint64_t)L) {
       const uint32_t m = net.meta_n[e];
       const uint32_t j = m & 0x00FFFFFFu;
@@ -343,72 +359,61 @@ __device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int64_t ts
 
 // One fused launch per step, valid when every synapse has delay >= 1 step:
 // (a) push the spikes of step t-1 (they land in ring slots >= t), (b) gather
-// the NMDA conductance of step t-1 for the wave's 64 neurons (gates of steps
-// <= t-2), (c) update the neurons of step t and record their spikes, (d) the
-// last block to finish publishes the step's spike-list end offset.
+// the NMDA conductance of step t-1 (gates of steps <= t-2) with a group of
+// L = gather_lanes lanes per neuron, (c) the group's first lane updates the
+// neuron for step t and appends its spike to the list of step k.  The per-step
+// lists rotate over three counters (k%3): launch k reads the final count of
+// list k-1, appends to list k (zeroed by launch k-1) and zeroes list k+1, so no
+// block waits for another within a launch.  The grid has N*L threads, so
+// small networks still present L times more waves to hide memory latency.
 template <typename T, bool REPLAY>
 __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
   const int64_t t = *net.t0 + k;
+  const int pm = (k + 2) % 3, pc = k % 3, pn = (k + 1) % 3;
+  const uint32_t prev_base = net.lbase[pm], prev_cnt = net.lcnt[pm];
+  const uint32_t base_k = prev_base + prev_cnt;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    net.lbase[pc] = base_k;
+    net.lcnt[pn] = 0u;
+    net.cum[k] = base_k < net.rec_cap ? base_k : (uint32_t)net.rec_cap;
+  }
   const int wave = (int)(threadIdx.x >> 6);
   const int lane = (int)(threadIdx.x & 63);
   const int wpb = (int)(blockDim.x >> 6);
-  const int gwave = (int)blockIdx.x * wpb + wave;
-  const int nwaves = (int)gridDim.x * wpb;
-  unsigned long long my_events = 0, my_nout = 0, my_terms = 0;
-  if (k >= 1) push_spikes(net, net.cum[k - 1], net.cum[k], t - 1, gwave, nwaves, lane,
-                          my_events, my_nout);
-  const int32_t n = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  unsigned long long my_events = 0, my_nout = 0;
+  {
+    const uint32_t cap = (uint32_t)net.rec_cap;
+    const uint32_t s0 = prev_base < cap ? prev_base : cap;
+    const uint32_t s1 = base_k < cap ? base_k : cap;
+    push_spikes(net, s0, s1, t - 1, (int)blockIdx.x * wpb + wave, (int)gridDim.x * wpb, lane,
+                my_events, my_nout);
+  }
+  const int L = net.gather_lanes;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nn = gt / L;
+  const int sl = (int)(gt % L);
   T gn = T(0);
-  if (net.has_nmda_syn) {
-    __shared__ T s_gn[16][64];
-    const int L = net.gather_lanes;
-    const int G = 64 / L;
-    const int g = lane / L, sl = lane % L;
-    const int64_t row0 = (int64_t)gwave * 64;
-    for (int r = 0; r < L; ++r) {
-      const T v = gather_row(net, row0 + r * G + g, t - 1, L, sl, my_terms);
-      if (sl == 0) s_gn[wave][r * G + g] = v;
-    }
-    __syncthreads();
-    gn = s_gn[wave][lane];
-  }
-  update_neuron<T, REPLAY>(net, n, t, true, gn);
-  __shared__ unsigned long long s_ev[3];
-  __shared__ bool s_last;
-  if (threadIdx.x == 0) { s_ev[0] = 0; s_ev[1] = 0; s_ev[2] = 0; }
-  __syncthreads();
-  if (my_events) atomicAdd(&s_ev[0], my_events);
-  if (my_nout) atomicAdd(&s_ev[1], my_nout);
-  if (my_terms) atomicAdd(&s_ev[2], my_terms);
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (s_ev[0]) atomicAdd(&net.counters[1], s_ev[0]);
-    if (s_ev[1]) atomicAdd(&net.counters[2], s_ev[1]);
-    if (s_ev[2]) atomicAdd(&net.counters[3], s_ev[2]);
-    const unsigned prev = atomicAdd(net.done, 1u);
-    s_last = prev == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (s_last && threadIdx.x == 0) {
-    const uint32_t tot = atomicAdd(net.total, 0u);
-    net.cum[k + 1] = tot < net.rec_cap ? tot : (uint32_t)net.rec_cap;
-    atomicExch(net.done, 0u);
-  }
+  if (net.has_nmda_syn) gn = gather_row(net, nn, t - 1, L, sl);
+  const int32_t n = (sl == 0 && nn < net.N) ? (int32_t)nn : net.N;  // N: inactive lane
+  update_neuron<T, REPLAY>(net, n, t, true, gn, net.lcnt + pc, base_k);
+  block_add_events(net, my_events, my_nout);
 }
 
-// Push of the last step of a fused batch (its spikes are in
-// [cum[k], cum[k+1])).
+// Push of the last step of a fused batch of K launches; publishes the end
+// offset of the batch's record.
 template <typename T>
-__global__ void __launch_bounds__(1024) k_push_only(Net<T> net, int32_t k) {
-  const int64_t t = *net.t0 + k;
+__global__ void __launch_bounds__(1024) k_push_tail(Net<T> net, int32_t K) {
+  const int64_t t = *net.t0 + (K - 1);
+  const int pl = (K - 1) % 3;
+  const uint32_t b = net.lbase[pl], e = b + net.lcnt[pl];
+  const uint32_t cap = (uint32_t)net.rec_cap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) net.cum[K] = e < cap ? e : cap;
   const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const int wpb = (int)(blockDim.x >> 6);
   unsigned long long ev = 0, no = 0;
-  push_spikes(net, net.cum[k], net.cum[k + 1], t, (int)blockIdx.x * wpb + wave,
+  push_spikes(net, b < cap ? b : cap, e < cap ? e : cap, t, (int)blockIdx.x * wpb + wave,
               (int)gridDim.x * wpb, lane, ev, no);
-  if (ev) atomicAdd(&net.counters[1], ev);
-  if (no) atomicAdd(&net.counters[2], no);
+  block_add_events(net, ev, no);
 }
 
 // NMDA gather of step ts into gn (state readback of the fused path).
@@ -416,8 +421,7 @@ template <typename T>
 __global__ void __launch_bounds__(1024) k_gather_only(Net<T> net, int64_t ts) {
   const int lane = (int)(threadIdx.x & 63);
   const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  unsigned long long terms = 0;
-  const T v = gather_row(net, p, ts, 64, lane, terms);
+  const T v = gather_row(net, p, ts, 64, lane);
   if (lane == 0 && p < net.N) net.gn[p] = v;
 }
 
@@ -430,7 +434,7 @@ __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
   const int waves_per_block = (int)(blockDim.x >> 6);
   const int gwave = (int)blockIdx.x * waves_per_block + wave;
   const int total_waves = (int)gridDim.x * waves_per_block;
-  unsigned long long my_events = 0, my_nout = 0, my_terms = 0;
+  unsigned long long my_events = 0, my_nout = 0;
   const uint32_t end_raw = *net.total;
   if (gwave < net.push_waves) {
     // ---- push: one wave per spike of step t
@@ -463,7 +467,6 @@ __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
       T acc = T(0);
       if (p < N) {
         const uint64_t r0 = net.rp_n[p], r1 = net.rp_n[p + 1];
-        if (sl == 0) my_terms += r1 - r0;
         for (uint64_t e = r0 + (uint64_t)sl; e < r1; e += (uint64_t)L) {
           const uint32_t m = net.meta_n[e];
           const uint32_t j = m & 0x00FFFFFFu;
@@ -476,26 +479,19 @@ __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
       if (p < N && sl == 0) net.gn[p] = acc;
     }
   }
-  // block-level reduction of the event counters: one atomic per block
-  __shared__ unsigned long long s_ev[3];
-  if (threadIdx.x == 0) { s_ev[0] = 0; s_ev[1] = 0; s_ev[2] = 0; }
-  __syncthreads();
-  if (my_events) atomicAdd(&s_ev[0], my_events);
-  if (my_nout) atomicAdd(&s_ev[1], my_nout);
-  if (my_terms) atomicAdd(&s_ev[2], my_terms);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (s_ev[0]) atomicAdd(&net.counters[1], s_ev[0]);
-    if (s_ev[1]) atomicAdd(&net.counters[2], s_ev[1]);
-    if (s_ev[2]) atomicAdd(&net.counters[3], s_ev[2]);
-  }
+  block_add_events(net, my_events, my_nout);
 }
 
 // batch prologue: first step of the batch, record buffer reset
-__global__ void k_batch_begin(int64_t *t0, int64_t value, uint32_t *total, uint32_t *cum) {
+// (the fused path's list rotation starts with lists 2 (step -1) and 0 empty)
+__global__ void k_batch_begin(int64_t *t0, int64_t value, uint32_t *total, uint32_t *cum,
+                              uint32_t *lcnt, uint32_t *lbase) {
   *t0 = value;
   *total = 0u;
   cum[0] = 0u;
+  lcnt[0] = 0u;
+  lcnt[2] = 0u;
+  lbase[2] = 0u;
 }
 
 template <typename T>

@@ -160,7 +160,7 @@ def c1_network(seed: int = 1, n_exc: int = 800, n_inh: int = 200, p: float = 0.1
 # ---------------------------------------------------------------------------
 # Synthetic Drosophila-scale connectome (20k neurons, 50 modules)
 # ---------------------------------------------------------------------------
-C2_G_EXT = 2.2          # nS per external AMPA event
+C2_G_EXT = 1.1          # nS per external AMPA event (~12 Hz mean; >50 Hz from ~1.5 nS)
 C2_NMDA_RATIO = 0.1
 
 
@@ -252,7 +252,7 @@ def c2_network(seed: int = 2, n: int = 20000, n_mod: int = 50, mu: float = 5.8,
 # ---------------------------------------------------------------------------
 # Scaled sparse network (1M neurons, fixed in-degree 100)
 # ---------------------------------------------------------------------------
-C3_G_EXT = 2.2
+C3_G_EXT = 1.4   # the 1M net is bistable: quiet below ~1.3 nS, refractory-limited above ~1.5
 
 
 def c3_network(seed: int = 3, n: int = 1_000_000, exc_frac: float = 0.8, k_in: int = 100,
