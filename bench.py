@@ -41,7 +41,21 @@ def parse():
                    help="steps of the CPU-baseline sample (-1: sized to ~15 s)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--schedule", default="", help="json dict of fs_schedule overrides")
+    p.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                   help="only run 10 device-to-device copies of MIB MiB (the known-size "
+                        "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return p.parse_args()
+
+
+def copy_calib(mib):
+    import torch
+    a = torch.ones(mib << 18, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calib_bytes_read": mib << 20, "copy_calib_bytes_written": mib << 20,
+                      "launches": 10}))
 
 
 def make_network(name):
@@ -139,6 +153,9 @@ def cpu_baseline_port(net, arr, events_per_sim_s, n_steps):
 
 def main():
     args = parse()
+    if args.copy_calib:
+        copy_calib(args.copy_calib)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
@@ -164,13 +181,21 @@ def main():
     # synaptic events: every receptor-synapse a spike reaches (linear pushes
     # plus the NMDA synapses of the spiking neurons)
     ev_value = ctr["syn_events"] / wall
-    # roofline of the dominant kernel (algorithmic bytes per launch)
+    # roofline of the dominant kernel (algorithmic bytes per launch): the LIF
+    # update moves 84 B per neuron-step (160 B in FP64); delivery moves 8 B per
+    # spike (row pointer) + 17 B per pushed synapse (col, weight, delay, atomic
+    # read-modify-write) and the NMDA gather 12 B per term + 8 B per row
     n_launch = args.steps
     upd_bytes = (UPDATE_BYTES_FP64 if args.fp64 else UPDATE_BYTES_FP32) * net.n
     del_bytes = (8 * ctr["spikes"] + 17 * ctr["lin_pushes"] + 12 * ctr["nmda_terms"]
                  + 8 * net.n * n_launch) / n_launch
     upd_ms, del_ms = kt["update_ms"], kt["deliver_ms"]
-    if del_ms >= upd_ms:
+    if eng.fused:
+        # one kernel per step: its launch time is the graph time per step
+        # (the eager first step of a batch also carries its dispatch latency)
+        dom, b = "k_step", upd_bytes + del_bytes
+        ms = kt["graph_step_ms"] if kt["graph_step_ms"] > 0 else upd_ms
+    elif del_ms >= upd_ms:
         dom, b, ms = "k_deliver", del_bytes, del_ms
     else:
         dom, b, ms = "k_update", upd_bytes, upd_ms
@@ -190,6 +215,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": b, "avg_launch_ms": ms,
                      "update_ms": upd_ms, "deliver_ms": del_ms,
+                     "graph_step_ms": kt["graph_step_ms"],
                      "timed_launches": kt["n_timed"]},
     }
     if not args.no_cpu:

@@ -59,6 +59,10 @@ class Schedule(C.Structure):
                 ("use_graph", C.c_int32), ("fused", C.c_int32)]
 
 
+class ReplicaParams(C.Structure):
+    _fields_ = [("gain", C.c_float * 5), ("ext_gain", C.c_float), ("stream_key", C.c_uint64)]
+
+
 class CliOpts(C.Structure):
     _fields_ = [("dt", C.c_float), ("threads", C.c_int32), ("repeats", C.c_int32),
                 ("stp", C.c_int32), ("std_", C.c_int32), ("ltp", C.c_int32),
@@ -87,7 +91,8 @@ EXPORTS = [
     "fs_pop_spike_counts", "fs_get_schedule", "fs_set_schedule", "fs_enable_timing",
     "fs_kernel_times", "fs_run_counters", "fs_destroy", "fs_last_error", "fs_conf_load",
     "fs_conf_free", "fs_conf_desc", "fs_conf_sizes", "fs_conf_seed", "fs_conf_ext_trains",
-    "fs_conf_ref_layout", "fs_conf_write_outputs", "fs_cli_main",
+    "fs_conf_ref_layout", "fs_conf_write_outputs", "fs_cli_main", "fs_create_replicated",
+    "fs_set_replica",
 ]
 
 _lib = None
@@ -112,6 +117,8 @@ def lib() -> C.CDLL:
         sig = {
             "fs_derive_params": [P(PopConf), C.c_float, C.c_uint32, P(PopParams)],
             "fs_create": [P(NetDesc), P(C.c_void_p)],
+            "fs_create_replicated": [P(NetDesc), C.c_int32, P(C.c_void_p)],
+            "fs_set_replica": [C.c_void_p, C.c_int32, P(ReplicaParams)],
             "fs_reset": [C.c_void_p, C.c_uint64],
             "fs_set_ext_rate": [C.c_void_p, C.c_int32, C.c_int32, C.c_float],
             "fs_set_injection": [C.c_void_p, C.c_int32, C.c_float, C.c_float],

@@ -53,41 +53,52 @@ struct EngineBase {
   virtual ~EngineBase() {}
   virtual void reset(uint64_t seed) = 0;
   virtual void set_pop(int32_t pop, const fs_pop_params &p) = 0;
-  virtual const fs_pop_params &pop(int32_t i) const = 0;
+  virtual fs_pop_params pop(int32_t i) const = 0;
+  virtual void set_ext(int32_t pop, int32_t receptor, float pb, bool on) = 0;
+  virtual void set_replica(int32_t rep, const fs_replica_params &p) = 0;
   virtual void load_ext(const uint32_t *bits, int64_t first, int64_t n) = 0;
   virtual void run(int64_t n, int64_t *off, uint32_t *ids, int64_t cap) = 0;
   virtual void get_state(int32_t f, double *out) = 0;
   virtual void pop_counts(uint64_t *out) = 0;
   virtual void times(double *u, double *p, double *g, int64_t *n) = 0;
   virtual void counters(int64_t *out) = 0;
+  virtual bool fused_active() const = 0;
   fs_schedule sched{};
   bool timing = false;
   int64_t step = 0;
   float dt = 0.1f;
-  int32_t n_pop = 0;
+  int32_t n_pop = 0;     // populations over all replicas (replica-major)
+  int32_t replicas = 1;
 };
 
 template <typename T>
 struct Engine final : EngineBase {
   fsk::Net<T> net{};
-  int32_t N = 0, P = 0;
+  int32_t N = 0, P = 0;   // N over all replicas; P of one replica
+  int32_t N0 = 0, B = 1;
   int32_t precision = FS_FP32;
   bool record = true;
   std::vector<fs_pop_params> hpop;
   std::vector<fsk::DevPop<T>> dpop;
+  std::vector<float> hpbr;      // [5][P][B]
+  std::vector<float> hgain;     // [6][B]
+  std::vector<uint64_t> hskey;  // [B]
   // device buffers
   DevBuf b_npop, b_pops, b_V, b_cs, b_G, b_ring, b_spre, b_xn, b_shist, b_gn, b_elin, b_enx,
       b_ens, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
-      b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase;
+      b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase,
+      b_pbr, b_gain, b_skey;
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
   bool fused() const { return fused_ok && sched.fused; }
+  bool fused_active() const override { return fused(); }
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int32_t graph_k = -1;
   bool graph_dirty = true;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  double t_upd = 0, t_del = 0;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double t_upd = 0, t_del = 0, t_graph = 0;
+  int64_t n_graph_steps = 0;
   int64_t n_timed = 0;
   int64_t syn_lin = 0, syn_nmda = 0;
   std::vector<uint32_t> cum_h;
@@ -117,7 +128,6 @@ struct Engine final : EngineBase {
       d.vrev[r] = T(p.vrev[r]);
       d.dfst[r] = T(p.dfst[r]);
       d.gext[r] = T(p.gext[r]);
-      d.pb[r] = p.pb[r];
     }
     d.dfxt_n = T(p.dfxt_n);
     if (sizeof(T) == 4) d.efas_n = T(p.efst_n * p.as_n);
@@ -136,33 +146,50 @@ struct Engine final : EngineBase {
     HIPCHECK(hipStreamSynchronize(stream));
   }
 
+  void upload_replica_tables() {
+    HIPCHECK(hipMemcpyAsync(b_pbr.p, hpbr.data(), hpbr.size() * 4, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(b_gain.p, hgain.data(), hgain.size() * 4, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(b_skey.p, hskey.data(), hskey.size() * 8, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  float &pbr(int r, int32_t p0, int32_t rep) { return hpbr[((size_t)r * P + p0) * B + rep]; }
+  // pb of population p0 in every replica from its parameter block (a cleared
+  // activity bit is stored as -1)
+  void pbr_from_pop(int32_t p0, int32_t rep, const fs_pop_params &p) {
+    static const uint32_t bit[5] = {FS_ACT_AMPA, FS_ACT_GABA, FS_ACT_NMDA, FS_ACT_ACH, FS_ACT_GCL};
+    for (int r = 0; r < 5; ++r) pbr(r, p0, rep) = (p.act & bit[r]) ? p.pb[r] : -1.0f;
+  }
+
   void update_ext_mask() {
     int32_t m = 0;
-    for (const fs_pop_params &p : hpop)
-      for (int r = 0; r < 5; ++r)
-        if (p.pb[r] > 0.0f) m |= 1 << r;
+    for (size_t i = 0; i < hpbr.size(); ++i)
+      if (hpbr[i] > 0.0f) m |= 1 << (int)(i / ((size_t)P * B));
     if ((m | net.ext_mask) != net.ext_mask) {
       net.ext_mask |= m;
       graph_dirty = true;
     }
   }
 
-  explicit Engine(const fs_net_desc &d) {
-    N = d.n_neuron;
+  Engine(const fs_net_desc &d, int32_t n_rep) {
+    N0 = d.n_neuron;
     P = d.n_pop;
-    n_pop = P;
+    B = n_rep;
+    replicas = B;
     dt = d.dt;
     precision = d.precision;
     record = d.record_spikes != 0;
-    if (N <= 0 || P <= 0) throw std::invalid_argument("network needs neurons and populations");
-    if (N >= (1 << 24)) throw std::invalid_argument("at most 2^24-1 neurons per device");
+    if (N0 <= 0 || P <= 0) throw std::invalid_argument("network needs neurons and populations");
+    if (B < 1) throw std::invalid_argument("replicas must be >= 1");
+    if ((int64_t)N0 * B >= (1 << 24)) throw std::invalid_argument("at most 2^24-1 neurons per device");
+    N = N0 * B;
+    n_pop = P * B;
     HIPCHECK(hipStreamCreate(&stream));
     for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
     hpop.assign(d.pops, d.pops + P);
-    for (int32_t i = 0; i < N; ++i)
+    for (int32_t i = 0; i < N0; ++i)
       if (d.neuron_pop[i] >= (uint32_t)P) throw std::invalid_argument("neuron_pop out of range");
     // ---------------- synapses: classify, validate, CSR by pre / by post
-    std::vector<uint64_t> cnt_p(N + 1, 0), cnt_n(N + 1, 0);
+    std::vector<uint64_t> cnt_p(N0 + 1, 0), cnt_n(N0 + 1, 0);
     int32_t maxd_p = 0, maxd_n = 0, mind_all = 1 << 30;
     int32_t lin_mask = 0;
     // presynaptic gate parameters per population (targets' receptor params)
@@ -173,7 +200,7 @@ struct Engine final : EngineBase {
       const uint32_t a = d.pre[s], b = d.post[s];
       const uint8_t r = d.receptor[s];
       const int32_t dl = d.delay ? d.delay[s] : 0;
-      if (a >= (uint32_t)N || b >= (uint32_t)N) throw std::invalid_argument("synapse endpoint out of range");
+      if (a >= (uint32_t)N0 || b >= (uint32_t)N0) throw std::invalid_argument("synapse endpoint out of range");
       if (r > FS_GCL) throw std::invalid_argument("synapse receptor id out of range");
       const uint32_t pp = d.neuron_pop[a];
       const fs_pop_params &q = hpop[d.neuron_pop[b]];
@@ -205,9 +232,9 @@ struct Engine final : EngineBase {
               "one presynaptic linear gate per population is supported");
       }
     }
-    for (int32_t i = 0; i < N; ++i) { cnt_p[i + 1] += cnt_p[i]; cnt_n[i + 1] += cnt_n[i]; }
-    syn_lin = (int64_t)cnt_p[N];
-    syn_nmda = (int64_t)cnt_n[N];
+    for (int32_t i = 0; i < N0; ++i) { cnt_p[i + 1] += cnt_p[i]; cnt_n[i + 1] += cnt_n[i]; }
+    syn_lin = (int64_t)cnt_p[N0];
+    syn_nmda = (int64_t)cnt_n[N0];
     std::vector<uint32_t> meta_p(syn_lin), meta_n(syn_nmda);
     std::vector<float> w_p(syn_lin), w_n(syn_nmda);
     {
@@ -252,10 +279,26 @@ struct Engine final : EngineBase {
     net.has_nmda_syn = syn_nmda > 0;
     net.ext_mode = d.ext_mode;
     net.words = (N + 31) / 32;
-    b_npop.alloc(n * 4);
-    HIPCHECK(hipMemcpy(b_npop.p, d.neuron_pop, n * 4, hipMemcpyHostToDevice));
+    net.B = B;
+    net.N0 = N0;
+    net.P0 = P;
+    const size_t n0 = (size_t)N0;
+    b_npop.alloc(n0 * 4);
+    HIPCHECK(hipMemcpy(b_npop.p, d.neuron_pop, n0 * 4, hipMemcpyHostToDevice));
     b_pops.alloc(sizeof(fsk::DevPop<T>) * P);
     upload_pops();
+    // per-replica tables: pb from the population blocks, unit gains, and
+    // stream keys spaced by the 64-bit golden ratio (replica 0 keeps the seed)
+    hpbr.assign((size_t)5 * P * B, 0.0f);
+    for (int32_t rep = 0; rep < B; ++rep)
+      for (int32_t i = 0; i < P; ++i) pbr_from_pop(i, rep, hpop[i]);
+    hgain.assign((size_t)6 * B, 1.0f);
+    hskey.resize(B);
+    for (int32_t rep = 0; rep < B; ++rep) hskey[rep] = (uint64_t)rep * 0x9E3779B97F4A7C15ull;
+    b_pbr.alloc(hpbr.size() * 4);
+    b_gain.alloc(hgain.size() * 4);
+    b_skey.alloc(hskey.size() * 8);
+    upload_replica_tables();
     update_ext_mask();
     b_V.alloc(n * sizeof(T));
     b_cs.alloc(n * 4);
@@ -269,16 +312,16 @@ struct Engine final : EngineBase {
     b_elin.alloc((size_t)nlin * n * sizeof(T));
     b_enx.alloc(n * sizeof(T));
     b_ens.alloc(n * sizeof(T));
-    b_rp_p.alloc((n + 1) * 8);
-    HIPCHECK(hipMemcpy(b_rp_p.p, cnt_p.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+    b_rp_p.alloc((n0 + 1) * 8);
+    HIPCHECK(hipMemcpy(b_rp_p.p, cnt_p.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
     b_meta_p.alloc(syn_lin * 4);
     b_w_p.alloc(syn_lin * 4);
     if (syn_lin) {
       HIPCHECK(hipMemcpy(b_meta_p.p, meta_p.data(), syn_lin * 4, hipMemcpyHostToDevice));
       HIPCHECK(hipMemcpy(b_w_p.p, w_p.data(), syn_lin * 4, hipMemcpyHostToDevice));
     }
-    b_rp_n.alloc((n + 1) * 8);
-    HIPCHECK(hipMemcpy(b_rp_n.p, cnt_n.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+    b_rp_n.alloc((n0 + 1) * 8);
+    HIPCHECK(hipMemcpy(b_rp_n.p, cnt_n.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
     b_meta_n.alloc(syn_nmda * 4);
     b_w_n.alloc(syn_nmda * 4);
     if (syn_nmda) {
@@ -286,18 +329,18 @@ struct Engine final : EngineBase {
       HIPCHECK(hipMemcpy(b_w_n.p, w_n.data(), syn_nmda * 4, hipMemcpyHostToDevice));
     }
     {
-      std::vector<uint32_t> nout(n, 0u);
+      std::vector<uint32_t> nout(n0, 0u);
       for (int64_t s = 0; s < d.n_syn; ++s)
         if (d.receptor[s] == FS_NMDA) nout[d.pre[s]]++;
-      b_nmda_out.alloc(n * 4);
-      HIPCHECK(hipMemcpy(b_nmda_out.p, nout.data(), n * 4, hipMemcpyHostToDevice));
+      b_nmda_out.alloc(n0 * 4);
+      HIPCHECK(hipMemcpy(b_nmda_out.p, nout.data(), n0 * 4, hipMemcpyHostToDevice));
     }
     b_total.alloc(4);
     b_overflow.alloc(4);
     b_lcnt.alloc(4 * 4);
     b_lbase.alloc(4 * 4);
     b_t0.alloc(8);
-    b_popcnt.alloc((size_t)P * 8);
+    b_popcnt.alloc((size_t)P * B * 8);
     b_counters.alloc(3 * 8);
     b_tmp.alloc(n * sizeof(T));
     net.npop = b_npop.as<uint32_t>();
@@ -327,6 +370,9 @@ struct Engine final : EngineBase {
     net.t0 = b_t0.as<int64_t>();
     net.pop_cnt = b_popcnt.as<unsigned long long>();
     net.counters = b_counters.as<unsigned long long>();
+    net.pbr = b_pbr.as<float>();
+    net.gainr = b_gain.as<float>();
+    net.skey = b_skey.as<uint64_t>();
     net.ext_bits = nullptr;
     net.ext_first = 0;
     net.ext_steps = 0;
@@ -336,7 +382,7 @@ struct Engine final : EngineBase {
     sched.push_block = 256;
     sched.steps_per_graph = 100;
     sched.use_graph = 1;
-    sched.fused = 0;
+    sched.fused = 1;
     apply_schedule();
     reset(0);
   }
@@ -459,7 +505,7 @@ struct Engine final : EngineBase {
     fill(b_elin.p, 4 * n, T(0));
     fill(b_enx.p, n, T(0));
     fill(b_ens.p, n, T(0));
-    HIPCHECK(hipMemsetAsync(b_popcnt.p, 0, (size_t)P * 8, stream));
+    HIPCHECK(hipMemsetAsync(b_popcnt.p, 0, (size_t)P * B * 8, stream));
     HIPCHECK(hipMemsetAsync(b_counters.p, 0, 24, stream));
     HIPCHECK(hipMemsetAsync(b_overflow.p, 0, 4, stream));
     HIPCHECK(hipMemsetAsync(b_lcnt.p, 0, 16, stream));
@@ -471,16 +517,53 @@ struct Engine final : EngineBase {
     std::memset(ctr_last, 0, sizeof(ctr_last));
   }
 
-  void set_pop(int32_t i, const fs_pop_params &p) override {
-    if (i < 0 || i >= P) throw std::invalid_argument("population index out of range");
+  // Population g = replica * P + p0.  The parameters of p0 are shared by
+  // every replica; the external probabilities and activity bits are set for
+  // replica g / P only.
+  void set_pop(int32_t g, const fs_pop_params &p) override {
+    if (g < 0 || g >= P * B) throw std::invalid_argument("population index out of range");
+    const int32_t rep = g / P, i = g - rep * P;
     hpop[i] = p;
     dpop[i] = to_dev(p, dpop[i]);
     HIPCHECK(hipMemcpyAsync(b_pops.as<fsk::DevPop<T>>() + i, &dpop[i], sizeof(fsk::DevPop<T>),
                             hipMemcpyHostToDevice, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
+    pbr_from_pop(i, rep, p);
+    upload_replica_tables();
     update_ext_mask();
   }
-  const fs_pop_params &pop(int32_t i) const override { return hpop.at(i); }
+  fs_pop_params pop(int32_t g) const override {
+    if (g < 0 || g >= P * B) throw std::invalid_argument("population index out of range");
+    const int32_t rep = g / P, i = g - rep * P;
+    fs_pop_params p = hpop[i];
+    static const uint32_t bit[5] = {FS_ACT_AMPA, FS_ACT_GABA, FS_ACT_NMDA, FS_ACT_ACH, FS_ACT_GCL};
+    for (int r = 0; r < 5; ++r) {
+      const float v = hpbr[((size_t)r * P + i) * B + rep];
+      if (v < 0.0f) p.act &= ~bit[r];
+      else { p.act |= bit[r]; p.pb[r] = v; }
+    }
+    return p;
+  }
+  // ChangeExtFreq of one receptor of population g (code/NetSimGnl2.cpp:60-75):
+  // on = false clears the activity bit, else pb is the new probability
+  void set_ext(int32_t g, int32_t r, float pb, bool on) override {
+    if (g < 0 || g >= P * B) throw std::invalid_argument("population index out of range");
+    const int32_t rep = g / P, i = g - rep * P;
+    pbr(r, i, rep) = on ? pb : -1.0f;
+    if (B == 1) {  // keep the parameter block in step for fs_get/set round trips
+      static const uint32_t bit[5] = {FS_ACT_AMPA, FS_ACT_GABA, FS_ACT_NMDA, FS_ACT_ACH, FS_ACT_GCL};
+      if (on) { hpop[i].act |= bit[r]; hpop[i].pb[r] = pb; }
+      else hpop[i].act &= ~bit[r];
+    }
+    upload_replica_tables();
+    update_ext_mask();
+  }
+  void set_replica(int32_t rep, const fs_replica_params &p) override {
+    if (rep < 0 || rep >= B) throw std::invalid_argument("replica index out of range");
+    for (int r = 0; r < 5; ++r) hgain[(size_t)r * B + rep] = p.gain[r];
+    hgain[(size_t)5 * B + rep] = p.ext_gain;
+    hskey[rep] = p.stream_key;
+    upload_replica_tables();
+  }
 
   void load_ext(const uint32_t *bits, int64_t first, int64_t nsteps) override {
     if (net.ext_mode != FS_EXT_REPLAY) throw std::invalid_argument("engine not created in replay mode");
@@ -500,8 +583,9 @@ struct Engine final : EngineBase {
     }
     int64_t written = 0;
     if (off) off[0] = 0;
-    t_upd = t_del = 0;
+    t_upd = t_del = t_graph = 0;
     n_timed = 0;
+    n_graph_steps = 0;
     HIPCHECK(hipMemsetAsync(b_counters.p, 0, 24, stream));
     int64_t done = 0;
     while (done < nsteps) {
@@ -514,7 +598,9 @@ struct Engine final : EngineBase {
       if (K > 1) {
         if (sched.use_graph) {
           if (graph_dirty || graph_k != K) build_graph(K);
+          if (timing) HIPCHECK(hipEventRecord(ev[3], stream));
           HIPCHECK(hipGraphLaunch(gexec, stream));
+          if (timing) HIPCHECK(hipEventRecord(ev[4], stream));
         } else {
           for (int32_t k = 1; k < K; ++k) launch_step(k, false);
           if (fused()) launch_tail(K);
@@ -530,6 +616,12 @@ struct Engine final : EngineBase {
         t_upd += a;
         t_del += b;
         n_timed++;
+        if (K > 1 && sched.use_graph) {
+          float c = 0;
+          HIPCHECK(hipEventElapsedTime(&c, ev[3], ev[4]));
+          t_graph += c;
+          n_graph_steps += K - 1;
+        }
       }
       uint32_t ovf = 0;
       HIPCHECK(hipMemcpy(&ovf, b_overflow.p, 4, hipMemcpyDeviceToHost));
@@ -553,7 +645,7 @@ struct Engine final : EngineBase {
     }
     HIPCHECK(hipMemcpy(ctr_last, b_counters.p, 24, hipMemcpyDeviceToHost));
     // every step gathers each NMDA synapse's term once
-    ctr_last[3] = (unsigned long long)syn_nmda * (unsigned long long)nsteps;
+    ctr_last[3] = (unsigned long long)syn_nmda * (unsigned long long)B * (unsigned long long)nsteps;
   }
 
   void get_state(int32_t f, double *out) override {
@@ -614,13 +706,17 @@ struct Engine final : EngineBase {
     }
   }
 
+  // device layout [P][B] (coalesced atomics), returned replica-major
   void pop_counts(uint64_t *out) override {
-    HIPCHECK(hipMemcpy(out, b_popcnt.p, (size_t)P * 8, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> c((size_t)P * B);
+    HIPCHECK(hipMemcpy(c.data(), b_popcnt.p, c.size() * 8, hipMemcpyDeviceToHost));
+    for (int32_t rep = 0; rep < B; ++rep)
+      for (int32_t i = 0; i < P; ++i) out[(size_t)rep * P + i] = c[(size_t)i * B + rep];
   }
   void times(double *u, double *p, double *g, int64_t *n) override {
     if (u) *u = n_timed ? t_upd / n_timed : 0.0;
     if (p) *p = n_timed ? t_del / n_timed : 0.0;
-    if (g) *g = 0.0;
+    if (g) *g = n_graph_steps ? t_graph / n_graph_steps : 0.0;
     if (n) *n = n_timed;
   }
   void counters(int64_t *out) override {
@@ -657,8 +753,23 @@ int fs_create(const fs_net_desc *d, fs_handle **out) {
   if (!(d->dt > 0.0f)) return fs::set_error(FS_ERR_ARG, "dt must be positive");
   ENGINE_CALL({
     auto h = std::make_unique<fs_handle>();
-    if (d->precision == FS_FP64) h->e.reset(new Engine<double>(*d));
-    else if (d->precision == FS_FP32) h->e.reset(new Engine<float>(*d));
+    if (d->precision == FS_FP64) h->e.reset(new Engine<double>(*d, 1));
+    else if (d->precision == FS_FP32) h->e.reset(new Engine<float>(*d, 1));
+    else throw std::invalid_argument("precision must be FS_FP32 or FS_FP64");
+    *out = h.release();
+  })
+}
+
+int fs_create_replicated(const fs_net_desc *d, int32_t replicas, fs_handle **out) {
+  if (!d || !out) return fs::set_error(FS_ERR_ARG, "null argument");
+  if (!d->neuron_pop || !d->pops || (d->n_syn > 0 && (!d->pre || !d->post || !d->w || !d->receptor)))
+    return fs::set_error(FS_ERR_ARG, "incomplete network description");
+  if (!(d->dt > 0.0f)) return fs::set_error(FS_ERR_ARG, "dt must be positive");
+  if (replicas < 1) return fs::set_error(FS_ERR_ARG, "replicas must be >= 1");
+  ENGINE_CALL({
+    auto h = std::make_unique<fs_handle>();
+    if (d->precision == FS_FP64) h->e.reset(new Engine<double>(*d, replicas));
+    else if (d->precision == FS_FP32) h->e.reset(new Engine<float>(*d, replicas));
     else throw std::invalid_argument("precision must be FS_FP32 or FS_FP64");
     *out = h.release();
   })
@@ -678,15 +789,14 @@ int fs_set_ext_rate(fs_handle *h, int32_t pop, int32_t receptor, float hz) {
   if (!h) return fs::set_error(FS_ERR_ARG, "null handle");
   if (receptor < 0 || receptor > FS_GCL) return fs::set_error(FS_ERR_ARG, "receptor out of range");
   ENGINE_CALL({
-    fs_pop_params p = h->e->pop(pop);
-    static const uint32_t bit[5] = {FS_ACT_AMPA, FS_ACT_GABA, FS_ACT_NMDA, FS_ACT_ACH, FS_ACT_GCL};
-    if (hz == 0.0f) p.act &= ~bit[receptor];
-    else {
-      p.act |= bit[receptor];
-      p.pb[receptor] = (float)(hz * h->e->dt * 1e-3f);
-    }
-    h->e->set_pop(pop, p);
+    if (hz == 0.0f) h->e->set_ext(pop, receptor, 0.0f, false);
+    else h->e->set_ext(pop, receptor, (float)(hz * h->e->dt * 1e-3f), true);
   })
+}
+
+int fs_set_replica(fs_handle *h, int32_t replica, const fs_replica_params *p) {
+  if (!h || !p) return fs::set_error(FS_ERR_ARG, "null argument");
+  ENGINE_CALL(h->e->set_replica(replica, *p))
 }
 
 int fs_set_injection(fs_handle *h, int32_t pop, float mean, float sd) {
@@ -735,6 +845,7 @@ int fs_pop_spike_counts(fs_handle *h, uint64_t *out) {
 int fs_get_schedule(fs_handle *h, fs_schedule *s) {
   if (!h || !s) return fs::set_error(FS_ERR_ARG, "null argument");
   *s = h->e->sched;
+  s->fused = h->e->fused_active() ? 1 : 0;
   return FS_OK;
 }
 

@@ -17,6 +17,14 @@
 // where s_j is the presynaptic gate shared by all of j's synapses; NMDA keeps
 // the per-step gather G_N(p) = sum_j g_jp s_j(t-d_jp) because its gate
 // saturates (the (1-s) term) and has no push form.
+//
+// Replicas: B copies of one network share its synapse tables.  Their state is
+// interleaved, global neuron n = local * B + replica, so a wave of B >= 64
+// consecutive neurons is one local neuron in 64 copies: the CSR row of that
+// neuron is read once (same address in every lane) and the presynaptic state
+// it points at is read as contiguous B-wide vectors.  Per copy: recurrent
+// gains by receptor, an external efficacy gain, external event probabilities
+// per (population, receptor) and the key of the counter-based random stream.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -45,9 +53,8 @@ struct DevPop {
   T pre_df;             // presynaptic linear gate decay (targets' dfst)
   T pre_dfxt, pre_dfst, pre_efas;  // presynaptic NMDA gate (targets' params)
   T inj_mean, inj_std;
-  float pb[5];
   int32_t rfped, spidy;
-  uint32_t act;
+  uint32_t act;             // only ACT_INJ is read on the device
   uint32_t has_lin_out, has_nmda_out;
   uint32_t pad;
 };
@@ -62,8 +69,15 @@ struct Net {
   int32_t has_nmda_syn;
   int32_t ext_mode;     // 0 philox, 1 replay
   int32_t words;        // ceil(N/32)
-  const uint32_t *npop;
-  const DevPop<T> *pops;
+  int32_t B;            // replicas (interleaved: n = local * B + replica)
+  int32_t N0;           // neurons of one replica (N = N0 * B)
+  int32_t P0;           // populations of one replica
+  const uint32_t *npop; // [N0] population of each local neuron
+  const DevPop<T> *pops;// [P0]
+  const float *pbr;     // [5][P0][B] external event probability per step; < 0: the
+                        // receptor's activity bit is cleared (no draw, gate frozen)
+  const float *gainr;   // [6][B] recurrent gain by receptor id, then external efficacy gain
+  const uint64_t *skey; // [B] random-stream key of each replica (xor into the seed)
   T *V;
   uint32_t *cs;
   T *G;                 // [4][N] running linear conductances
@@ -97,7 +111,7 @@ struct Net {
   uint32_t *lbase;          // [3] fused path: record offset of the list of step k%3
   uint64_t rec_cap;
   const int64_t *t0;        // [1] first step of the batch
-  unsigned long long *pop_cnt;  // [P]
+  unsigned long long *pop_cnt;  // [P0][B]
   unsigned long long *counters; // [3]: spikes, linear pushes, NMDA out-synapses of spikers
   int32_t gather_lanes;
   int32_t push_waves;       // waves of k_deliver dedicated to the push
@@ -130,9 +144,13 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
   const bool active = n < net.N;
   bool spk = false;
   T a_push = T(0);
+  uint32_t pid = 0;  // population x replica index of the spike counters
   if (active) {
     const int32_t N = net.N;
-    const DevPop<T> &P = net.pops[net.npop[n]];
+    const int32_t nl = n / net.B, rep = n - nl * net.B;
+    const uint32_t p0 = net.npop[nl];
+    pid = p0 * (uint32_t)net.B + (uint32_t)rep;
+    const DevPop<T> &P = net.pops[p0];
     uint32_t cs = net.cs[n];
     uint32_t cnt = cs & CNT_MASK;
     T v = net.V[n];
@@ -153,43 +171,54 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
     }
     if (net.has_nmda_syn) g[2] = gn_given ? gn_in : net.gn[n];
     // (2) external Poisson gates, order A, G, ACh, GluCl, NMDA (BgInit)
+    // (counter = local neuron and step, key = seed ^ replica key: a replica
+    // draws the same stream as the same network run alone with that key)
     const uint32_t act = P.act;
+    const uint64_t key = net.seed ^ net.skey[rep];
     uint32_t rnd[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
     if ((!REPLAY && net.ext_mask) || (act & ACT_INJ)) {
-      uint32_t c[4] = {(uint32_t)n, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0u};
-      philox10(c, (uint32_t)net.seed, (uint32_t)(net.seed >> 32));
+      uint32_t c[4] = {(uint32_t)nl, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0u};
+      philox10(c, (uint32_t)key, (uint32_t)(key >> 32));
       rnd[0] = c[0]; rnd[1] = c[1]; rnd[2] = c[2]; rnd[3] = c[3];
       if ((net.ext_mask & (1 << 4)) || (act & ACT_INJ)) {
-        uint32_t e[4] = {(uint32_t)n, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 1u};
-        philox10(e, (uint32_t)net.seed, (uint32_t)(net.seed >> 32));
+        uint32_t e[4] = {(uint32_t)nl, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 1u};
+        philox10(e, (uint32_t)key, (uint32_t)(key >> 32));
         rnd[4] = e[0]; rnd[5] = e[1]; rnd[6] = e[2]; rnd[7] = e[3];
       }
     }
-    auto draw = [&](int rid) -> bool {
+    const float *pbp = net.pbr + (size_t)p0 * net.B + rep;
+    const size_t pbs = (size_t)net.P0 * net.B;  // receptor stride of pbr
+    auto draw = [&](int rid, float pb) -> bool {
       if (REPLAY) {
         const int64_t s = t - net.ext_first;
         if (s < 0 || s >= net.ext_steps) return false;
         return (net.ext_bits[((size_t)s * 5 + rid) * net.words + (n >> 5)] >> (n & 31)) & 1u;
       }
-      return P.pb[rid] > u24(rnd[rid]);
+      return pb > u24(rnd[rid]);
     };
+    const T eg = T(net.gainr[5 * net.B + rep]);
 #pragma unroll
     for (int li = 0; li < 4; ++li) {
       const int rid = lin_rid(li);
-      if (!(net.ext_mask & (1 << rid)) || !(act & lin_act(li))) continue;
+      if (!(net.ext_mask & (1 << rid))) continue;
+      const float pb = pbp[rid * pbs];
+      if (pb < 0.0f) continue;
       T *ep = net.elin + (size_t)li * N + n;
       T e = *ep;
-      e = draw(rid) ? e + T(1) : e * P.dfst[rid];
+      e = draw(rid, pb) ? e + T(1) : e * P.dfst[rid];
       *ep = e;
-      g[rid] = g[rid] + P.gext[rid] * e;
+      g[rid] = g[rid] + P.gext[rid] * eg * e;
     }
-    if ((net.ext_mask & (1 << 2)) && (act & ACT_NMDA)) {
-      T x = net.enx[n], s = net.ens[n];
-      x = draw(2) ? x + T(1) : x * P.dfxt_n;
-      s = P.dfst[2] * s + P.efas_n * x * (T(1) - s);
-      net.enx[n] = x;
-      net.ens[n] = s;
-      g[2] = g[2] + P.gext[2] * s;
+    if (net.ext_mask & (1 << 2)) {
+      const float pb = pbp[2 * pbs];
+      if (pb >= 0.0f) {
+        T x = net.enx[n], s = net.ens[n];
+        x = draw(2, pb) ? x + T(1) : x * P.dfxt_n;
+        s = P.dfst[2] * s + P.efas_n * x * (T(1) - s);
+        net.enx[n] = x;
+        net.ens[n] = s;
+        g[2] = g[2] + P.gext[2] * eg * s;
+      }
     }
     T isyn = T(0);
     if (act & ACT_INJ) {
@@ -271,7 +300,7 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
         atomicOr(net.overflow, 1u);
       }
       // per-population spike count, aggregated when the wave shares one population
-      const uint32_t p = net.npop[n];
+      const uint32_t p = pid;
       const uint32_t p0 = __shfl(p, leader);
       const bool same = __ballot(spk && p != p0) == 0ull;
       if (same) {
@@ -303,19 +332,23 @@ __device__ __forceinline__ void push_spikes(const Net<T> &net, uint32_t start, u
                                             int64_t ts, int gwave, int nwaves, int lane,
                                             unsigned long long &events,
                                             unsigned long long &nout) {
-  const int32_t N = net.N;
+  const int32_t N = net.N, B = net.B;
   for (uint32_t i = start + (uint32_t)gwave; i < end; i += (uint32_t)nwaves) {
     const uint32_t j = net.rec_ids[i];
+    const uint32_t jl = j / (uint32_t)B, rep = j - jl * (uint32_t)B;
     const T a = net.rec_a[i];
-    const uint64_t r0 = net.rp_p[j], r1 = net.rp_p[j + 1];
-    if (lane == 0) { events += r1 - r0; nout += net.nmda_out[j]; }
+    const T ga = a * T(net.gainr[0 * B + rep]), gg = a * T(net.gainr[1 * B + rep]),
+            gh = a * T(net.gainr[3 * B + rep]), gl = a * T(net.gainr[4 * B + rep]);
+    const uint64_t r0 = net.rp_p[jl], r1 = net.rp_p[jl + 1];
+    if (lane == 0) { events += r1 - r0; nout += net.nmda_out[jl]; }
     for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
       const uint32_t m = net.meta_p[e];
-      const uint32_t col = m & 0x00FFFFFFu;
+      const uint32_t col = (m & 0x00FFFFFFu) * (uint32_t)B + rep;
       const uint32_t d = (m >> 24) & 63u;
       const uint32_t li = m >> 30;
+      const T s = li == 0 ? ga : li == 1 ? gg : li == 2 ? gh : gl;
       const int32_t slot = (int32_t)((ts + (int64_t)d) % net.D);
-      atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(net.w_p[e]) * a);
+      atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(net.w_p[e]) * s);
     }
   }
 }
@@ -344,14 +377,17 @@ template <typename T>
 __device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int64_t ts, int L, int sl) {
   T acc = T(0);
   if (p < net.N) {
-    const uint64_t r0 = net.rp_n[p], r1 = net.rp_n[p + 1];
+    const uint32_t B = (uint32_t)net.B;
+    const uint32_t pl = (uint32_t)p / B, rep = (uint32_t)p - pl * B;
+    const uint64_t r0 = net.rp_n[pl], r1 = net.rp_n[pl + 1];
     for (uint64_t e = r0 + (uint64_t)sl; e < r1; e += (uint64_t)L) {
       const uint32_t m = net.meta_n[e];
-      const uint32_t j = m & 0x00FFFFFFu;
+      const uint32_t j = (m & 0x00FFFFFFu) * B + rep;
       const uint32_t d = m >> 24;
       const int32_t hs = (int32_t)(((ts - (int64_t)d) % net.DN + net.DN) % net.DN);
       acc += T(net.w_n[e]) * net.shist[(size_t)hs * net.N + j];
     }
+    acc *= T(net.gainr[2 * B + rep]);
   }
   for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
   return acc;
@@ -441,20 +477,7 @@ __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
     const uint32_t start = net.cum[k];
     const uint32_t end = end_raw < net.rec_cap ? end_raw : (uint32_t)net.rec_cap;
     if (blockIdx.x == 0 && threadIdx.x == 0) net.cum[k + 1] = end;
-    for (uint32_t i = start + (uint32_t)gwave; i < end; i += (uint32_t)net.push_waves) {
-      const uint32_t j = net.rec_ids[i];
-      const T a = net.rec_a[i];
-      const uint64_t r0 = net.rp_p[j], r1 = net.rp_p[j + 1];
-      if (lane == 0) { my_events += r1 - r0; my_nout += net.nmda_out[j]; }
-      for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
-        const uint32_t m = net.meta_p[e];
-        const uint32_t col = m & 0x00FFFFFFu;
-        const uint32_t d = (m >> 24) & 63u;
-        const uint32_t li = m >> 30;
-        const int32_t slot = (int32_t)((t + (int64_t)d) % net.D);
-        atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(net.w_p[e]) * a);
-      }
-    }
+    push_spikes(net, start, end, t, gwave, net.push_waves, lane, my_events, my_nout);
   } else if (net.has_nmda_syn) {
     // ---- NMDA gather over CSR-by-post, L lanes per row
     const int L = net.gather_lanes;
@@ -464,18 +487,7 @@ __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
     const int sub = lane / L, sl = lane % L;
     const int64_t stride = (int64_t)gwaves * groups_per_wave;
     for (int64_t p = (int64_t)gw * groups_per_wave + sub; p - sub < N; p += stride) {
-      T acc = T(0);
-      if (p < N) {
-        const uint64_t r0 = net.rp_n[p], r1 = net.rp_n[p + 1];
-        for (uint64_t e = r0 + (uint64_t)sl; e < r1; e += (uint64_t)L) {
-          const uint32_t m = net.meta_n[e];
-          const uint32_t j = m & 0x00FFFFFFu;
-          const uint32_t d = m >> 24;
-          const int32_t hs = (int32_t)(((t - (int64_t)d) % net.DN + net.DN) % net.DN);
-          acc += T(net.w_n[e]) * net.shist[(size_t)hs * N + j];
-        }
-      }
-      for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+      const T acc = gather_row(net, p, t, L, sl);
       if (p < N && sl == 0) net.gn[p] = acc;
     }
   }
@@ -507,7 +519,7 @@ template <typename T>
 __global__ void k_read_g(Net<T> net, int64_t t, int li, T *out) {
   const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= net.N) return;
-  const DevPop<T> &P = net.pops[net.npop[n]];
+  const DevPop<T> &P = net.pops[net.npop[n / net.B]];
   const int32_t slot = (int32_t)(((t % net.D) + net.D) % net.D);
   out[n] = P.dfst[lin_rid(li)] * net.G[(size_t)li * net.N + n] +
            net.ring[((size_t)li * net.D + slot) * net.N + n];

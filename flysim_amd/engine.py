@@ -166,25 +166,48 @@ class Conf:
 
 
 class Engine:
-    """One network (or a batch of independent instances) resident on the GPU."""
+    """One network (or a batch of independent instances) resident on the GPU.
+
+    With ``replicas = B > 1`` the handle holds B copies of the network that
+    share its synapse tables (parameter sweeps): neuron ids are interleaved
+    (global = local * B + replica) and population ids replica-major
+    (global = replica * n_pop_base + pop)."""
 
     def __init__(self, net, precision: int = A.FS_FP32, ext_mode: int = A.FS_EXT_PHILOX,
-                 record: bool = True):
+                 record: bool = True, replicas: int = 1):
         self._keep = net
         if isinstance(net, Conf):
             d = net.desc()
             d.precision, d.ext_mode, d.record_spikes = precision, ext_mode, int(record)
-            self.n = net.n_neuron
-            self.n_pop = net.n_pop
+            n0, p0 = net.n_neuron, net.n_pop
         else:
             d = net.desc(precision, ext_mode, record)
-            self.n = net.n
-            self.n_pop = len(net.pops)
+            n0, p0 = net.n, len(net.pops)
+        self.replicas = int(replicas)
+        self.n_base, self.n_pop_base = n0, p0
+        self.n = n0 * self.replicas
+        self.n_pop = p0 * self.replicas
         self.dt = float(d.dt)
         h = C.c_void_p()
-        A.check(A.lib().fs_create(C.byref(d), C.byref(h)))
+        if self.replicas == 1:
+            A.check(A.lib().fs_create(C.byref(d), C.byref(h)))
+        else:
+            A.check(A.lib().fs_create_replicated(C.byref(d), self.replicas, C.byref(h)))
         self._h = h
         self.precision = precision
+
+    def set_replica(self, replica: int, gain=(1.0, 1.0, 1.0, 1.0, 1.0), ext_gain: float = 1.0,
+                    stream_key: Optional[int] = None) -> None:
+        """Per-copy recurrent gains by receptor id, external efficacy gain and
+        random-stream key (default: replica * 0x9E3779B97F4A7C15)."""
+        p = A.ReplicaParams()
+        for i, g in enumerate(gain):
+            p.gain[i] = float(g)
+        p.ext_gain = float(ext_gain)
+        if stream_key is None:
+            stream_key = (replica * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        p.stream_key = int(stream_key) & 0xFFFFFFFFFFFFFFFF
+        A.check(A.lib().fs_set_replica(self._h, int(replica), C.byref(p)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -259,6 +282,11 @@ class Engine:
         A.check(A.lib().fs_get_schedule(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in A.Schedule._fields_}
 
+    @property
+    def fused(self) -> bool:
+        """True while the one-launch-per-step path is in effect."""
+        return bool(self.schedule["fused"])
+
     def set_schedule(self, **kw) -> None:
         cur = self.schedule
         cur.update(kw)
@@ -271,7 +299,8 @@ class Engine:
     def kernel_times(self) -> Dict[str, float]:
         u, p, g, n = C.c_double(), C.c_double(), C.c_double(), C.c_int64()
         A.check(A.lib().fs_kernel_times(self._h, C.byref(u), C.byref(p), C.byref(g), C.byref(n)))
-        return {"update_ms": u.value, "deliver_ms": p.value, "n_timed": n.value}
+        return {"update_ms": u.value, "deliver_ms": p.value, "graph_step_ms": g.value,
+                "n_timed": n.value}
 
     def counters(self) -> Dict[str, int]:
         """Counters of the last run(): spikes, linear receptor-synapses

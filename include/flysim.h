@@ -95,6 +95,28 @@ int fs_derive_params(const fs_pop_conf *conf, float dt, uint32_t act, fs_pop_par
  * device of the calling thread). */
 int fs_create(const fs_net_desc *desc, fs_handle **out);
 
+/* Parameter sweeps over one network (no counterpart in the reference, which
+ * runs one network per process): `replicas` copies of the network of desc in
+ * one handle, sharing its synapse tables.  Neuron ids are interleaved,
+ * global = local * replicas + replica (fs_run ids, fs_get_state order);
+ * population ids are replica-major, global = replica * n_pop + pop
+ * (fs_set_ext_rate, fs_set_pop_params, fs_pop_spike_counts).  Population
+ * parameters other than the external rates and activity bits are shared by
+ * all copies.  replicas = 1 is fs_create. */
+int fs_create_replicated(const fs_net_desc *desc, int32_t replicas, fs_handle **out);
+
+/* Per-copy knobs of a replicated handle: gain[r] multiplies every recurrent
+ * efficacy of receptor r, ext_gain every external efficacy, and stream_key
+ * is xor-ed into the reset seed to key the copy's counter-based stream (the
+ * copy then draws what a single network with that key draws).  Defaults:
+ * gains 1, stream_key = replica * 0x9E3779B97F4A7C15. */
+typedef struct fs_replica_params {
+  float gain[5];
+  float ext_gain;
+  uint64_t stream_key;
+} fs_replica_params;
+int fs_set_replica(fs_handle *h, int32_t replica, const fs_replica_params *p);
+
 /* Replaces EqsGnl2::VarInit + the reseeding of NetSimGnl2::Run
  * (code/EqsGnl2.cpp:339-401, code/NetSimGnl2.cpp:277-323). */
 int fs_reset(fs_handle *h, uint64_t seed);
@@ -139,7 +161,8 @@ int fs_pop_spike_counts(fs_handle *h, uint64_t *out);
 /* Launch schedule (autotuner output): block size of the neuron update,
  * lanes per row of the NMDA gather, block size of the delivery, steps per
  * captured graph, graph on/off, and the one-launch-per-step fused path
- * (used only when every synapse has a delay of at least one step). */
+ * (used only when every synapse has a delay of at least one step;
+ * fs_get_schedule reports fused = 1 only while that path is in effect). */
 typedef struct fs_schedule {
   int32_t update_block, gather_lanes, push_block, steps_per_graph, use_graph, fused;
 } fs_schedule;
@@ -147,8 +170,12 @@ int fs_get_schedule(fs_handle *h, fs_schedule *s);
 int fs_set_schedule(fs_handle *h, const fs_schedule *s);
 
 /* Kernel timing with HIP events on the engine's stream over the last fs_run
- * (enabled by fs_enable_timing): average duration in ms of the update,
- * push and gather kernels, and the number of timed launches. */
+ * (enabled by fs_enable_timing): average duration in ms of the first step of
+ * each captured batch, split into the update (or fused step) kernel and the
+ * delivery kernel, then the average time per step of the graph-replayed
+ * steps (events around each graph launch divided by its steps; for the fused
+ * path that is the per-launch time of its one kernel), and the number of
+ * timed batches. */
 int fs_enable_timing(fs_handle *h, int32_t on);
 int fs_kernel_times(fs_handle *h, double *update_ms, double *push_ms, double *gather_ms,
                     int64_t *n_timed);

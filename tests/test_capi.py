@@ -31,6 +31,7 @@ def test_struct_sizes_match_header_layout():
     # fs_pop_params: 8 floats, 2 ints, 5+5 floats, 4 floats, 5+5 floats, uint, 3 floats
     assert C.sizeof(A.PopParams) == 4 * (8 + 2 + 10 + 4 + 10 + 1 + 3)
     assert C.sizeof(A.Schedule) == 24
+    assert C.sizeof(A.ReplicaParams) == 32
 
 
 def test_derive_params_reference_values():
