@@ -63,6 +63,11 @@ class ReplicaParams(C.Structure):
     _fields_ = [("gain", C.c_float * 5), ("ext_gain", C.c_float), ("stream_key", C.c_uint64)]
 
 
+class Adam(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("weight_decay", C.c_float)]
+
+
 class CliOpts(C.Structure):
     _fields_ = [("dt", C.c_float), ("threads", C.c_int32), ("repeats", C.c_int32),
                 ("stp", C.c_int32), ("std_", C.c_int32), ("ltp", C.c_int32),
@@ -92,7 +97,9 @@ EXPORTS = [
     "fs_kernel_times", "fs_run_counters", "fs_destroy", "fs_last_error", "fs_conf_load",
     "fs_conf_free", "fs_conf_desc", "fs_conf_sizes", "fs_conf_seed", "fs_conf_ext_trains",
     "fs_conf_ref_layout", "fs_conf_write_outputs", "fs_cli_main", "fs_create_replicated",
-    "fs_set_replica",
+    "fs_set_replica", "fs_mlp_create", "fs_mlp_destroy", "fs_mlp_n_params", "fs_mlp_get_params",
+    "fs_mlp_set_params", "fs_mlp_loss_grad", "fs_mlp_predict", "fs_mlp_set_data", "fs_mlp_train",
+    "fs_device_alloc", "fs_device_free", "fs_device_copy",
 ]
 
 _lib = None
@@ -145,6 +152,21 @@ def lib() -> C.CDLL:
             "fs_conf_write_outputs": [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
                                       C.c_char_p],
             "fs_cli_main": [C.c_int, P(C.c_char_p)],
+            "fs_mlp_create": [C.c_void_p, C.c_int32, C.c_uint64, P(C.c_void_p)],
+            "fs_mlp_destroy": [C.c_void_p],
+            "fs_mlp_n_params": [C.c_void_p, P(C.c_int64)],
+            "fs_mlp_get_params": [C.c_void_p, C.c_void_p],
+            "fs_mlp_set_params": [C.c_void_p, C.c_void_p],
+            "fs_mlp_loss_grad": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, P(C.c_float),
+                                 C.c_void_p],
+            "fs_mlp_predict": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
+                               P(C.c_double)],
+            "fs_mlp_set_data": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64],
+            "fs_mlp_train": [C.c_void_p, C.c_int32, C.c_int32, P(Adam), C.c_uint64, C.c_int32,
+                             C.c_void_p],
+            "fs_device_alloc": [C.c_int64, P(C.c_void_p)],
+            "fs_device_free": [C.c_void_p],
+            "fs_device_copy": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -152,6 +174,38 @@ def lib() -> C.CDLL:
             f.restype = C.c_int
         _lib = L
     return _lib
+
+
+class DeviceBuffer:
+    """Device memory owned by the library's HIP runtime (the process must
+    not also initialise another HIP runtime, e.g. PyTorch's bundled one)."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        check(lib().fs_device_alloc(int(nbytes), C.byref(p)))
+        self.ptr = int(p.value or 0)
+        self.nbytes = int(nbytes)
+
+    def upload(self, a) -> None:
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().fs_device_copy(C.c_void_p(self.ptr), C.c_void_p(a.ctypes.data), a.nbytes, 0))
+
+    def download(self, a) -> None:
+        assert a.flags["C_CONTIGUOUS"] and a.nbytes <= self.nbytes
+        check(lib().fs_device_copy(C.c_void_p(a.ctypes.data), C.c_void_p(self.ptr), a.nbytes, 1))
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().fs_device_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def check(rc: int) -> None:

@@ -96,6 +96,7 @@ struct Engine final : EngineBase {
   hipGraph_t graph = nullptr;
   int32_t graph_k = -1;
   bool graph_dirty = true;
+  bool tables_dirty = false;  // per-replica tables changed since the last upload
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double t_upd = 0, t_del = 0, t_graph = 0;
   int64_t n_graph_steps = 0;
@@ -160,6 +161,14 @@ struct Engine final : EngineBase {
     for (int r = 0; r < 5; ++r) pbr(r, p0, rep) = (p.act & bit[r]) ? p.pb[r] : -1.0f;
   }
 
+  // receptor r gets the external-input code path once any population of any
+  // replica has pb > 0 (a graph argument: changing it re-captures)
+  void note_pb(int r, float pb) {
+    if (pb > 0.0f && !(net.ext_mask & (1 << r))) {
+      net.ext_mask |= 1 << r;
+      graph_dirty = true;
+    }
+  }
   void update_ext_mask() {
     int32_t m = 0;
     for (size_t i = 0; i < hpbr.size(); ++i)
@@ -528,8 +537,8 @@ struct Engine final : EngineBase {
     HIPCHECK(hipMemcpyAsync(b_pops.as<fsk::DevPop<T>>() + i, &dpop[i], sizeof(fsk::DevPop<T>),
                             hipMemcpyHostToDevice, stream));
     pbr_from_pop(i, rep, p);
-    upload_replica_tables();
-    update_ext_mask();
+    for (int r = 0; r < 5; ++r) note_pb(r, pbr(r, i, rep));
+    tables_dirty = true;
   }
   fs_pop_params pop(int32_t g) const override {
     if (g < 0 || g >= P * B) throw std::invalid_argument("population index out of range");
@@ -554,15 +563,15 @@ struct Engine final : EngineBase {
       if (on) { hpop[i].act |= bit[r]; hpop[i].pb[r] = pb; }
       else hpop[i].act &= ~bit[r];
     }
-    upload_replica_tables();
-    update_ext_mask();
+    if (on) note_pb(r, pb);
+    tables_dirty = true;
   }
   void set_replica(int32_t rep, const fs_replica_params &p) override {
     if (rep < 0 || rep >= B) throw std::invalid_argument("replica index out of range");
     for (int r = 0; r < 5; ++r) hgain[(size_t)r * B + rep] = p.gain[r];
     hgain[(size_t)5 * B + rep] = p.ext_gain;
     hskey[rep] = p.stream_key;
-    upload_replica_tables();
+    tables_dirty = true;
   }
 
   void load_ext(const uint32_t *bits, int64_t first, int64_t nsteps) override {
@@ -580,6 +589,10 @@ struct Engine final : EngineBase {
     if (net.ext_mode == FS_EXT_REPLAY) {
       if (!net.ext_bits || step < net.ext_first || step + nsteps > net.ext_first + net.ext_steps)
         throw std::invalid_argument("replayed external trains do not cover the requested steps");
+    }
+    if (tables_dirty) {
+      upload_replica_tables();
+      tables_dirty = false;
     }
     int64_t written = 0;
     if (off) off[0] = 0;

@@ -10,7 +10,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    txt = open(os.path.join(ROOT, "include", "flysim.h")).read()
+    inc = os.path.join(ROOT, "include")
+    txt = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc))
+                  if f.endswith(".h"))
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(?:int|const char \*)\s*(fs_[a-z_0-9]+)\s*\(", txt)))
 
@@ -32,6 +34,7 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(A.PopParams) == 4 * (8 + 2 + 10 + 4 + 10 + 1 + 3)
     assert C.sizeof(A.Schedule) == 24
     assert C.sizeof(A.ReplicaParams) == 32
+    assert C.sizeof(A.Adam) == 20
 
 
 def test_derive_params_reference_values():

@@ -1,0 +1,119 @@
+"""GPU tests of the rate surrogate's kernels (include/flysim_mlp.h) against
+the CPU oracle (oracle/mlp.py, pinned by finite differences and closed forms
+in tests/test_mlp_oracle.py).
+
+Bars: the forward pass equals the oracle run with the GPU's bfloat16 rounding
+points to 2e-3 relative; the gradient equals the oracle's bfloat16-rounded
+gradient to 1e-2 relative (L2, per layer) and the exact FP64 gradient to
+3e-2, with cosine similarity > 0.999; one Adam step equals the oracle's step
+from the same gradient; training reduces the loss of a learnable target.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+from flysim_amd.surrogate import Mlp
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [54, 256, 256, 256, 50]
+
+
+def _data(rows, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(rows, SIZES[0])).astype(np.float32)
+    t = rng.normal(size=(rows, SIZES[-1])).astype(np.float32)
+    return x, t
+
+
+def _layer_slices(sizes):
+    out, o = [], 0
+    for i in range(len(sizes) - 1):
+        n = sizes[i + 1] * sizes[i]
+        out.append((slice(o, o + n), slice(o + n, o + n + sizes[i + 1])))
+        o += n + sizes[i + 1]
+    return out
+
+
+@pytest.mark.parametrize("rows", [1, 37, 256, 1000])
+def test_forward_matches_oracle(rows):
+    print("case mlp forward rows=%d" % rows, flush=True)
+    m = Mlp(SIZES, seed=3)
+    p = m.params.astype(np.float64)
+    x, _ = _data(rows, rows)
+    y = m.predict(x).astype(np.float64)
+    ref, _ = M.forward(p, SIZES, x, bf16=True)
+    scale = np.max(np.abs(ref)) + 1e-12
+    assert np.max(np.abs(y - ref)) < 2e-3 * scale
+    ref64, _ = M.forward(p, SIZES, x)
+    assert np.max(np.abs(y - ref64)) < 3e-2 * scale
+
+
+def test_gradient_matches_oracle():
+    print("case mlp gradient rows=300", flush=True)
+    m = Mlp(SIZES, seed=4)
+    p = m.params.astype(np.float64)
+    x, t = _data(300, 9)
+    loss, g = m.loss_grad(x, t)
+    l16, g16 = M.loss_grad(p, SIZES, x, t, bf16=True)
+    l64, g64 = M.loss_grad(p, SIZES, x, t)
+    assert abs(loss - l16) < 1e-3 * l16
+    assert abs(loss - l64) < 2e-2 * l64
+    g = g.astype(np.float64)
+    for i, (ws, bs) in enumerate(_layer_slices(SIZES)):
+        for sl in (ws, bs):
+            e16 = np.linalg.norm(g[sl] - g16[sl]) / np.linalg.norm(g16[sl])
+            e64 = np.linalg.norm(g[sl] - g64[sl]) / np.linalg.norm(g64[sl])
+            print("layer %d %s: rel err vs bf16 oracle %.2e, vs fp64 %.2e"
+                  % (i, "W" if sl is ws else "b", e16, e64))
+            assert e16 < 1e-2
+            assert e64 < 3e-2
+    cos = g @ g64 / np.linalg.norm(g) / np.linalg.norm(g64)
+    assert cos > 0.999
+
+
+def test_one_adam_step_matches_oracle():
+    print("case mlp adam step", flush=True)
+    m = Mlp(SIZES, seed=5)
+    p0 = m.params.astype(np.float64)
+    x, t = _data(128, 11)
+    _, g = m.loss_grad(x, t)
+    m.set_data(x, t)
+    lr = 1e-3
+    m.train(1, 128, lr=lr, shuffle=False)   # one full-batch step in data order
+    p1 = m.params.astype(np.float64)
+    ref, _, _ = M.adam_step(p0, g.astype(np.float64), np.zeros_like(p0), np.zeros_like(p0), 1, lr)
+    # the update is +-lr wherever |g| >> eps; f32 master weights
+    np.testing.assert_allclose(p1, ref, rtol=0, atol=lr * 1e-3 + 1e-6)
+    assert np.max(np.abs(p1 - p0)) > 0.5 * lr
+
+
+def test_training_reduces_loss():
+    print("case mlp training", flush=True)
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-1, 1, size=(4096, SIZES[0])).astype(np.float32)
+    A = rng.normal(size=(SIZES[0], SIZES[-1])) / np.sqrt(SIZES[0])
+    t = np.tanh(x @ A).astype(np.float32)
+    m = Mlp(SIZES, seed=6)
+    m.set_data(x, t)
+    losses = m.train(30, 256, lr=1e-3)
+    print("loss", losses[0], "->", losses[-1])
+    assert losses[-1] < 0.1 * losses[0]
+    y = m.predict(x[:512])
+    assert np.mean((y - t[:512]) ** 2) < 2 * losses[-1] + 1e-3
+
+
+def test_predict_device_pointers_equal_host():
+    from flysim_amd._capi import DeviceBuffer
+    print("case mlp device predict", flush=True)
+    m = Mlp(SIZES, seed=7)
+    x, _ = _data(5000, 13)
+    y_host = m.predict(x)
+    xd = DeviceBuffer(x.nbytes)
+    yd = DeviceBuffer(x.shape[0] * SIZES[-1] * 4)
+    xd.upload(x)
+    ms = m.predict_device(xd.ptr, x.shape[0], yd.ptr)
+    assert ms > 0
+    y = np.zeros((x.shape[0], SIZES[-1]), np.float32)
+    yd.download(y)
+    np.testing.assert_array_equal(y, y_host)
