@@ -87,7 +87,7 @@ struct Engine final : EngineBase {
   DevBuf b_npop, b_pops, b_V, b_cs, b_G, b_ring, b_spre, b_xn, b_shist, b_gn, b_elin, b_enx,
       b_ens, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
       b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase,
-      b_pbr, b_gain, b_skey;
+      b_pbr, b_gain, b_skey, b_tmod;
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
   bool fused() const { return fused_ok && sched.fused; }
   bool fused_active() const override { return fused(); }
@@ -349,8 +349,9 @@ struct Engine final : EngineBase {
     b_lcnt.alloc(4 * 4);
     b_lbase.alloc(4 * 4);
     b_t0.alloc(8);
+    b_tmod.alloc(8);
     b_popcnt.alloc((size_t)P * B * 8);
-    b_counters.alloc(3 * 8);
+    b_counters.alloc(3 * fsk::NCTR * 8);
     b_tmp.alloc(n * sizeof(T));
     net.npop = b_npop.as<uint32_t>();
     net.pops = b_pops.as<fsk::DevPop<T>>();
@@ -377,6 +378,7 @@ struct Engine final : EngineBase {
     net.lcnt = b_lcnt.as<uint32_t>();
     net.lbase = b_lbase.as<uint32_t>();
     net.t0 = b_t0.as<int64_t>();
+    net.tmod = b_tmod.as<int32_t>();
     net.pop_cnt = b_popcnt.as<unsigned long long>();
     net.counters = b_counters.as<unsigned long long>();
     net.pbr = b_pbr.as<float>();
@@ -515,7 +517,7 @@ struct Engine final : EngineBase {
     fill(b_enx.p, n, T(0));
     fill(b_ens.p, n, T(0));
     HIPCHECK(hipMemsetAsync(b_popcnt.p, 0, (size_t)P * B * 8, stream));
-    HIPCHECK(hipMemsetAsync(b_counters.p, 0, 24, stream));
+    HIPCHECK(hipMemsetAsync(b_counters.p, 0, 3 * fsk::NCTR * 8, stream));
     HIPCHECK(hipMemsetAsync(b_overflow.p, 0, 4, stream));
     HIPCHECK(hipMemsetAsync(b_lcnt.p, 0, 16, stream));
     HIPCHECK(hipMemsetAsync(b_lbase.p, 0, 16, stream));
@@ -599,13 +601,13 @@ struct Engine final : EngineBase {
     t_upd = t_del = t_graph = 0;
     n_timed = 0;
     n_graph_steps = 0;
-    HIPCHECK(hipMemsetAsync(b_counters.p, 0, 24, stream));
+    HIPCHECK(hipMemsetAsync(b_counters.p, 0, 3 * fsk::NCTR * 8, stream));
     int64_t done = 0;
     while (done < nsteps) {
       const int32_t K = (int32_t)std::min<int64_t>(sched.steps_per_graph, nsteps - done);
       hipLaunchKernelGGL(fsk::k_batch_begin, dim3(1), dim3(1), 0, stream, b_t0.as<int64_t>(), step,
                          b_total.as<uint32_t>(), b_cum.as<uint32_t>(), b_lcnt.as<uint32_t>(),
-                         b_lbase.as<uint32_t>());
+                         b_lbase.as<uint32_t>(), b_tmod.as<int32_t>(), net.D, net.DN);
       HIPCHECK(hipGetLastError());
       launch_step(0, timing);
       if (K > 1) {
@@ -656,7 +658,14 @@ struct Engine final : EngineBase {
       step += K;
       done += K;
     }
-    HIPCHECK(hipMemcpy(ctr_last, b_counters.p, 24, hipMemcpyDeviceToHost));
+    {
+      std::vector<unsigned long long> c(3 * fsk::NCTR);
+      HIPCHECK(hipMemcpy(c.data(), b_counters.p, c.size() * 8, hipMemcpyDeviceToHost));
+      for (int i = 0; i < 3; ++i) {
+        ctr_last[i] = 0;
+        for (int s = 0; s < fsk::NCTR; ++s) ctr_last[i] += c[(size_t)i * fsk::NCTR + s];
+      }
+    }
     // every step gathers each NMDA synapse's term once
     ctr_last[3] = (unsigned long long)syn_nmda * (unsigned long long)B * (unsigned long long)nsteps;
   }
@@ -699,7 +708,7 @@ struct Engine final : EngineBase {
       case 5:
         if (fused() && net.has_nmda_syn) {  // the fused step keeps gN in registers
           hipLaunchKernelGGL((fsk::k_gather_only<T>), dim3((N + 3) / 4), dim3(256), 0, stream, net,
-                             step - 1);
+                             (int32_t)(((step - 1) % net.DN + net.DN) % net.DN));
           HIPCHECK(hipGetLastError());
           HIPCHECK(hipStreamSynchronize(stream));
         }

@@ -31,6 +31,9 @@
 
 namespace fsk {
 
+// event counters are spread over NCTR slots (block index) against contention
+constexpr int NCTR = 64;
+
 // state word: refractory count in bits 0..23, flags above
 constexpr uint32_t CNT_MASK = 0x00FFFFFFu;
 constexpr uint32_t F_FIRE = 1u << 24, F_SPIKE = 1u << 25, F_OVF = 1u << 28, F_UDF = 1u << 29;
@@ -111,8 +114,10 @@ struct Net {
   uint32_t *lbase;          // [3] fused path: record offset of the list of step k%3
   uint64_t rec_cap;
   const int64_t *t0;        // [1] first step of the batch
+  const int32_t *tmod;      // [2] t0 % D, t0 % DN (slot arithmetic stays 32-bit)
   unsigned long long *pop_cnt;  // [P0][B]
-  unsigned long long *counters; // [3]: spikes, linear pushes, NMDA out-synapses of spikers
+  unsigned long long *counters; // [3][NCTR]: spikes, linear pushes, NMDA out-synapses of
+                                // spikers, each spread over NCTR slots by block index
   int32_t gather_lanes;
   int32_t push_waves;       // waves of k_deliver dedicated to the push
 };
@@ -132,13 +137,18 @@ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1
 }
 __device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
 
+// (t0 % M + k) mod M for a step offset k >= -M of the batch starting at t0
+__device__ __forceinline__ int32_t slot_of(int32_t t0mod, int32_t k, int32_t M) {
+  return (int32_t)((uint32_t)(t0mod + k + M) % (uint32_t)M);
+}
+
 // ------------------------------------------------------------------ update
 // One neuron's step t (the reference's Activation) followed by the wave's
 // spike compaction; every lane of the wave must reach it (ballots).  With
 // gn_given the NMDA conductance comes from the caller (fused step) instead of
 // the gn array written by the separate gather.
 template <typename T, bool REPLAY>
-__device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int64_t t,
+__device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int64_t t, int32_t k,
                                               bool gn_given, T gn_in, uint32_t *list_ctr,
                                               uint32_t list_base) {
   const bool active = n < net.N;
@@ -156,7 +166,7 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
     T v = net.V[n];
     // (1) linear conductances of the previous Spike phase: G = df*G + slot
     T g[5] = {T(0), T(0), T(0), T(0), T(0)};
-    const int32_t slot = (int32_t)((t + net.D - 1) % net.D);
+    const int32_t slot = slot_of(net.tmod[0], k - 1, net.D);
 #pragma unroll
     for (int li = 0; li < 4; ++li) {
       if (!(net.lin_mask & (1 << li))) continue;
@@ -269,8 +279,8 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
     }
     if (P.has_nmda_out) {
       T x = net.xn[n];
-      const int32_t sp = (int32_t)((t + net.DN - 1) % net.DN);
-      const int32_t sc = (int32_t)(t % net.DN);
+      const int32_t sp = slot_of(net.tmod[1], k - 1, net.DN);
+      const int32_t sc = slot_of(net.tmod[1], k, net.DN);
       const T s0 = net.shist[(size_t)sp * N + n];
       x = spk ? x + T(1) : x * P.pre_dfxt;
       net.xn[n] = x;
@@ -279,42 +289,48 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
     net.V[n] = v;
     net.cs[n] = (cs & ~CNT_MASK) | (cnt & CNT_MASK);
   }
-  // (5) wave-ballot compaction of the spike list
+  // (5) spike-list compaction: wave ballot, then one block-level LDS count,
+  // so one global atomic per block claims the block's range of the list
+  // (a global atomic per wave serialises on one address at high rates)
+  __shared__ uint32_t s_cnt, s_base;
+  if (threadIdx.x == 0) s_cnt = 0u;
+  __syncthreads();
   const uint64_t mask = __ballot(spk);
-  if (mask) {
-    const int lane = (int)__lane_id();
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    const uint32_t cntw = (uint32_t)__popcll(mask);
-    uint32_t base = 0;
-    if (lane == leader) {
-      base = list_base + atomicAdd(list_ctr, cntw);
-      atomicAdd(&net.counters[0], (unsigned long long)cntw);
+  const int lane = (int)__lane_id();
+  const int leader = mask ? __ffsll((unsigned long long)mask) - 1 : 0;
+  const uint32_t cntw = (uint32_t)__popcll(mask);
+  uint32_t woff = 0u;
+  if (mask && lane == leader) woff = atomicAdd(&s_cnt, cntw);
+  woff = __shfl(woff, leader);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_cnt) {
+    s_base = list_base + atomicAdd(list_ctr, s_cnt);
+    atomicAdd(&net.counters[blockIdx.x & (NCTR - 1)], (unsigned long long)s_cnt);
+  }
+  __syncthreads();
+  if (spk) {
+    const uint32_t pos = s_base + woff + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (pos < net.rec_cap) {
+      net.rec_ids[pos] = (uint32_t)n;
+      net.rec_a[pos] = a_push;
+    } else {
+      atomicOr(net.overflow, 1u);
     }
-    base = __shfl(base, leader);
-    if (spk) {
-      const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      if (pos < net.rec_cap) {
-        net.rec_ids[pos] = (uint32_t)n;
-        net.rec_a[pos] = a_push;
-      } else {
-        atomicOr(net.overflow, 1u);
-      }
-      // per-population spike count, aggregated when the wave shares one population
-      const uint32_t p = pid;
-      const uint32_t p0 = __shfl(p, leader);
-      const bool same = __ballot(spk && p != p0) == 0ull;
-      if (same) {
-        if (lane == leader) atomicAdd(&net.pop_cnt[p0], (unsigned long long)cntw);
-      } else {
-        atomicAdd(&net.pop_cnt[p], 1ull);
-      }
+    // per-population spike count, aggregated when the wave shares one population
+    const uint32_t p = pid;
+    const uint32_t p0 = __shfl(p, leader);
+    const bool same = __ballot(spk && p != p0) == 0ull;
+    if (same) {
+      if (lane == leader) atomicAdd(&net.pop_cnt[p0], (unsigned long long)cntw);
+    } else {
+      atomicAdd(&net.pop_cnt[p], 1ull);
     }
   }
 }
 
 template <typename T, bool REPLAY>
 __global__ void __launch_bounds__(1024) k_update(Net<T> net, int32_t k) {
-  update_neuron<T, REPLAY>(net, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x), *net.t0 + k,
+  update_neuron<T, REPLAY>(net, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x), *net.t0 + k, k,
                            false, T(0), net.total, 0u);
 }
 
@@ -329,10 +345,12 @@ __device__ __forceinline__ void atomic_add_t(T *p, T v) {
 // `nwaves` waves.  Lane 0 of each wave accumulates the event counters.
 template <typename T>
 __device__ __forceinline__ void push_spikes(const Net<T> &net, uint32_t start, uint32_t end,
-                                            int64_t ts, int gwave, int nwaves, int lane,
+                                            int32_t s0, int gwave, int nwaves, int lane,
                                             unsigned long long &events,
                                             unsigned long long &nout) {
   const int32_t N = net.N, B = net.B;
+  // s0: ring slot of the emission step; delay d lands in s0 + d wrapped once
+  // (d <= maxd < D)
   for (uint32_t i = start + (uint32_t)gwave; i < end; i += (uint32_t)nwaves) {
     const uint32_t j = net.rec_ids[i];
     const uint32_t jl = j / (uint32_t)B, rep = j - jl * (uint32_t)B;
@@ -347,7 +365,8 @@ __device__ __forceinline__ void push_spikes(const Net<T> &net, uint32_t start, u
       const uint32_t d = (m >> 24) & 63u;
       const uint32_t li = m >> 30;
       const T s = li == 0 ? ga : li == 1 ? gg : li == 2 ? gh : gl;
-      const int32_t slot = (int32_t)((ts + (int64_t)d) % net.D);
+      int32_t slot = s0 + (int32_t)d;
+      slot -= slot >= net.D ? net.D : 0;
       atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(net.w_p[e]) * s);
     }
   }
@@ -365,8 +384,9 @@ __device__ __forceinline__ void block_add_events(const Net<T> &net, unsigned lon
   if (nout) atomicAdd(&s_ev[1], nout);
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (s_ev[0]) atomicAdd(&net.counters[1], s_ev[0]);
-    if (s_ev[1]) atomicAdd(&net.counters[2], s_ev[1]);
+    const int slot = (int)(blockIdx.x & (NCTR - 1));
+    if (s_ev[0]) atomicAdd(&net.counters[NCTR + slot], s_ev[0]);
+    if (s_ev[1]) atomicAdd(&net.counters[2 * NCTR + slot], s_ev[1]);
   }
 }
 
@@ -374,19 +394,32 @@ __device__ __forceinline__ void block_add_events(const Net<T> &net, unsigned lon
 // with L lanes of a group striding the row and a shuffle reduction (result
 // valid in every lane of the group).  All lanes of the group must call it.
 template <typename T>
-__device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int64_t ts, int L, int sl) {
+__device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int32_t h0, int L, int sl) {
   T acc = T(0);
   if (p < net.N) {
     const uint32_t B = (uint32_t)net.B;
     const uint32_t pl = (uint32_t)p / B, rep = (uint32_t)p - pl * B;
     const uint64_t r0 = net.rp_n[pl], r1 = net.rp_n[pl + 1];
-    for (uint64_t e = r0 + (uint64_t)sl; e < r1; e += (uint64_t)L) {
+    const int32_t DN = net.DN;  // h0: history slot of ts; the slot of ts - d wraps once
+    const T *sh = net.shist + rep;
+    auto term = [&](uint64_t e) -> T {
       const uint32_t m = net.meta_n[e];
-      const uint32_t j = (m & 0x00FFFFFFu) * B + rep;
-      const uint32_t d = m >> 24;
-      const int32_t hs = (int32_t)(((ts - (int64_t)d) % net.DN + net.DN) % net.DN);
-      acc += T(net.w_n[e]) * net.shist[(size_t)hs * net.N + j];
+      int32_t hs = h0 - (int32_t)(m >> 24);
+      hs += hs < 0 ? DN : 0;
+      return T(net.w_n[e]) * sh[(size_t)hs * net.N + (size_t)(m & 0x00FFFFFFu) * B];
+    };
+    // four independent partial sums keep four row entries in flight per lane
+    T a1 = T(0), a2 = T(0), a3 = T(0);
+    uint64_t e = r0 + (uint64_t)sl;
+    const uint64_t st = (uint64_t)L;
+    for (; e + 3 * st < r1; e += 4 * st) {
+      acc += term(e);
+      a1 += term(e + st);
+      a2 += term(e + 2 * st);
+      a3 += term(e + 3 * st);
     }
+    for (; e < r1; e += st) acc += term(e);
+    acc = (acc + a1) + (a2 + a3);
     acc *= T(net.gainr[2 * B + rep]);
   }
   for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
@@ -421,7 +454,8 @@ __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
     const uint32_t cap = (uint32_t)net.rec_cap;
     const uint32_t s0 = prev_base < cap ? prev_base : cap;
     const uint32_t s1 = base_k < cap ? base_k : cap;
-    push_spikes(net, s0, s1, t - 1, (int)blockIdx.x * wpb + wave, (int)gridDim.x * wpb, lane,
+    push_spikes(net, s0, s1, slot_of(net.tmod[0], k - 1, net.D), (int)blockIdx.x * wpb + wave,
+                (int)gridDim.x * wpb, lane,
                 my_events, my_nout);
   }
   const int L = net.gather_lanes;
@@ -429,9 +463,9 @@ __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
   const int64_t nn = gt / L;
   const int sl = (int)(gt % L);
   T gn = T(0);
-  if (net.has_nmda_syn) gn = gather_row(net, nn, t - 1, L, sl);
+  if (net.has_nmda_syn) gn = gather_row(net, nn, slot_of(net.tmod[1], k - 1, net.DN), L, sl);
   const int32_t n = (sl == 0 && nn < net.N) ? (int32_t)nn : net.N;  // N: inactive lane
-  update_neuron<T, REPLAY>(net, n, t, true, gn, net.lcnt + pc, base_k);
+  update_neuron<T, REPLAY>(net, n, t, k, true, gn, net.lcnt + pc, base_k);
   block_add_events(net, my_events, my_nout);
 }
 
@@ -447,17 +481,18 @@ __global__ void __launch_bounds__(1024) k_push_tail(Net<T> net, int32_t K) {
   const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const int wpb = (int)(blockDim.x >> 6);
   unsigned long long ev = 0, no = 0;
-  push_spikes(net, b < cap ? b : cap, e < cap ? e : cap, t, (int)blockIdx.x * wpb + wave,
+  push_spikes(net, b < cap ? b : cap, e < cap ? e : cap, slot_of(net.tmod[0], K - 1, net.D),
+              (int)blockIdx.x * wpb + wave,
               (int)gridDim.x * wpb, lane, ev, no);
   block_add_events(net, ev, no);
 }
 
 // NMDA gather of step ts into gn (state readback of the fused path).
 template <typename T>
-__global__ void __launch_bounds__(1024) k_gather_only(Net<T> net, int64_t ts) {
+__global__ void __launch_bounds__(1024) k_gather_only(Net<T> net, int32_t h0) {
   const int lane = (int)(threadIdx.x & 63);
   const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const T v = gather_row(net, p, ts, 64, lane);
+  const T v = gather_row(net, p, h0, 64, lane);
   if (lane == 0 && p < net.N) net.gn[p] = v;
 }
 
@@ -477,17 +512,19 @@ __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
     const uint32_t start = net.cum[k];
     const uint32_t end = end_raw < net.rec_cap ? end_raw : (uint32_t)net.rec_cap;
     if (blockIdx.x == 0 && threadIdx.x == 0) net.cum[k + 1] = end;
-    push_spikes(net, start, end, t, gwave, net.push_waves, lane, my_events, my_nout);
+    push_spikes(net, start, end, slot_of(net.tmod[0], k, net.D), gwave, net.push_waves, lane,
+                my_events, my_nout);
   } else if (net.has_nmda_syn) {
     // ---- NMDA gather over CSR-by-post, L lanes per row
     const int L = net.gather_lanes;
+    const int32_t h0 = slot_of(net.tmod[1], k, net.DN);
     const int gw = gwave - net.push_waves;
     const int gwaves = total_waves - net.push_waves;
     const int groups_per_wave = 64 / L;
     const int sub = lane / L, sl = lane % L;
     const int64_t stride = (int64_t)gwaves * groups_per_wave;
     for (int64_t p = (int64_t)gw * groups_per_wave + sub; p - sub < N; p += stride) {
-      const T acc = gather_row(net, p, t, L, sl);
+      const T acc = gather_row(net, p, h0, L, sl);
       if (p < N && sl == 0) net.gn[p] = acc;
     }
   }
@@ -497,8 +534,10 @@ __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
 // batch prologue: first step of the batch, record buffer reset
 // (the fused path's list rotation starts with lists 2 (step -1) and 0 empty)
 __global__ void k_batch_begin(int64_t *t0, int64_t value, uint32_t *total, uint32_t *cum,
-                              uint32_t *lcnt, uint32_t *lbase) {
+                              uint32_t *lcnt, uint32_t *lbase, int32_t *tmod, int32_t D, int32_t DN) {
   *t0 = value;
+  tmod[0] = (int32_t)(value % D);
+  tmod[1] = (int32_t)(value % DN);
   *total = 0u;
   cum[0] = 0u;
   lcnt[0] = 0u;
