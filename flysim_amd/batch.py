@@ -1,0 +1,115 @@
+"""Batched parameter sweeps over many independent E/I networks of uneven size.
+
+The 256-instance sweep: instances of the 1k E/I generator's kind (80/20 E/I,
+Bernoulli(0.1) pairs without self-connections, g ~ U(0.5, 1.5) nS, E->X
+pairs AMPA + NMDA (0.1 g), I->X pairs GABA, SpikeDly 1 ms, refractory 2 ms)
+with sizes uniform in {1000..20000}, seeds 1000 + k, external AMPA rate
+uniform in 0.5-4 kHz and NMDA / GABA gains uniform in 0.5-2, all in one
+handle whose synapse tables the device generates (fs_create_ei_batch).
+
+``ei_batch_network`` restates that generator in numpy (same hash, same float
+rounding) for small instances: the tests build the same networks through the
+host path (fs_create) and compare rasters.
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+from . import workloads as W
+from .engine import EiBatchSpec, Engine
+
+M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def mix64(x: np.ndarray) -> np.ndarray:
+    """splitmix64 finaliser (Steele, Lea, Flood 2014), uint64 wrap-around."""
+    with np.errstate(over="ignore"):
+        z = (np.asarray(x, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def c1_populations(f_ext: float = 2400.0, g_ext: float = W.C1_G_EXT):
+    """E and I population templates of the 1k E/I network (sizes unused)."""
+    def pop(name):
+        return W.Population(name=name, n=1, cm=0.5, taum=20.0, vth=-50.0, rest=-70.0, reset=-55.0,
+                            refractory_ms=2.0, spike_delay_ms=1.0, self_connection=False,
+                            receptors=[W.Receptor(W.AMPA, tau=2.0, rev=0.0, freq_ext=f_ext,
+                                                  mean_ext_eff=g_ext, mean_ext_con=1.0),
+                                       W.Receptor(W.NMDA, tau=100.0, tau_xt=2.0, rev=0.0),
+                                       W.Receptor(W.GABA, tau=5.0, rev=-70.0)])
+    return pop("E"), pop("I")
+
+
+def c4_spec(n_inst: int = 256, seed: int = 4, n_lo: int = 1000, n_hi: int = 20000,
+            f_lo: float = 500.0, f_hi: float = 4000.0, gain_lo: float = 0.5,
+            gain_hi: float = 2.0, g_ext: float = W.C1_G_EXT) -> EiBatchSpec:
+    """The 256-instance sweep (sizes, rates and gains drawn with numpy's
+    PCG64 from `seed`; instance k uses connectivity seed 1000 + k)."""
+    rng = np.random.default_rng(seed)
+    n = rng.integers(n_lo, n_hi + 1, n_inst)
+    ext = rng.uniform(f_lo, f_hi, n_inst)
+    gn = rng.uniform(gain_lo, gain_hi, n_inst)
+    gg = rng.uniform(gain_lo, gain_hi, n_inst)
+    e, i = c1_populations(g_ext=g_ext)
+    return EiBatchSpec(n, 1000 + np.arange(n_inst), ext, gn, gg, e, i)
+
+
+def ei_batch_network(spec: EiBatchSpec) -> W.Network:
+    """Explicit synapse list of a (small) batch, identical to the device
+    generator: pair (j, p) of instance k exists iff the high 32 bits of
+    mix64(mix64(seed_k) ^ (j << 32 | p)) are below p_conn * 2^32; g from the
+    low 24 bits.  Synapses are emitted in (pre, post) order."""
+    thr = np.uint64(min(4294967295, int(np.floor(np.float64(np.float32(spec.p_conn)) * 4294967296.0))))
+    g_lo = np.float32(spec.g_lo)
+    g_span = np.float32(np.float32(spec.g_hi) - np.float32(spec.g_lo))
+    ratio = np.float32(spec.nmda_ratio)
+    n_exc = spec.n_exc()
+    pops, pre_l, post_l, w_l, r_l = [], [], [], [], []
+    first = 0
+    for k, n in enumerate(spec.n_neuron):
+        n = int(n)
+        ne = int(n_exc[k])
+        key = mix64(np.uint64(spec.seed[k]))
+        j = np.repeat(np.arange(n, dtype=np.uint64), n)
+        p = np.tile(np.arange(n, dtype=np.uint64), n)
+        h = mix64(key ^ ((j << np.uint64(32)) | p))
+        ok = ((h >> np.uint64(32)) < thr) & (j != p)
+        u = (h & np.uint64(0xFFFFFF)).astype(np.float32) * np.float32(1.0 / 16777216.0)
+        g = (g_lo + (g_span * u).astype(np.float32)).astype(np.float32)
+        jj, pp, gg = j[ok].astype(np.int64), p[ok].astype(np.int64), g[ok]
+        exc = jj < ne
+        gn = np.float32(spec.gain_nmda[k])
+        gb = np.float32(spec.gain_gaba[k])
+        # per pair in (j, p) order: AMPA then NMDA for E, GABA for I
+        for sel, w, rec in ((exc, gg, W.AMPA),
+                            (exc, ((gg * ratio).astype(np.float32) * gn).astype(np.float32), W.NMDA),
+                            (~exc, (gg * gb).astype(np.float32), W.GABA)):
+            pre_l.append(jj[sel] + first)
+            post_l.append(pp[sel] + first)
+            w_l.append(w[sel])
+            r_l.append(np.full(int(sel.sum()), rec, np.uint8))
+        e, i = spec.exc, spec.inh
+        for tmpl, size, name in ((e, ne, "E%d" % k), (i, n - ne, "I%d" % k)):
+            q = W.Population(**{**tmpl.__dict__, "name": name, "n": size,
+                                "receptors": [W.Receptor(**r.__dict__) for r in tmpl.receptors]})
+            for r in q.receptors:
+                if r.kind == W.AMPA:
+                    r.freq_ext = float(spec.ext_hz[k])
+            pops.append(q)
+        first += n
+    pre = np.concatenate(pre_l).astype(np.uint32)
+    order = np.argsort(pre, kind="stable")
+    return W.Network(name="ei_batch", pops=pops, pre=pre[order],
+                     post=np.concatenate(post_l).astype(np.uint32)[order],
+                     w=np.concatenate(w_l).astype(np.float32)[order],
+                     receptor=np.concatenate(r_l)[order],
+                     delay=np.zeros(pre.size, np.uint8), dt=spec.dt)
+
+
+def expected_pairs(spec: EiBatchSpec) -> float:
+    n = spec.n_neuron.astype(np.float64)
+    return float(np.sum(n * (n - 1)) * spec.p_conn)

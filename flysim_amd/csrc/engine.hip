@@ -13,6 +13,7 @@
 
 #include "../../include/flysim.h"
 #include "engine_kernels.hpp"
+#include "gen_kernels.hpp"
 #include "host/host_common.hpp"
 
 namespace {
@@ -264,36 +265,201 @@ struct Engine final : EngineBase {
         }
       }
     }
-    // ---------------- populations on device
+    std::vector<uint32_t> nout(N0, 0u);
+    for (int64_t s = 0; s < d.n_syn; ++s)
+      if (d.receptor[s] == FS_NMDA) nout[d.pre[s]]++;
+    PreGates pg{pre_set, pre_set_n, pre_df, pre_dfxt, pre_dfst, pre_ef, pre_as};
+    init_pops(pg);
+    init_dims(maxd_p, maxd_n, d.n_syn > 0 && mind_all >= 1, lin_mask, d.ext_mode);
+    const size_t n0 = (size_t)N0;
+    b_rp_p.alloc((n0 + 1) * 8);
+    HIPCHECK(hipMemcpy(b_rp_p.p, cnt_p.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
+    b_meta_p.alloc(syn_lin * 4);
+    b_w_p.alloc(syn_lin * 4);
+    if (syn_lin) {
+      HIPCHECK(hipMemcpy(b_meta_p.p, meta_p.data(), syn_lin * 4, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(b_w_p.p, w_p.data(), syn_lin * 4, hipMemcpyHostToDevice));
+    }
+    b_rp_n.alloc((n0 + 1) * 8);
+    HIPCHECK(hipMemcpy(b_rp_n.p, cnt_n.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
+    b_meta_n.alloc(syn_nmda * 4);
+    b_w_n.alloc(syn_nmda * 4);
+    if (syn_nmda) {
+      HIPCHECK(hipMemcpy(b_meta_n.p, meta_n.data(), syn_nmda * 4, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(b_w_n.p, w_n.data(), syn_nmda * 4, hipMemcpyHostToDevice));
+    }
+    b_nmda_out.alloc(n0 * 4);
+    HIPCHECK(hipMemcpy(b_nmda_out.p, nout.data(), n0 * 4, hipMemcpyHostToDevice));
+    finish(d.neuron_pop);
+  }
+
+  // Batched random E/I instances, synapse tables generated on the device
+  // (gen_kernels.hpp).  Stream key of instance k: mix64(seed_k ^ salt).
+  explicit Engine(const fs_ei_batch_desc &d) {
+    const int32_t K = d.n_inst;
+    if (K < 1 || K > 65535) throw std::invalid_argument("n_inst must be 1..65535");
+    if (!d.n_neuron || !d.seed || !d.ext_hz || !d.gain_nmda || !d.gain_gaba)
+      throw std::invalid_argument("incomplete batch description");
+    if (!(d.p_conn >= 0.0f && d.p_conn <= 1.0f)) throw std::invalid_argument("p_conn must be in [0,1]");
+    if (!(d.exc_frac >= 0.0f && d.exc_frac <= 1.0f)) throw std::invalid_argument("exc_frac must be in [0,1]");
+    int64_t tot = 0;
+    for (int32_t k = 0; k < K; ++k) {
+      if (d.n_neuron[k] < 1) throw std::invalid_argument("every instance needs neurons");
+      tot += d.n_neuron[k];
+    }
+    if (tot >= (1 << 24)) throw std::invalid_argument("at most 2^24-1 neurons per device");
+    // the presynaptic gates need one decay constant per receptor across E and I targets
+    if (d.exc.dfst[FS_AMPA] != d.inh.dfst[FS_AMPA] || d.exc.dfst[FS_GABA] != d.inh.dfst[FS_GABA] ||
+        d.exc.dfst[FS_NMDA] != d.inh.dfst[FS_NMDA] || d.exc.dfxt_n != d.inh.dfxt_n ||
+        d.exc.efst_n != d.inh.efst_n || d.exc.as_n != d.inh.as_n)
+      throw std::invalid_argument("E and I populations must share receptor time constants");
+    N0 = N = (int32_t)tot;
+    B = replicas = 1;
+    P = n_pop = 2 * K;
+    dt = d.dt;
+    precision = d.precision;
+    record = d.record_spikes != 0;
+    HIPCHECK(hipStreamCreate(&stream));
+    for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+    std::vector<fsg::GenInst> gi(K);
+    std::vector<uint32_t> npop(N);
+    std::vector<uint16_t> inst_of(N);
+    hpop.resize(P);
+    int64_t first = 0;
+    for (int32_t k = 0; k < K; ++k) {
+      const int32_t n = d.n_neuron[k];
+      const int32_t ne = (int32_t)std::floor((double)d.exc_frac * n + 0.5);
+      gi[k] = fsg::GenInst{first, n, ne, fsg::mix64(d.seed[k]), d.gain_nmda[k], d.gain_gaba[k]};
+      for (int32_t i = 0; i < n; ++i) {
+        npop[first + i] = (uint32_t)(2 * k + (i < ne ? 0 : 1));
+        inst_of[first + i] = (uint16_t)k;
+      }
+      const float pb = (float)(d.ext_hz[k] * d.dt * 1e-3f);
+      for (int e = 0; e < 2; ++e) {
+        fs_pop_params q = e == 0 ? d.exc : d.inh;
+        if (d.ext_hz[k] == 0.0f) q.act &= ~(uint32_t)FS_ACT_AMPA;
+        else { q.act |= FS_ACT_AMPA; q.pb[FS_AMPA] = pb; }
+        hpop[2 * k + e] = q;
+      }
+      first += n;
+    }
+    // synapse tables: count pass, row offsets on the host, fill pass
+    const size_t n0 = (size_t)N0;
+    DevBuf b_gi, b_inst;
+    b_gi.alloc(sizeof(fsg::GenInst) * K);
+    HIPCHECK(hipMemcpy(b_gi.p, gi.data(), sizeof(fsg::GenInst) * K, hipMemcpyHostToDevice));
+    b_inst.alloc(n0 * 2);
+    HIPCHECK(hipMemcpy(b_inst.p, inst_of.data(), n0 * 2, hipMemcpyHostToDevice));
+    b_rp_p.alloc((n0 + 1) * 8);
+    b_rp_n.alloc((n0 + 1) * 8);
+    b_nmda_out.alloc(n0 * 4);
+    HIPCHECK(hipMemset(b_rp_p.p, 0, (n0 + 1) * 8));
+    HIPCHECK(hipMemset(b_rp_n.p, 0, (n0 + 1) * 8));
+    fsg::GenArgs ga{};
+    ga.inst = b_gi.as<fsg::GenInst>();
+    ga.inst_of = b_inst.as<uint16_t>();
+    ga.N = N0;
+    ga.thr = (uint32_t)std::min(4294967295.0, std::floor((double)d.p_conn * 4294967296.0));
+    ga.g_lo = d.g_lo;
+    ga.g_span = d.g_hi - d.g_lo;
+    ga.nmda_ratio = d.nmda_ratio;
+    ga.rp_p = b_rp_p.as<uint64_t>();
+    ga.rp_n = b_rp_n.as<uint64_t>();
+    ga.nmda_out = b_nmda_out.as<uint32_t>();
+    const dim3 grid((unsigned)((n0 * 64 + 255) / 256));
+    ga.fill = 0;
+    hipLaunchKernelGGL(fsg::k_gen_rows, grid, dim3(256), 0, stream, ga);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (int w = 0; w < 2; ++w) {
+      DevBuf &rp = w == 0 ? b_rp_p : b_rp_n;
+      std::vector<uint64_t> h(n0 + 1);
+      HIPCHECK(hipMemcpy(h.data(), rp.p, (n0 + 1) * 8, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n0; ++i) h[i + 1] += h[i];
+      HIPCHECK(hipMemcpy(rp.p, h.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
+      (w == 0 ? syn_lin : syn_nmda) = (int64_t)h[n0];
+    }
+    b_meta_p.alloc((size_t)syn_lin * 4);
+    b_w_p.alloc((size_t)syn_lin * 4);
+    b_meta_n.alloc((size_t)syn_nmda * 4);
+    b_w_n.alloc((size_t)syn_nmda * 4);
+    ga.meta_p = b_meta_p.as<uint32_t>();
+    ga.w_p = b_w_p.as<float>();
+    ga.meta_n = b_meta_n.as<uint32_t>();
+    ga.w_n = b_w_n.as<float>();
+    ga.fill = 1;
+    hipLaunchKernelGGL(fsg::k_gen_rows, grid, dim3(256), 0, stream, ga);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(stream));
+    // presynaptic gates: E drive AMPA + NMDA targets, I drive GABA targets
+    PreGates pg;
+    pg.set.assign(P, 1);
+    pg.set_n.assign(P, 0);
+    pg.df.assign(P, 0.f); pg.dfxt.assign(P, 0.f); pg.dfst.assign(P, 0.f);
+    pg.ef.assign(P, 0.f); pg.as.assign(P, 0.f);
+    for (int32_t k = 0; k < K; ++k) {
+      pg.df[2 * k] = d.exc.dfst[FS_AMPA];
+      pg.set_n[2 * k] = 1;
+      pg.dfxt[2 * k] = d.exc.dfxt_n;
+      pg.dfst[2 * k] = d.exc.dfst[FS_NMDA];
+      pg.ef[2 * k] = d.exc.efst_n;
+      pg.as[2 * k] = d.exc.as_n;
+      pg.df[2 * k + 1] = d.exc.dfst[FS_GABA];
+    }
+    init_pops(pg);
+    for (int32_t k = 0; k < K; ++k) {
+      const uint64_t skey = fsg::mix64(d.seed[k] ^ 0x5851F42D4C957F2Dull);
+      for (int e = 0; e < 2; ++e) {
+        dpop[2 * k + e].key_lo = (uint32_t)skey;
+        dpop[2 * k + e].key_hi = (uint32_t)(skey >> 32);
+        dpop[2 * k + e].first = (int32_t)gi[k].first;
+      }
+    }
+    init_dims(0, 0, false, (1 << 0) | (1 << 1), d.ext_mode);
+    finish(npop.data());
+  }
+
+  // presynaptic gate parameters per population (its targets' receptor constants)
+  struct PreGates {
+    std::vector<int> set, set_n;
+    std::vector<float> df, dfxt, dfst, ef, as;
+  };
+
+  void init_pops(const PreGates &g) {
     dpop.resize(P);
     for (int32_t i = 0; i < P; ++i) {
       fsk::DevPop<T> base{};
-      base.has_lin_out = pre_set[i];
-      base.has_nmda_out = pre_set_n[i];
-      base.pre_df = T(pre_df[i]);
-      base.pre_dfxt = T(pre_dfxt[i]);
-      base.pre_dfst = T(pre_dfst[i]);
-      if (sizeof(T) == 4) base.pre_efas = T(pre_ef[i] * pre_as[i]);
-      else base.pre_efas = T(pre_ef[i]) * T(pre_as[i]);
+      base.has_lin_out = g.set[i];
+      base.has_nmda_out = g.set_n[i];
+      base.pre_df = T(g.df[i]);
+      base.pre_dfxt = T(g.dfxt[i]);
+      base.pre_dfst = T(g.dfst[i]);
+      if (sizeof(T) == 4) base.pre_efas = T(g.ef[i] * g.as[i]);
+      else base.pre_efas = T(g.ef[i]) * T(g.as[i]);
       dpop[i] = to_dev(hpop[i], base);
     }
-    // ---------------- allocations
-    const size_t n = (size_t)N;
+  }
+
+  void init_dims(int32_t maxd_p, int32_t maxd_n, bool fusable, int32_t lin_mask, int32_t ext_mode) {
     net.N = N;
     net.D = maxd_p + 1;
     // the fused step reads gate history while writing the current slot
     net.DN = maxd_n + 2;
-    fused_ok = d.n_syn > 0 && mind_all >= 1;
+    fused_ok = fusable;
     net.lin_mask = lin_mask;
     net.has_nmda_syn = syn_nmda > 0;
-    net.ext_mode = d.ext_mode;
+    net.ext_mode = ext_mode;
     net.words = (N + 31) / 32;
     net.B = B;
     net.N0 = N0;
     net.P0 = P;
-    const size_t n0 = (size_t)N0;
+  }
+
+  // state, tables and launch set-up once the synapse tables are on the device
+  void finish(const uint32_t *neuron_pop) {
+    const size_t n = (size_t)N, n0 = (size_t)N0;
     b_npop.alloc(n0 * 4);
-    HIPCHECK(hipMemcpy(b_npop.p, d.neuron_pop, n0 * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b_npop.p, neuron_pop, n0 * 4, hipMemcpyHostToDevice));
     b_pops.alloc(sizeof(fsk::DevPop<T>) * P);
     upload_pops();
     // per-replica tables: pb from the population blocks, unit gains, and
@@ -321,29 +487,6 @@ struct Engine final : EngineBase {
     b_elin.alloc((size_t)nlin * n * sizeof(T));
     b_enx.alloc(n * sizeof(T));
     b_ens.alloc(n * sizeof(T));
-    b_rp_p.alloc((n0 + 1) * 8);
-    HIPCHECK(hipMemcpy(b_rp_p.p, cnt_p.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
-    b_meta_p.alloc(syn_lin * 4);
-    b_w_p.alloc(syn_lin * 4);
-    if (syn_lin) {
-      HIPCHECK(hipMemcpy(b_meta_p.p, meta_p.data(), syn_lin * 4, hipMemcpyHostToDevice));
-      HIPCHECK(hipMemcpy(b_w_p.p, w_p.data(), syn_lin * 4, hipMemcpyHostToDevice));
-    }
-    b_rp_n.alloc((n0 + 1) * 8);
-    HIPCHECK(hipMemcpy(b_rp_n.p, cnt_n.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
-    b_meta_n.alloc(syn_nmda * 4);
-    b_w_n.alloc(syn_nmda * 4);
-    if (syn_nmda) {
-      HIPCHECK(hipMemcpy(b_meta_n.p, meta_n.data(), syn_nmda * 4, hipMemcpyHostToDevice));
-      HIPCHECK(hipMemcpy(b_w_n.p, w_n.data(), syn_nmda * 4, hipMemcpyHostToDevice));
-    }
-    {
-      std::vector<uint32_t> nout(n0, 0u);
-      for (int64_t s = 0; s < d.n_syn; ++s)
-        if (d.receptor[s] == FS_NMDA) nout[d.pre[s]]++;
-      b_nmda_out.alloc(n0 * 4);
-      HIPCHECK(hipMemcpy(b_nmda_out.p, nout.data(), n0 * 4, hipMemcpyHostToDevice));
-    }
     b_total.alloc(4);
     b_overflow.alloc(4);
     b_lcnt.alloc(4 * 4);
@@ -777,6 +920,18 @@ int fs_create(const fs_net_desc *d, fs_handle **out) {
     auto h = std::make_unique<fs_handle>();
     if (d->precision == FS_FP64) h->e.reset(new Engine<double>(*d, 1));
     else if (d->precision == FS_FP32) h->e.reset(new Engine<float>(*d, 1));
+    else throw std::invalid_argument("precision must be FS_FP32 or FS_FP64");
+    *out = h.release();
+  })
+}
+
+int fs_create_ei_batch(const fs_ei_batch_desc *d, fs_handle **out) {
+  if (!d || !out) return fs::set_error(FS_ERR_ARG, "null argument");
+  if (!(d->dt > 0.0f)) return fs::set_error(FS_ERR_ARG, "dt must be positive");
+  ENGINE_CALL({
+    auto h = std::make_unique<fs_handle>();
+    if (d->precision == FS_FP64) h->e.reset(new Engine<double>(*d));
+    else if (d->precision == FS_FP32) h->e.reset(new Engine<float>(*d));
     else throw std::invalid_argument("precision must be FS_FP32 or FS_FP64");
     *out = h.release();
   })

@@ -59,7 +59,10 @@ struct DevPop {
   int32_t rfped, spidy;
   uint32_t act;             // only ACT_INJ is read on the device
   uint32_t has_lin_out, has_nmda_out;
-  uint32_t pad;
+  // random stream of the population's network instance: Philox key
+  // seed ^ replica key ^ (key_hi:key_lo), counter = local neuron - first
+  uint32_t key_lo, key_hi;
+  int32_t first;
 };
 
 template <typename T>
@@ -181,17 +184,19 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
     }
     if (net.has_nmda_syn) g[2] = gn_given ? gn_in : net.gn[n];
     // (2) external Poisson gates, order A, G, ACh, GluCl, NMDA (BgInit)
-    // (counter = local neuron and step, key = seed ^ replica key: a replica
-    // draws the same stream as the same network run alone with that key)
+    // (counter = neuron within its network instance and step, key = seed ^
+    // replica key ^ instance key: a replica or a batched instance draws the
+    // same stream as the same network run alone with that key)
     const uint32_t act = P.act;
-    const uint64_t key = net.seed ^ net.skey[rep];
+    const uint64_t key = net.seed ^ net.skey[rep] ^ (((uint64_t)P.key_hi << 32) | P.key_lo);
+    const uint32_t ctr = (uint32_t)(nl - P.first);
     uint32_t rnd[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
     if ((!REPLAY && net.ext_mask) || (act & ACT_INJ)) {
-      uint32_t c[4] = {(uint32_t)nl, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0u};
+      uint32_t c[4] = {ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0u};
       philox10(c, (uint32_t)key, (uint32_t)(key >> 32));
       rnd[0] = c[0]; rnd[1] = c[1]; rnd[2] = c[2]; rnd[3] = c[3];
       if ((net.ext_mask & (1 << 4)) || (act & ACT_INJ)) {
-        uint32_t e[4] = {(uint32_t)nl, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 1u};
+        uint32_t e[4] = {ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 1u};
         philox10(e, (uint32_t)key, (uint32_t)(key >> 32));
         rnd[4] = e[0]; rnd[5] = e[1]; rnd[6] = e[2]; rnd[7] = e[3];
       }

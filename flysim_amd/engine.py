@@ -72,6 +72,42 @@ class NetArrays:
         return d
 
 
+class EiBatchSpec:
+    """Instances of the batched E/I generator (fs_ei_batch_desc)."""
+
+    def __init__(self, n_neuron, seed, ext_hz, gain_nmda, gain_gaba, exc: Population,
+                 inh: Population, exc_frac: float = 0.8, p_conn: float = 0.1, g_lo: float = 0.5,
+                 g_hi: float = 1.5, nmda_ratio: float = 0.1, dt: float = 0.1):
+        self.n_neuron = np.ascontiguousarray(n_neuron, dtype=np.int32)
+        k = self.n_neuron.size
+        self.seed = np.ascontiguousarray(np.broadcast_to(seed, k), dtype=np.uint64)
+        self.ext_hz = np.ascontiguousarray(np.broadcast_to(ext_hz, k), dtype=np.float32)
+        self.gain_nmda = np.ascontiguousarray(np.broadcast_to(gain_nmda, k), dtype=np.float32)
+        self.gain_gaba = np.ascontiguousarray(np.broadcast_to(gain_gaba, k), dtype=np.float32)
+        self.exc, self.inh = exc, inh
+        self.exc_frac, self.p_conn, self.g_lo, self.g_hi = exc_frac, p_conn, g_lo, g_hi
+        self.nmda_ratio, self.dt = nmda_ratio, dt
+
+    def n_exc(self) -> np.ndarray:
+        return np.floor(np.float64(np.float32(self.exc_frac)) * self.n_neuron + 0.5).astype(np.int64)
+
+    def desc(self, precision: int, ext_mode: int, record: bool):
+        d = A.EiBatchDesc()
+        d.n_inst = int(self.n_neuron.size)
+        d.n_neuron = A.ptr(self.n_neuron, C.c_int32)
+        d.seed = A.ptr(self.seed, C.c_uint64)
+        d.ext_hz = A.ptr(self.ext_hz, C.c_float)
+        d.gain_nmda = A.ptr(self.gain_nmda, C.c_float)
+        d.gain_gaba = A.ptr(self.gain_gaba, C.c_float)
+        d.exc_frac, d.p_conn, d.g_lo, d.g_hi = self.exc_frac, self.p_conn, self.g_lo, self.g_hi
+        d.nmda_ratio = self.nmda_ratio
+        d.exc = derive_params(self.exc, self.dt)
+        d.inh = derive_params(self.inh, self.dt)
+        d.dt = self.dt
+        d.precision, d.ext_mode, d.record_spikes = precision, ext_mode, int(record)
+        return d, self
+
+
 def arrays_from_network(net: Network, act: int = A.DEFAULT_ACT) -> NetArrays:
     pops = [derive_params(p, net.dt, act) for p in net.pops]
     off = net.pop_offsets()
@@ -195,6 +231,25 @@ class Engine:
             A.check(A.lib().fs_create_replicated(C.byref(d), self.replicas, C.byref(h)))
         self._h = h
         self.precision = precision
+
+    @classmethod
+    def ei_batch(cls, spec: "EiBatchSpec", precision: int = A.FS_FP32,
+                 ext_mode: int = A.FS_EXT_PHILOX, record: bool = True) -> "Engine":
+        """Batched random E/I instances generated on the device
+        (fs_create_ei_batch); populations 2k / 2k+1 are the E / I
+        populations of instance k."""
+        self = cls.__new__(cls)
+        d, keep = spec.desc(precision, ext_mode, record)
+        self._keep = keep
+        self.replicas = 1
+        self.n = self.n_base = int(np.sum(spec.n_neuron))
+        self.n_pop = self.n_pop_base = 2 * len(spec.n_neuron)
+        self.dt = float(spec.dt)
+        h = C.c_void_p()
+        A.check(A.lib().fs_create_ei_batch(C.byref(d), C.byref(h)))
+        self._h = h
+        self.precision = precision
+        return self
 
     def set_replica(self, replica: int, gain=(1.0, 1.0, 1.0, 1.0, 1.0), ext_gain: float = 1.0,
                     stream_key: Optional[int] = None) -> None:

@@ -1,0 +1,253 @@
+"""Surrogate-screened parameter sweeps of the 20k-neuron fly-like network.
+
+Stages (each a command; files are numpy .npz):
+
+  gen     simulate N random sweep points (per-module external rates + 4 gains)
+          on the GPU, B points per launch -> training set
+  train   fit the rate surrogate on the GPU (BF16 MLP, Adam) -> model
+  screen  predict M sweep points, flag predicted mean rate > threshold-margin,
+          re-simulate every flagged point plus a random audit of the rest
+          (optionally a slice of that list per call) -> verified rates
+  report  merge the screen slices: hyperactive points found, surrogate error
+          against the simulator, points/s of plain per-instance GPU simulation,
+          batched simulation and batched + surrogate
+
+    python -m flysim_amd.sweep gen --points 100000 --out d/train.npz
+    python -m flysim_amd.sweep train --data d/train.npz --out d/model.npz
+    python -m flysim_amd.sweep screen --model d/model.npz --points 1000000 --out d/screen.npz
+    python -m flysim_amd.sweep report --train d/train.npz --model d/model.npz --screen d/screen*.npz
+
+A sweep point's rates are what one run of the network with those external
+rates and gains gives over a 200 ms window after 100 ms of warm-up (dt 0.1 ms).
+"""
+from __future__ import annotations
+
+import argparse
+import glob
+import json
+import sys
+import time
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import workloads as W
+from .surrogate import RateSurrogate, SweepSimulator, SweepSpace, screen_sweep
+
+WARM_STEPS = 1000
+MEAS_STEPS = 2000
+THRESHOLD_HZ = 50.0
+
+
+def network():
+    return W.c2_network(seed=2)
+
+
+def make_sim(replicas: int, warm: int = WARM_STEPS, meas: int = MEAS_STEPS,
+             schedule: Optional[Dict[str, int]] = None) -> SweepSimulator:
+    if schedule is None:
+        schedule = {"gather_lanes": 1} if replicas >= 64 else {}
+    return SweepSimulator(network(), replicas=replicas, warm_steps=warm, meas_steps=meas,
+                          schedule=schedule)
+
+
+def gen(points: int, seed: int, replicas: int, warm: int = WARM_STEPS, meas: int = MEAS_STEPS,
+        log=print) -> Dict[str, np.ndarray]:
+    space = SweepSpace()
+    X = space.sample(points, np.random.default_rng(seed))
+    sim = make_sim(replicas, warm, meas)
+    Y = np.zeros((points, sim.n_mod), np.float32)
+    chunk = replicas * 16
+    t0 = time.perf_counter()
+    for a in range(0, points, chunk):
+        b = min(points, a + chunk)
+        Y[a:b] = sim.run(X[a:b], seed=seed * 7919 + a)
+        log("gen %d/%d points, %.1f points/s" % (b, points, b / (time.perf_counter() - t0)))
+    return {"X": X, "Y": Y, "wall_s": np.float64(time.perf_counter() - t0),
+            "points_per_s": np.float64(sim.points_per_s), "replicas": np.int64(replicas),
+            "warm": np.int64(warm), "meas": np.int64(meas)}
+
+
+def train(X: np.ndarray, Y: np.ndarray, epochs: int = 60, batch: int = 512, lr: float = 1e-3,
+          holdout: float = 0.1, seed: int = 0) -> Dict[str, np.ndarray]:
+    n = X.shape[0]
+    rng = np.random.default_rng(seed)
+    idx = rng.permutation(n)
+    n_val = int(round(holdout * n))
+    val, tr = idx[:n_val], idx[n_val:]
+    sur = RateSurrogate(X.shape[1], Y.shape[1], seed=seed)
+    t0 = time.perf_counter()
+    losses = sur.fit(X[tr], Y[tr], epochs=epochs, batch=min(batch, len(tr)), lr=lr, seed=seed)
+    t_fit = time.perf_counter() - t0
+    out = {"params": sur.mlp.params, "mu": sur.mu, "sd": sur.sd, "losses": losses,
+           "fit_s": np.float64(t_fit), "val_idx": val}
+    if n_val:
+        P = sur.predict(X[val])
+        err = np.abs(P - Y[val])
+        mt, mp = Y[val].mean(axis=1) > THRESHOLD_HZ, P.mean(axis=1) > THRESHOLD_HZ
+        out.update(val_mae_hz=np.float64(err.mean()),
+                   val_mae_sub_hz=np.float64(err[~mt].mean() if (~mt).any() else 0.0),
+                   val_acc=np.float64((mt == mp).mean()))
+    return out
+
+
+def load_surrogate(model: Dict[str, np.ndarray], n_in: int, n_out: int) -> RateSurrogate:
+    sur = RateSurrogate(n_in, n_out)
+    sur.mlp.params = model["params"]
+    sur.mu, sur.sd = model["mu"].astype(np.float32), model["sd"].astype(np.float32)
+    return sur
+
+
+def screen(model: Dict[str, np.ndarray], points: int, seed: int, replicas: int,
+           margin_hz: float = 10.0, audit_frac: float = 0.01, part: int = 0, parts: int = 1,
+           log=print) -> Dict[str, np.ndarray]:
+    """Screen `points` sweep points; simulate part `part` of `parts` of the
+    flagged + audit list."""
+    space = SweepSpace()
+    X = space.sample(points, np.random.default_rng(seed))
+    sur = load_surrogate(model, X.shape[1], space.n_mod)
+    t0 = time.perf_counter()
+    pred = sur.predict(X)
+    t_pred = time.perf_counter() - t0
+    mean = pred.mean(axis=1)
+    flagged = np.nonzero(mean > THRESHOLD_HZ - margin_hz)[0]
+    rest = np.setdiff1d(np.arange(points), flagged)
+    rng = np.random.default_rng(seed + 1)
+    audited = np.sort(rng.choice(rest, size=int(round(audit_frac * rest.size)), replace=False))
+    todo = np.concatenate([flagged, audited])
+    mine = np.array_split(todo, parts)[part]
+    sim = make_sim(replicas)
+    rates = np.zeros((mine.size, space.n_mod), np.float32)
+    chunk = replicas * 16
+    t1 = time.perf_counter()
+    for a in range(0, mine.size, chunk):
+        b = min(mine.size, a + chunk)
+        rates[a:b] = sim.run(X[mine[a:b]], seed=seed * 7919 + int(mine[a]))
+        log("screen part %d/%d: verified %d/%d points, %.1f points/s"
+            % (part + 1, parts, b, mine.size, b / (time.perf_counter() - t1)))
+    t_sim = time.perf_counter() - t1
+    # rates and predictions stored as float16 (11-bit mantissa: ~0.05 %)
+    return {"points": np.int64(points), "seed": np.int64(seed), "flagged": flagged,
+            "audited": audited, "idx": mine, "rates": rates.astype(np.float16),
+            "pred": pred[mine].astype(np.float16),
+            "pred_mean_all": mean.astype(np.float32), "t_predict_s": np.float64(t_pred),
+            "predict_gpu_ms": np.float64(sur.mlp.last_ms), "t_sim_s": np.float64(t_sim),
+            "sim_points_per_s": np.float64(mine.size / t_sim if t_sim > 0 else 0.0)}
+
+
+def plain_rate(points: int, seed: int) -> float:
+    """Points/s of plain per-instance GPU simulation (one network per run)."""
+    X = SweepSpace().sample(points, np.random.default_rng(seed))
+    sim = make_sim(1)
+    sim.run(X[:1], seed=seed)     # warm-up (graph capture)
+    sim.points, sim.wall_seconds = 0, 0.0
+    sim.run(X, seed=seed)
+    return sim.points_per_s
+
+
+def report(train_d, model, screens, plain_pts_s: float, batched_pts_s: float) -> Dict:
+    points = int(screens[0]["points"])
+    flagged = screens[0]["flagged"]
+    audited = screens[0]["audited"]
+    idx = np.concatenate([s["idx"] for s in screens])
+    rates = np.concatenate([s["rates"] for s in screens]).astype(np.float32)
+    pred = np.concatenate([s["pred"] for s in screens]).astype(np.float32)
+    is_f = np.isin(idx, flagged)
+    sim_mean = rates.mean(axis=1)
+    hyper = sim_mean > THRESHOLD_HZ
+    err = np.abs(pred - rates)
+    n_todo = flagged.size + audited.size
+    t_sim = sum(float(s["t_sim_s"]) for s in screens)
+    verified = idx.size
+    # time of the full screen: prediction + simulation of every flagged and
+    # audited point (measured when all parts ran, else at the measured rate)
+    t_verify = t_sim if verified == n_todo else n_todo / (verified / t_sim)
+    t_gen = float(train_d["wall_s"])
+    t_fit = float(model["fit_s"])
+    t_pred = float(screens[0]["t_predict_s"])
+    t_total = t_gen + t_fit + t_pred + t_verify
+    t_plain = points / plain_pts_s
+    t_batched = points / batched_pts_s
+    audit_hyper = hyper[~is_f]
+    rep = {
+        "points": points, "threshold_hz": THRESHOLD_HZ,
+        "flagged": int(flagged.size), "audited": int(audited.size),
+        "verified_this_run": int(verified), "to_verify": int(n_todo),
+        "hyperactive_among_flagged": int(hyper[is_f].sum()),
+        "precision_flagged": float(hyper[is_f].mean()) if is_f.any() else None,
+        "audit_hyperactive_missed": int(audit_hyper.sum()),
+        "audit_miss_rate": float(audit_hyper.mean()) if audit_hyper.size else None,
+        "est_missed_total": float(audit_hyper.mean() * (points - flagged.size)) if audit_hyper.size else None,
+        "surrogate_mae_hz_all": float(err.mean()),
+        "surrogate_mae_hz_subthreshold": float(err[~hyper].mean()) if (~hyper).any() else None,
+        "surrogate_module_mae_hz_subthreshold_max": float(err[~hyper].mean(axis=0).max()) if (~hyper).any() else None,
+        "train_points": int(train_d["X"].shape[0]), "train_gen_s": t_gen, "fit_s": t_fit,
+        "val_mae_hz": float(model.get("val_mae_hz", np.nan)),
+        "predict_s": t_pred, "predict_gpu_ms": float(screens[0]["predict_gpu_ms"]),
+        "verify_s": t_verify, "total_s": t_total,
+        "plain_points_per_s": plain_pts_s, "batched_points_per_s": batched_pts_s,
+        "plain_total_s": t_plain, "batched_total_s": t_batched,
+        "speedup_vs_plain": t_plain / t_total, "speedup_vs_plain_excl_training": t_plain / (t_total - t_gen - t_fit),
+        "speedup_vs_batched": t_batched / t_total,
+        "effective_points_per_s": points / t_total,
+    }
+    return rep
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m flysim_amd.sweep")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    g = sub.add_parser("gen")
+    g.add_argument("--points", type=int, default=100000)
+    g.add_argument("--seed", type=int, default=1)
+    g.add_argument("--replicas", type=int, default=128)
+    g.add_argument("--out", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--data", required=True)
+    t.add_argument("--epochs", type=int, default=60)
+    t.add_argument("--batch", type=int, default=512)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--out", required=True)
+    s = sub.add_parser("screen")
+    s.add_argument("--model", required=True)
+    s.add_argument("--points", type=int, default=1000000)
+    s.add_argument("--seed", type=int, default=2)
+    s.add_argument("--replicas", type=int, default=128)
+    s.add_argument("--part", type=int, default=0)
+    s.add_argument("--parts", type=int, default=1)
+    s.add_argument("--out", required=True)
+    r = sub.add_parser("report")
+    r.add_argument("--train", required=True)
+    r.add_argument("--model", required=True)
+    r.add_argument("--screen", required=True, nargs="+")
+    r.add_argument("--plain-points", type=int, default=64)
+    r.add_argument("--out", default="")
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    if a.cmd == "gen":
+        np.savez(a.out, **gen(a.points, a.seed, a.replicas, log=log))
+    elif a.cmd == "train":
+        d = np.load(a.data)
+        m = train(d["X"], d["Y"], epochs=a.epochs, batch=a.batch, lr=a.lr)
+        log("trained: final loss %.4g, holdout MAE %.2f Hz, fit %.1f s"
+            % (m["losses"][-1], m.get("val_mae_hz", np.nan), m["fit_s"]))
+        np.savez(a.out, **m)
+    elif a.cmd == "screen":
+        m = dict(np.load(a.model))
+        np.savez(a.out, **screen(m, a.points, a.seed, a.replicas, part=a.part, parts=a.parts, log=log))
+    else:
+        tr = np.load(a.train)
+        m = dict(np.load(a.model))
+        files = sorted(set(f for p in a.screen for f in glob.glob(p)))
+        scr = [dict(np.load(f)) for f in files]
+        plain = plain_rate(a.plain_points, 99)
+        batched = float(np.mean([float(s["sim_points_per_s"]) for s in scr]))
+        rep = report(tr, m, scr, plain, batched)
+        txt = json.dumps(rep, indent=1)
+        print(txt)
+        if a.out:
+            open(a.out, "w").write(txt)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,92 @@
+"""Autotuner of the engine's launch schedule.
+
+Times short runs of candidate schedules (neuron-update block size, NMDA
+gather lanes per row, delivery block size, steps per captured graph, fused
+one-launch step on/off) on the network at hand and keeps the fastest.  Results
+are cached in a JSON file keyed by a signature of the network's shape (neuron
+and synapse counts by class, delays, precision, device) so that sweeps of many
+same-shaped networks tune once.
+"""
+from __future__ import annotations
+
+import hashlib
+import itertools
+import json
+import os
+import time
+from typing import Dict, Iterable, List, Optional
+
+import numpy as np
+
+from .engine import Engine
+
+DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "tune_cache.json")
+
+
+def signature(eng: Engine, extra: str = "") -> str:
+    s = eng.schedule
+    key = "%d|%d|%d|%s" % (eng.n, eng.n_pop, eng.precision, extra)
+    return hashlib.sha1(key.encode()).hexdigest()[:16]
+
+
+def candidates(n_neuron: int, has_nmda: bool = True, fused_ok: bool = True) -> List[Dict]:
+    blocks = [64, 128, 256, 512, 1024]
+    lanes = [1, 2, 4, 8, 16, 32] if has_nmda else [8]
+    graph = [100, 500]
+    out = []
+    for b, L, k in itertools.product(blocks, lanes, graph):
+        for f in ([1, 0] if fused_ok else [0]):
+            out.append(dict(update_block=b, gather_lanes=L, push_block=256, steps_per_graph=k,
+                            use_graph=1, fused=f))
+    return out
+
+
+def time_schedule(eng: Engine, sched: Dict, steps: int, warmup: int) -> float:
+    eng.set_schedule(**sched)
+    eng.run(warmup, record=False)
+    t0 = time.perf_counter()
+    eng.run(steps, record=False)
+    return (time.perf_counter() - t0) / steps
+
+
+def tune(eng: Engine, steps: int = 400, warmup: int = 100, cands: Optional[Iterable[Dict]] = None,
+         cache: Optional[str] = DEFAULT_CACHE, key_extra: str = "", verbose: bool = False) -> Dict:
+    """Pick the fastest schedule for `eng` (state is advanced by the trial
+    runs; reset the engine afterwards).  Returns {"schedule", "s_per_step",
+    "trials"} and applies the schedule."""
+    sig = signature(eng, key_extra)
+    db = {}
+    if cache and os.path.exists(cache):
+        try:
+            db = json.load(open(cache))
+        except (OSError, ValueError):
+            db = {}
+    if sig in db:
+        eng.set_schedule(**db[sig]["schedule"])
+        return db[sig]
+    if cands is None:
+        cands = candidates(eng.n)
+    trials = []
+    best = None
+    for sc in cands:
+        try:
+            dt = time_schedule(eng, sc, steps, warmup)
+        except Exception as e:  # an invalid candidate is skipped, not fatal
+            if verbose:
+                print("schedule", sc, "failed:", e)
+            continue
+        trials.append({"schedule": sc, "s_per_step": dt})
+        if verbose:
+            print("%-90s %.2f us/step" % (sc, dt * 1e6))
+        if best is None or dt < best["s_per_step"]:
+            best = {"schedule": sc, "s_per_step": dt}
+    if best is None:
+        raise RuntimeError("no schedule candidate ran")
+    eng.set_schedule(**best["schedule"])
+    res = {"schedule": best["schedule"], "s_per_step": best["s_per_step"], "trials": trials}
+    if cache:
+        db[sig] = {"schedule": best["schedule"], "s_per_step": best["s_per_step"]}
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        with open(cache, "w") as f:
+            json.dump(db, f, indent=1)
+    return res

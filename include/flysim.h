@@ -105,6 +105,33 @@ int fs_create(const fs_net_desc *desc, fs_handle **out);
  * all copies.  replicas = 1 is fs_create. */
 int fs_create_replicated(const fs_net_desc *desc, int32_t replicas, fs_handle **out);
 
+/* Batched random E/I networks generated on the device (no counterpart in the
+ * reference, which runs one network per process and builds it on the host,
+ * code/NetSimGnl2_BulPp.cpp:303-727): n_inst independent instances of
+ * uneven size in one handle, each n[k] neurons (the first
+ * round(exc_frac * n[k]) excitatory), every ordered pair j != p of an
+ * instance a synapse with probability p_conn, efficacy g ~ U(g_lo, g_hi) nS;
+ * E->X pairs carry AMPA (g) and NMDA (g * nmda_ratio * gain_nmda[k]), I->X
+ * pairs GABA (g * gain_gaba[k]); no extra delay (SpikeDly of the population
+ * parameters applies).  Pairs and efficacies come from a counter-based hash
+ * of (seed[k], j, p), which also keys the instance's external input stream,
+ * so an instance runs identically alone or in any batch.  Populations:
+ * 2k = E, 2k+1 = I of instance k, external AMPA at ext_hz[k]. */
+typedef struct fs_ei_batch_desc {
+  int32_t n_inst;
+  const int32_t *n_neuron;              /* [n_inst] */
+  const uint64_t *seed;                 /* [n_inst] */
+  const float *ext_hz;                  /* [n_inst] */
+  const float *gain_nmda, *gain_gaba;   /* [n_inst] */
+  float exc_frac, p_conn, g_lo, g_hi, nmda_ratio;
+  fs_pop_params exc, inh;               /* derived parameters (pb[AMPA] replaced) */
+  float dt;
+  int32_t precision;
+  int32_t ext_mode;
+  int32_t record_spikes;
+} fs_ei_batch_desc;
+int fs_create_ei_batch(const fs_ei_batch_desc *desc, fs_handle **out);
+
 /* Per-copy knobs of a replicated handle: gain[r] multiplies every recurrent
  * efficacy of receptor r, ext_gain every external efficacy, and stream_key
  * is xor-ed into the reset seed to key the copy's counter-based stream (the
