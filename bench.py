@@ -40,6 +40,7 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=-1,
                    help="steps of the CPU-baseline sample (-1: sized to ~15 s)")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-sweep", action="store_true", help="skip the sweep points/s figures")
     p.add_argument("--schedule", default="", help="json dict of fs_schedule overrides")
     p.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                    help="only run 10 device-to-device copies of MIB MiB (the known-size "
@@ -218,10 +219,61 @@ def main():
                      "graph_step_ms": kt["graph_step_ms"],
                      "timed_launches": kt["n_timed"]},
     }
+    traffic = measured_traffic(args.workload, dom, res["dtype"])
+    if traffic:
+        res["roofline"]["traffic"] = traffic
+    eng.close()
+    if not args.no_sweep and args.workload == "fly20k":
+        res["sweep"] = sweep_figures()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(net, arr, ctr["syn_events"] / sim_s, args.cpu_steps)
         res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
     print(json.dumps(res))
+
+
+def measured_traffic(workload, kernel, dtype):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path)).get("%s/%s/%s" % (workload, kernel, dtype))
+    return rec["bytes"] if rec else None
+
+
+def sweep_figures(points_plain=16, replicas=128, mlp_points=1 << 20):
+    """Sweep points/s on the fly20k topology (100 ms warm-up + 200 ms window
+    per point): plain per-instance GPU simulation vs replicated launches of
+    `replicas` points, and the surrogate's forward over 2^20 points (BF16
+    matrix-core GEMMs; GPU time from HIP events) with its MFMA roofline."""
+    from flysim_amd.surrogate import Mlp, SweepSpace
+    from flysim_amd.sweep import make_sim
+    space = SweepSpace()
+    rng = np.random.default_rng(7)
+    plain = make_sim(1)
+    plain.run(space.sample(1, rng), seed=1)            # graph capture outside the timing
+    plain.points, plain.wall_seconds = 0, 0.0
+    plain.run(space.sample(points_plain, rng), seed=2)
+    batched = make_sim(replicas)
+    batched.run(space.sample(replicas, rng), seed=3)
+    batched.points, batched.wall_seconds = 0, 0.0
+    batched.run(space.sample(2 * replicas, rng), seed=4)
+    sizes = [space.dim, 256, 256, 256, space.n_mod]
+    mlp = Mlp(sizes, seed=0)
+    x = space.sample(mlp_points, rng)
+    mlp.predict(x[:4096])
+    mlp.predict(x)
+    flop = 2.0 * sum(sizes[i] * sizes[i + 1] for i in range(len(sizes) - 1)) * mlp_points
+    tflops = flop / (mlp.last_ms * 1e-3) / 1e12
+    return {"topology": "fly20k", "window_ms": 200, "warmup_ms": 100,
+            "plain_points_per_s": plain.points_per_s,
+            "batched_points_per_s": batched.points_per_s, "replicas": replicas,
+            "batched_vs_plain": batched.points_per_s / plain.points_per_s,
+            "surrogate_points_per_s": mlp_points / (mlp.last_ms * 1e-3),
+            "surrogate_roofline": {"bound": "mfma", "kernel": "k_gemm (forward, 4 layers)",
+                                   "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                                   "frac": tflops / 2500.0, "points": mlp_points,
+                                   "gpu_ms": mlp.last_ms}}
 
 
 if __name__ == "__main__":

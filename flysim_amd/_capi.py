@@ -107,7 +107,7 @@ EXPORTS = [
     "fs_kernel_times", "fs_run_counters", "fs_destroy", "fs_last_error", "fs_conf_load",
     "fs_conf_free", "fs_conf_desc", "fs_conf_sizes", "fs_conf_seed", "fs_conf_ext_trains",
     "fs_conf_ref_layout", "fs_conf_write_outputs", "fs_cli_main", "fs_create_replicated",
-    "fs_set_replica", "fs_create_ei_batch", "fs_mlp_create", "fs_mlp_destroy", "fs_mlp_n_params", "fs_mlp_get_params",
+    "fs_set_replica", "fs_create_ei_batch", "fs_set_ext_rates", "fs_mlp_create", "fs_mlp_destroy", "fs_mlp_n_params", "fs_mlp_get_params",
     "fs_mlp_set_params", "fs_mlp_loss_grad", "fs_mlp_predict", "fs_mlp_set_data", "fs_mlp_train",
     "fs_device_alloc", "fs_device_free", "fs_device_copy",
 ]
@@ -137,6 +137,7 @@ def lib() -> C.CDLL:
             "fs_create_replicated": [P(NetDesc), C.c_int32, P(C.c_void_p)],
             "fs_set_replica": [C.c_void_p, C.c_int32, P(ReplicaParams)],
             "fs_create_ei_batch": [P(EiBatchDesc), P(C.c_void_p)],
+            "fs_set_ext_rates": [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32],
             "fs_reset": [C.c_void_p, C.c_uint64],
             "fs_set_ext_rate": [C.c_void_p, C.c_int32, C.c_int32, C.c_float],
             "fs_set_injection": [C.c_void_p, C.c_int32, C.c_float, C.c_float],

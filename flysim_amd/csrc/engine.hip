@@ -971,6 +971,18 @@ int fs_set_ext_rate(fs_handle *h, int32_t pop, int32_t receptor, float hz) {
   })
 }
 
+int fs_set_ext_rates(fs_handle *h, int32_t receptor, const float *hz, int32_t n) {
+  if (!h || !hz) return fs::set_error(FS_ERR_ARG, "null argument");
+  if (receptor < 0 || receptor > FS_GCL) return fs::set_error(FS_ERR_ARG, "receptor out of range");
+  if (n != h->e->n_pop) return fs::set_error(FS_ERR_ARG, "hz must hold one rate per population");
+  ENGINE_CALL({
+    for (int32_t g = 0; g < n; ++g) {
+      if (hz[g] == 0.0f) h->e->set_ext(g, receptor, 0.0f, false);
+      else h->e->set_ext(g, receptor, (float)(hz[g] * h->e->dt * 1e-3f), true);
+    }
+  })
+}
+
 int fs_set_replica(fs_handle *h, int32_t replica, const fs_replica_params *p) {
   if (!h || !p) return fs::set_error(FS_ERR_ARG, "null argument");
   ENGINE_CALL(h->e->set_replica(replica, *p))

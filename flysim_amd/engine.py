@@ -325,6 +325,11 @@ class Engine:
     def set_ext_rate(self, pop: int, receptor: int, hz: float) -> None:
         A.check(A.lib().fs_set_ext_rate(self._h, pop, receptor, C.c_float(hz)))
 
+    def set_ext_rates(self, receptor: int, hz) -> None:
+        """External rate of every population (global ids) at once."""
+        self._hz = np.ascontiguousarray(hz, dtype=np.float32)
+        A.check(A.lib().fs_set_ext_rates(self._h, receptor, self._hz.ctypes.data, self._hz.size))
+
     def set_injection(self, pop: int, mean: float, std: float) -> None:
         A.check(A.lib().fs_set_injection(self._h, pop, C.c_float(mean), C.c_float(std)))
 

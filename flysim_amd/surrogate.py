@@ -170,7 +170,7 @@ class SweepSimulator:
         self.eng.set_replica(r, gain=(gA, gG, gN, 1.0, 1.0), ext_gain=gE, stream_key=key)
         for m, pops in enumerate(self.mods):
             for pi in pops:
-                self.eng.set_ext_rate(r * self.P0 + pi, AMPA, float(x[m]) * 1000.0)
+                self._hz[r * self.P0 + pi] = float(x[m]) * 1000.0
 
     def run(self, X: np.ndarray, seed: int = 0) -> np.ndarray:
         """Rates (Hz) [n, n_mod] over the measurement window after warm-up."""
@@ -180,10 +180,12 @@ class SweepSimulator:
         t0 = time.perf_counter()
         for b0 in range(0, n, self.B):
             k = min(self.B, n - b0)
+            self._hz = np.zeros(self.B * self.P0, np.float32)
             for r in range(self.B):
                 # unused replicas of a partial batch repeat the last point
                 i = b0 + min(r, k - 1)
                 self._apply(r, X[i], key=(seed * 1_000_003 + i) * 0x9E3779B97F4A7C15 % (1 << 64))
+            self.eng.set_ext_rates(AMPA, self._hz)
             self.eng.reset(seed)
             if self.warm:
                 self.eng.run(self.warm, record=False)

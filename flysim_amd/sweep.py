@@ -216,11 +216,14 @@ def main(argv=None):
     s.add_argument("--part", type=int, default=0)
     s.add_argument("--parts", type=int, default=1)
     s.add_argument("--out", required=True)
+    pl = sub.add_parser("plain")
+    pl.add_argument("--points", type=int, default=64)
+    pl.add_argument("--out", required=True)
     r = sub.add_parser("report")
     r.add_argument("--train", required=True)
     r.add_argument("--model", required=True)
     r.add_argument("--screen", required=True, nargs="+")
-    r.add_argument("--plain-points", type=int, default=64)
+    r.add_argument("--plain", required=True, help="json written by the plain command")
     r.add_argument("--out", default="")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
@@ -235,12 +238,16 @@ def main(argv=None):
     elif a.cmd == "screen":
         m = dict(np.load(a.model))
         np.savez(a.out, **screen(m, a.points, a.seed, a.replicas, part=a.part, parts=a.parts, log=log))
+    elif a.cmd == "plain":
+        rate = plain_rate(a.points, 99)
+        log("plain per-instance simulation: %.2f points/s" % rate)
+        json.dump({"points": a.points, "points_per_s": rate}, open(a.out, "w"))
     else:
         tr = np.load(a.train)
         m = dict(np.load(a.model))
         files = sorted(set(f for p in a.screen for f in glob.glob(p)))
         scr = [dict(np.load(f)) for f in files]
-        plain = plain_rate(a.plain_points, 99)
+        plain = float(json.load(open(a.plain))["points_per_s"])
         batched = float(np.mean([float(s["sim_points_per_s"]) for s in scr]))
         rep = report(tr, m, scr, plain, batched)
         txt = json.dumps(rep, indent=1)

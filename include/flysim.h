@@ -152,6 +152,8 @@ int fs_reset(fs_handle *h, uint64_t seed);
  * hz == 0 clears the receptor's activity bit (no draws, external gate frozen),
  * otherwise sets it and pb = (float)(hz*dt*1e-3f). */
 int fs_set_ext_rate(fs_handle *h, int32_t pop, int32_t receptor, float hz);
+/* The same for every population at once: hz[n_pop] (sweep drivers). */
+int fs_set_ext_rates(fs_handle *h, int32_t receptor, const float *hz, int32_t n_pop);
 
 /* ChangeMembraneNoise / CurrentInjec semantics (code/NetSimGnl2.cpp:120-133). */
 int fs_set_injection(fs_handle *h, int32_t pop, float mean, float std_dev);

@@ -73,7 +73,7 @@ def test_instance_in_batch_runs_as_alone():
     ea.reset(9)
     ra = ea.raster(steps)
     mine = rb[(rb[:, 1] >= 400) & (rb[:, 1] < 1050)] - np.array([0, 400])
-    assert len(ra) > 100
+    assert len(ra) > 50
     np.testing.assert_array_equal(mine, ra)
     np.testing.assert_array_equal(eb.pop_counts()[2:4], ea.pop_counts())
 
