@@ -113,3 +113,92 @@ def ei_batch_network(spec: EiBatchSpec) -> W.Network:
 def expected_pairs(spec: EiBatchSpec) -> float:
     n = spec.n_neuron.astype(np.float64)
     return float(np.sum(n * (n - 1)) * spec.p_conn)
+
+
+def subset(spec: EiBatchSpec, idx) -> EiBatchSpec:
+    """The instances `idx` of a batch as a batch of their own."""
+    idx = np.atleast_1d(idx)
+    return EiBatchSpec(spec.n_neuron[idx], spec.seed[idx], spec.ext_hz[idx], spec.gain_nmda[idx],
+                       spec.gain_gaba[idx], spec.exc, spec.inh, spec.exc_frac, spec.p_conn,
+                       spec.g_lo, spec.g_hi, spec.nmda_ratio, spec.dt)
+
+
+def run_batch(spec: EiBatchSpec, steps: int, seed: int = 1, schedule=None, log=print):
+    """Generate and run all instances in one handle; per-instance E/I rates."""
+    import time
+    from . import _capi as A
+    t0 = time.perf_counter()
+    eng = Engine.ei_batch(spec, precision=A.FS_FP32, record=False)
+    t_gen = time.perf_counter() - t0
+    if schedule:
+        eng.set_schedule(**schedule)
+    eng.reset(seed)
+    eng.enable_timing(True)
+    t1 = time.perf_counter()
+    eng.run(steps, record=False)
+    wall = time.perf_counter() - t1
+    ctr = eng.counters()
+    kt = eng.kernel_times()
+    counts = eng.pop_counts().astype(np.float64).reshape(-1, 2)
+    ne = spec.n_exc()
+    sizes = np.stack([ne, spec.n_neuron - ne], axis=1).astype(np.float64)
+    rates = counts / np.maximum(sizes, 1) / (steps * spec.dt * 1e-3)
+    out = {"t_gen_s": t_gen, "wall_s": wall, "ms_per_step": wall / steps * 1e3,
+           "counters": ctr, "kernel_times": kt, "rates": rates, "schedule": eng.schedule}
+    eng.close()
+    return out
+
+
+def main(argv=None):
+    """Batched vs plain per-instance simulation of the 256-instance sweep."""
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser(prog="python -m flysim_amd.batch")
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--plain", type=int, default=256,
+                    help="instances also run one by one (0: skip)")
+    ap.add_argument("--lanes", type=int, default=16, help="NMDA gather lanes per row")
+    ap.add_argument("--out", default="")
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    spec = c4_spec(n_inst=a.instances)
+    sched = {"gather_lanes": a.lanes}
+    n = spec.n_neuron.astype(np.float64)
+    ne = spec.n_exc().astype(np.float64)
+    pairs_expect = float(np.sum(n * (n - 1)) * spec.p_conn)
+    log("batch: %d instances, %d neurons, ~%.3g pairs" % (n.size, n.sum(), pairs_expect))
+    b = run_batch(spec, a.steps, schedule=sched, log=log)
+    nmda = b["counters"]["nmda_terms"] / a.steps
+    log("batched: gen %.1f s, %.3f ms/step, wall %.1f s" % (b["t_gen_s"], b["ms_per_step"], b["wall_s"]))
+    res = {"instances": int(n.size), "neurons": int(n.sum()), "steps": a.steps,
+           "sim_seconds_each": a.steps * spec.dt * 1e-3,
+           "e_pairs_nmda": nmda, "pairs_expected": pairs_expect,
+           "receptor_synapses": float(nmda * 2 + (pairs_expect - nmda)),
+           "batched": {k: v for k, v in b.items() if k not in ("rates",)},
+           "rates_quantiles_hz": np.quantile(b["rates"].mean(axis=1), [0, .25, .5, .75, 1]).tolist(),
+           "frac_mean_rate_over_50hz": float((b["rates"].mean(axis=1) > 50).mean())}
+    # gather bytes per step: 8 B per NMDA term (column+weight) + a 4 B gate read
+    res["batched"]["gather_bytes_per_step"] = 12.0 * nmda + 8.0 * n.sum()
+    if a.plain:
+        t0 = time.perf_counter()
+        walls = []
+        for k in range(min(a.plain, n.size)):
+            r = run_batch(subset(spec, k), a.steps, schedule=sched)
+            walls.append(r["t_gen_s"] + r["wall_s"])
+            if (k + 1) % 16 == 0:
+                log("plain: %d instances, %.1f s" % (k + 1, time.perf_counter() - t0))
+        plain = float(np.sum(walls)) * n.size / len(walls)
+        res["plain"] = {"instances_run": len(walls), "total_s": plain,
+                        "instances_per_s": n.size / plain}
+        res["batched_instances_per_s"] = n.size / (b["t_gen_s"] + b["wall_s"])
+        res["speedup_batched_vs_plain"] = plain / (b["t_gen_s"] + b["wall_s"])
+    txt = json.dumps(res, default=float)
+    print(txt)
+    if a.out:
+        open(a.out, "w").write(txt)
+
+
+if __name__ == "__main__":
+    main()

@@ -24,19 +24,25 @@ DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib",
 
 
 def signature(eng: Engine, extra: str = "") -> str:
-    s = eng.schedule
-    key = "%d|%d|%d|%s" % (eng.n, eng.n_pop, eng.precision, extra)
+    key = "%d|%d|%d|%d|%s" % (eng.n, eng.n_pop, getattr(eng, "replicas", 1), eng.precision, extra)
     return hashlib.sha1(key.encode()).hexdigest()[:16]
 
 
-def candidates(n_neuron: int, has_nmda: bool = True, fused_ok: bool = True) -> List[Dict]:
-    blocks = [64, 128, 256, 512, 1024]
-    lanes = [1, 2, 4, 8, 16, 32] if has_nmda else [8]
-    graph = [100, 500]
+def candidates(replicas: int = 1, has_nmda: bool = True, fused_ok: bool = True) -> List[Dict]:
+    """Block sizes x NMDA lanes per row x fused on/off.  Replicas in whole
+    waves read one CSR row per wave, so one lane per row is the natural
+    choice there; single networks spread a row over 1-16 lanes."""
+    blocks = [128, 256, 512, 1024]
+    if not has_nmda:
+        lanes = [8]
+    elif replicas >= 64:
+        lanes = [1, 2]
+    else:
+        lanes = [1, 2, 4, 8, 16]
     out = []
-    for b, L, k in itertools.product(blocks, lanes, graph):
+    for b, L in itertools.product(blocks, lanes):
         for f in ([1, 0] if fused_ok else [0]):
-            out.append(dict(update_block=b, gather_lanes=L, push_block=256, steps_per_graph=k,
+            out.append(dict(update_block=b, gather_lanes=L, push_block=256, steps_per_graph=100,
                             use_graph=1, fused=f))
     return out
 
@@ -65,7 +71,7 @@ def tune(eng: Engine, steps: int = 400, warmup: int = 100, cands: Optional[Itera
         eng.set_schedule(**db[sig]["schedule"])
         return db[sig]
     if cands is None:
-        cands = candidates(eng.n)
+        cands = candidates(getattr(eng, "replicas", 1))
     trials = []
     best = None
     for sc in cands:
