@@ -595,11 +595,16 @@ struct Engine final : EngineBase {
     const int dgrid = deliver_grid(&pw);
     net.push_waves = pw;
     if (timed) HIPCHECK(hipEventRecord(ev[0], stream));
+    // replicas in whole waves: one local neuron per wave (scalar-load variants)
+    const bool uni = (B % 64) == 0;
+    const bool uni_row = uni && sched.gather_lanes == 1;
+    const bool rp = net.ext_mode == FS_EXT_REPLAY;
     if (fused()) {
-      if (net.ext_mode == FS_EXT_REPLAY)
-        hipLaunchKernelGGL((fsk::k_step<T, true>), dim3(step_grid()), dim3(sched.update_block), 0, stream, net, k);
-      else
-        hipLaunchKernelGGL((fsk::k_step<T, false>), dim3(step_grid()), dim3(sched.update_block), 0, stream, net, k);
+      const dim3 g(step_grid()), b(sched.update_block);
+      if (rp && uni_row) hipLaunchKernelGGL((fsk::k_step<T, true, true>), g, b, 0, stream, net, k);
+      else if (rp) hipLaunchKernelGGL((fsk::k_step<T, true, false>), g, b, 0, stream, net, k);
+      else if (uni_row) hipLaunchKernelGGL((fsk::k_step<T, false, true>), g, b, 0, stream, net, k);
+      else hipLaunchKernelGGL((fsk::k_step<T, false, false>), g, b, 0, stream, net, k);
       HIPCHECK(hipGetLastError());
       if (timed) {
         HIPCHECK(hipEventRecord(ev[1], stream));
@@ -607,13 +612,19 @@ struct Engine final : EngineBase {
       }
       return;
     }
-    if (net.ext_mode == FS_EXT_REPLAY)
-      hipLaunchKernelGGL((fsk::k_update<T, true>), dim3(update_grid()), dim3(sched.update_block), 0, stream, net, k);
-    else
-      hipLaunchKernelGGL((fsk::k_update<T, false>), dim3(update_grid()), dim3(sched.update_block), 0, stream, net, k);
+    {
+      const dim3 g(update_grid()), b(sched.update_block);
+      if (rp && uni) hipLaunchKernelGGL((fsk::k_update<T, true, true>), g, b, 0, stream, net, k);
+      else if (rp) hipLaunchKernelGGL((fsk::k_update<T, true, false>), g, b, 0, stream, net, k);
+      else if (uni) hipLaunchKernelGGL((fsk::k_update<T, false, true>), g, b, 0, stream, net, k);
+      else hipLaunchKernelGGL((fsk::k_update<T, false, false>), g, b, 0, stream, net, k);
+    }
     HIPCHECK(hipGetLastError());
     if (timed) HIPCHECK(hipEventRecord(ev[1], stream));
-    hipLaunchKernelGGL((fsk::k_deliver<T>), dim3(dgrid), dim3(sched.push_block), 0, stream, net, k);
+    if (uni_row)
+      hipLaunchKernelGGL((fsk::k_deliver<T, true>), dim3(dgrid), dim3(sched.push_block), 0, stream, net, k);
+    else
+      hipLaunchKernelGGL((fsk::k_deliver<T, false>), dim3(dgrid), dim3(sched.push_block), 0, stream, net, k);
     HIPCHECK(hipGetLastError());
     if (timed) HIPCHECK(hipEventRecord(ev[2], stream));
   }

@@ -140,6 +140,14 @@ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1
 }
 __device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
 
+// Load through the constant address space: for tables no kernel writes
+// (CSR, population parameters) a wave-uniform address then becomes a scalar
+// load instead of a 64-lane vector load of one address.
+template <typename X>
+__device__ __forceinline__ X ldc(const X *p) {
+  return *(const __attribute__((address_space(4))) X *)p;
+}
+
 // (t0 % M + k) mod M for a step offset k >= -M of the batch starting at t0
 __device__ __forceinline__ int32_t slot_of(int32_t t0mod, int32_t k, int32_t M) {
   return (int32_t)((uint32_t)(t0mod + k + M) % (uint32_t)M);
@@ -150,7 +158,10 @@ __device__ __forceinline__ int32_t slot_of(int32_t t0mod, int32_t k, int32_t M) 
 // spike compaction; every lane of the wave must reach it (ballots).  With
 // gn_given the NMDA conductance comes from the caller (fused step) instead of
 // the gn array written by the separate gather.
-template <typename T, bool REPLAY>
+// UNI: the caller guarantees that all 64 lanes of the wave hold the same
+// local neuron (replicas in whole waves); its index, population and
+// parameters are then wave-uniform and come through scalar loads.
+template <typename T, bool REPLAY, bool UNI>
 __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int64_t t, int32_t k,
                                               bool gn_given, T gn_in, uint32_t *list_ctr,
                                               uint32_t list_base) {
@@ -160,10 +171,14 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
   uint32_t pid = 0;  // population x replica index of the spike counters
   if (active) {
     const int32_t N = net.N;
-    const int32_t nl = n / net.B, rep = n - nl * net.B;
-    const uint32_t p0 = net.npop[nl];
+    int32_t nl = n / net.B;
+    if constexpr (UNI) nl = __builtin_amdgcn_readfirstlane(nl);
+    const int32_t rep = n - nl * net.B;
+    uint32_t p0 = ldc(net.npop + nl);
+    if constexpr (UNI) p0 = __builtin_amdgcn_readfirstlane(p0);
     pid = p0 * (uint32_t)net.B + (uint32_t)rep;
-    const DevPop<T> &P = net.pops[p0];
+    const __attribute__((address_space(4))) DevPop<T> &P =
+        *(const __attribute__((address_space(4))) DevPop<T> *)(net.pops + p0);
     uint32_t cs = net.cs[n];
     uint32_t cnt = cs & CNT_MASK;
     T v = net.V[n];
@@ -333,9 +348,9 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
   }
 }
 
-template <typename T, bool REPLAY>
+template <typename T, bool REPLAY, bool UNI>
 __global__ void __launch_bounds__(1024) k_update(Net<T> net, int32_t k) {
-  update_neuron<T, REPLAY>(net, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x), *net.t0 + k, k,
+  update_neuron<T, REPLAY, UNI>(net, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x), *net.t0 + k, k,
                            false, T(0), net.total, 0u);
 }
 
@@ -357,22 +372,23 @@ __device__ __forceinline__ void push_spikes(const Net<T> &net, uint32_t start, u
   // s0: ring slot of the emission step; delay d lands in s0 + d wrapped once
   // (d <= maxd < D)
   for (uint32_t i = start + (uint32_t)gwave; i < end; i += (uint32_t)nwaves) {
-    const uint32_t j = net.rec_ids[i];
+    // i is the same in every lane, so the spike and its row are wave-uniform
+    const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)net.rec_ids[i]);
     const uint32_t jl = j / (uint32_t)B, rep = j - jl * (uint32_t)B;
     const T a = net.rec_a[i];
     const T ga = a * T(net.gainr[0 * B + rep]), gg = a * T(net.gainr[1 * B + rep]),
             gh = a * T(net.gainr[3 * B + rep]), gl = a * T(net.gainr[4 * B + rep]);
-    const uint64_t r0 = net.rp_p[jl], r1 = net.rp_p[jl + 1];
-    if (lane == 0) { events += r1 - r0; nout += net.nmda_out[jl]; }
+    const uint64_t r0 = ldc(net.rp_p + jl), r1 = ldc(net.rp_p + jl + 1);
+    if (lane == 0) { events += r1 - r0; nout += ldc(net.nmda_out + jl); }
     for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
-      const uint32_t m = net.meta_p[e];
+      const uint32_t m = ldc(net.meta_p + e);
       const uint32_t col = (m & 0x00FFFFFFu) * (uint32_t)B + rep;
       const uint32_t d = (m >> 24) & 63u;
       const uint32_t li = m >> 30;
       const T s = li == 0 ? ga : li == 1 ? gg : li == 2 ? gh : gl;
       int32_t slot = s0 + (int32_t)d;
       slot -= slot >= net.D ? net.D : 0;
-      atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(net.w_p[e]) * s);
+      atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(ldc(net.w_p + e)) * s);
     }
   }
 }
@@ -398,20 +414,24 @@ __device__ __forceinline__ void block_add_events(const Net<T> &net, unsigned lon
 // NMDA conductance of one row: sum_j w * s_j(ts - d) over CSR-by-post row p,
 // with L lanes of a group striding the row and a shuffle reduction (result
 // valid in every lane of the group).  All lanes of the group must call it.
-template <typename T>
+// UNI: all lanes of the wave gather copies of one local row (L == 1,
+// replicas in whole waves), so the row's columns and weights are scalar loads.
+template <typename T, bool UNI>
 __device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int32_t h0, int L, int sl) {
   T acc = T(0);
   if (p < net.N) {
     const uint32_t B = (uint32_t)net.B;
-    const uint32_t pl = (uint32_t)p / B, rep = (uint32_t)p - pl * B;
-    const uint64_t r0 = net.rp_n[pl], r1 = net.rp_n[pl + 1];
+    uint32_t pl = (uint32_t)p / B;
+    if constexpr (UNI) pl = __builtin_amdgcn_readfirstlane(pl);
+    const uint32_t rep = (uint32_t)p - pl * B;
+    const uint64_t r0 = ldc(net.rp_n + pl), r1 = ldc(net.rp_n + pl + 1);
     const int32_t DN = net.DN;  // h0: history slot of ts; the slot of ts - d wraps once
     const T *sh = net.shist + rep;
     auto term = [&](uint64_t e) -> T {
-      const uint32_t m = net.meta_n[e];
+      const uint32_t m = ldc(net.meta_n + e);
       int32_t hs = h0 - (int32_t)(m >> 24);
       hs += hs < 0 ? DN : 0;
-      return T(net.w_n[e]) * sh[(size_t)hs * net.N + (size_t)(m & 0x00FFFFFFu) * B];
+      return T(ldc(net.w_n + e)) * sh[(size_t)hs * net.N + (size_t)(m & 0x00FFFFFFu) * B];
     };
     // four independent partial sums keep four row entries in flight per lane
     T a1 = T(0), a2 = T(0), a3 = T(0);
@@ -440,7 +460,7 @@ __device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int32_t h0
 // list k-1, appends to list k (zeroed by launch k-1) and zeroes list k+1, so no
 // block waits for another within a launch.  The grid has N*L threads, so
 // small networks still present L times more waves to hide memory latency.
-template <typename T, bool REPLAY>
+template <typename T, bool REPLAY, bool UNI>
 __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
   const int64_t t = *net.t0 + k;
   const int pm = (k + 2) % 3, pc = k % 3, pn = (k + 1) % 3;
@@ -468,9 +488,9 @@ __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
   const int64_t nn = gt / L;
   const int sl = (int)(gt % L);
   T gn = T(0);
-  if (net.has_nmda_syn) gn = gather_row(net, nn, slot_of(net.tmod[1], k - 1, net.DN), L, sl);
+  if (net.has_nmda_syn) gn = gather_row<T, UNI>(net, nn, slot_of(net.tmod[1], k - 1, net.DN), L, sl);
   const int32_t n = (sl == 0 && nn < net.N) ? (int32_t)nn : net.N;  // N: inactive lane
-  update_neuron<T, REPLAY>(net, n, t, k, true, gn, net.lcnt + pc, base_k);
+  update_neuron<T, REPLAY, UNI>(net, n, t, k, true, gn, net.lcnt + pc, base_k);
   block_add_events(net, my_events, my_nout);
 }
 
@@ -497,11 +517,11 @@ template <typename T>
 __global__ void __launch_bounds__(1024) k_gather_only(Net<T> net, int32_t h0) {
   const int lane = (int)(threadIdx.x & 63);
   const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const T v = gather_row(net, p, h0, 64, lane);
+  const T v = gather_row<T, false>(net, p, h0, 64, lane);
   if (lane == 0 && p < net.N) net.gn[p] = v;
 }
 
-template <typename T>
+template <typename T, bool UNI>
 __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
   const int64_t t = *net.t0 + k;
   const int32_t N = net.N;
@@ -529,7 +549,7 @@ __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
     const int sub = lane / L, sl = lane % L;
     const int64_t stride = (int64_t)gwaves * groups_per_wave;
     for (int64_t p = (int64_t)gw * groups_per_wave + sub; p - sub < N; p += stride) {
-      const T acc = gather_row(net, p, h0, L, sl);
+      const T acc = gather_row<T, UNI>(net, p, h0, L, sl);
       if (p < N && sl == 0) net.gn[p] = acc;
     }
   }
