@@ -87,7 +87,7 @@ struct Engine final : EngineBase {
   // device buffers
   DevBuf b_npop, b_pops, b_V, b_cs, b_G, b_ring, b_spre, b_xn, b_shist, b_gn, b_elin, b_enx,
       b_ens, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
-      b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase,
+      b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase, b_suboff, b_subcap,
       b_pbr, b_gain, b_skey, b_tmod;
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
   bool fused() const { return fused_ok && sched.fused; }
@@ -103,7 +103,7 @@ struct Engine final : EngineBase {
   int64_t n_graph_steps = 0;
   int64_t n_timed = 0;
   int64_t syn_lin = 0, syn_nmda = 0;
-  std::vector<uint32_t> cum_h;
+  std::vector<uint32_t> cum_h, hsub_off, hsub_cap, stage;
   std::vector<uint32_t> ids_h;
   unsigned long long ctr_last[4] = {0, 0, 0, 0};
 
@@ -487,10 +487,12 @@ struct Engine final : EngineBase {
     b_elin.alloc((size_t)nlin * n * sizeof(T));
     b_enx.alloc(n * sizeof(T));
     b_ens.alloc(n * sizeof(T));
-    b_total.alloc(4);
+    b_total.alloc(fsk::NSUB * 4);
     b_overflow.alloc(4);
-    b_lcnt.alloc(4 * 4);
-    b_lbase.alloc(4 * 4);
+    b_lcnt.alloc(3 * fsk::NSUB * 4);
+    b_lbase.alloc(3 * fsk::NSUB * 4);
+    b_suboff.alloc(fsk::NSUB * 4);
+    b_subcap.alloc(fsk::NSUB * 4);
     b_t0.alloc(8);
     b_tmod.alloc(8);
     b_popcnt.alloc((size_t)P * B * 8);
@@ -552,18 +554,38 @@ struct Engine final : EngineBase {
       throw std::invalid_argument("steps_per_graph out of range");
     net.gather_lanes = L;
     const int K = sched.steps_per_graph;
-    // record buffer: every neuron could hold SPIKE_F every step (sticky case)
+    // record buffer: every neuron could hold SPIKE_F every step (sticky
+    // case).  Block b of the update grid appends to sub-list b % NSUB, whose
+    // region holds its blocks' neurons for K steps.
+    const int64_t npb = fused() ? ub / L : ub;  // neurons per block
+    std::vector<uint64_t> cov(fsk::NSUB, 0);
+    const int64_t nblk = ((int64_t)N + npb - 1) / npb;
+    for (int64_t b = 0; b < nblk; ++b) cov[b % fsk::NSUB] += std::min<int64_t>(npb, N - b * npb);
     uint64_t cap = (uint64_t)N * (uint64_t)K;
-    const uint64_t cap_max = (uint64_t)1 << 31;
-    if (cap > cap_max) cap = cap_max;
-    if (b_rec_ids.bytes < cap * 4) {
-      b_rec_ids.alloc(cap * 4);
-      b_rec_a.alloc(cap * sizeof(T));
+    const uint64_t cap_max = ((uint64_t)1 << 31) - 1;
+    hsub_off.assign(fsk::NSUB, 0u);
+    hsub_cap.assign(fsk::NSUB, 0u);
+    uint64_t acc = 0;
+    for (int s = 0; s < fsk::NSUB; ++s) {
+      uint64_t c = cov[s] * (uint64_t)K;
+      if (cap > cap_max) c = c * cap_max / cap;  // beyond 2^31 entries: overflow is detected
+      hsub_off[s] = (uint32_t)acc;
+      hsub_cap[s] = (uint32_t)c;
+      acc += c;
     }
-    net.rec_cap = cap;
+    if (b_rec_ids.bytes < acc * 4 || b_rec_a.bytes < acc * sizeof(T)) {
+      b_rec_ids.alloc(acc * 4);
+      b_rec_a.alloc(acc * sizeof(T));
+    }
+    HIPCHECK(hipMemcpy(b_suboff.p, hsub_off.data(), fsk::NSUB * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b_subcap.p, hsub_cap.data(), fsk::NSUB * 4, hipMemcpyHostToDevice));
+    net.sub_off = b_suboff.as<uint32_t>();
+    net.sub_cap = b_subcap.as<uint32_t>();
+    net.rec_cap = acc;
     net.rec_ids = b_rec_ids.as<uint32_t>();
     net.rec_a = b_rec_a.as<T>();
-    if (b_cum.bytes < (size_t)(K + 1) * 4) b_cum.alloc((size_t)(K + 1) * 4);
+    const size_t cum_bytes = (size_t)(K + 1) * fsk::NSUB * 4;
+    if (b_cum.bytes < cum_bytes) b_cum.alloc(cum_bytes);
     net.cum = b_cum.as<uint32_t>();
     graph_dirty = true;
   }
@@ -673,8 +695,8 @@ struct Engine final : EngineBase {
     HIPCHECK(hipMemsetAsync(b_popcnt.p, 0, (size_t)P * B * 8, stream));
     HIPCHECK(hipMemsetAsync(b_counters.p, 0, 3 * fsk::NCTR * 8, stream));
     HIPCHECK(hipMemsetAsync(b_overflow.p, 0, 4, stream));
-    HIPCHECK(hipMemsetAsync(b_lcnt.p, 0, 16, stream));
-    HIPCHECK(hipMemsetAsync(b_lbase.p, 0, 16, stream));
+    HIPCHECK(hipMemsetAsync(b_lcnt.p, 0, 3 * fsk::NSUB * 4, stream));
+    HIPCHECK(hipMemsetAsync(b_lbase.p, 0, 3 * fsk::NSUB * 4, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     net.seed = seed;
     graph_dirty = true;  // the seed is a kernel argument
@@ -759,7 +781,7 @@ struct Engine final : EngineBase {
     int64_t done = 0;
     while (done < nsteps) {
       const int32_t K = (int32_t)std::min<int64_t>(sched.steps_per_graph, nsteps - done);
-      hipLaunchKernelGGL(fsk::k_batch_begin, dim3(1), dim3(1), 0, stream, b_t0.as<int64_t>(), step,
+      hipLaunchKernelGGL(fsk::k_batch_begin, dim3(1), dim3(fsk::NSUB), 0, stream, b_t0.as<int64_t>(), step,
                          b_total.as<uint32_t>(), b_cum.as<uint32_t>(), b_lcnt.as<uint32_t>(),
                          b_lbase.as<uint32_t>(), b_tmod.as<int32_t>(), net.D, net.DN);
       HIPCHECK(hipGetLastError());
@@ -796,18 +818,42 @@ struct Engine final : EngineBase {
       HIPCHECK(hipMemcpy(&ovf, b_overflow.p, 4, hipMemcpyDeviceToHost));
       if (ovf) throw std::length_error("device spike record overflow; lower steps_per_graph");
       if (off || ids) {
-        cum_h.resize(K + 1);
-        HIPCHECK(hipMemcpy(cum_h.data(), b_cum.p, (size_t)(K + 1) * 4, hipMemcpyDeviceToHost));
-        const uint32_t tot = cum_h[K];
+        // cum[k][s]: offset of step k's spikes within sub-list s
+        const int S = fsk::NSUB;
+        cum_h.resize((size_t)(K + 1) * S);
+        HIPCHECK(hipMemcpy(cum_h.data(), b_cum.p, cum_h.size() * 4, hipMemcpyDeviceToHost));
+        uint64_t tot = 0;
+        for (int s = 0; s < S; ++s) tot += cum_h[(size_t)K * S + s];
         if (ids) {
           if (written + (int64_t)tot > cap) throw std::length_error("spike id buffer too small");
-          if (tot) HIPCHECK(hipMemcpy(ids + written, b_rec_ids.p, (size_t)tot * 4, hipMemcpyDeviceToHost));
-          for (int32_t k = 0; k < K; ++k)
-            std::sort(ids + written + cum_h[k], ids + written + cum_h[k + 1]);
+          stage.resize(tot);
+          std::vector<uint64_t> st(S + 1, 0);
+          for (int s = 0; s < S; ++s) {
+            const uint32_t ns = cum_h[(size_t)K * S + s];
+            st[s + 1] = st[s] + ns;
+            if (ns)
+              HIPCHECK(hipMemcpy(stage.data() + st[s], b_rec_ids.as<uint32_t>() + hsub_off[s],
+                                 (size_t)ns * 4, hipMemcpyDeviceToHost));
+          }
+          int64_t pos = written;
+          for (int32_t k = 0; k < K; ++k) {
+            const int64_t p0 = pos;
+            for (int s = 0; s < S; ++s) {
+              const uint32_t a = cum_h[(size_t)k * S + s], b = cum_h[(size_t)(k + 1) * S + s];
+              std::memcpy(ids + pos, stage.data() + st[s] + a, (size_t)(b - a) * 4);
+              pos += b - a;
+            }
+            std::sort(ids + p0, ids + pos);
+          }
         }
-        if (off)
-          for (int32_t k = 0; k < K; ++k) off[done + k + 1] = written + cum_h[k + 1];
-        written += tot;
+        if (off) {
+          int64_t pos = written;
+          for (int32_t k = 0; k < K; ++k) {
+            for (int s = 0; s < S; ++s) pos += cum_h[(size_t)(k + 1) * S + s] - cum_h[(size_t)k * S + s];
+            off[done + k + 1] = pos;
+          }
+        }
+        written += (int64_t)tot;
       }
       step += K;
       done += K;

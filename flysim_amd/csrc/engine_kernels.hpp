@@ -33,6 +33,12 @@ namespace fsk {
 
 // event counters are spread over NCTR slots (block index) against contention
 constexpr int NCTR = 64;
+// The spike record of a batch is split into NSUB sub-lists: block b appends
+// to sub-list b % NSUB, so the claim of each block's range is an atomic on one
+// of NSUB counters instead of one counter for the whole grid.  Sub-list s owns
+// a fixed region of the record sized for every neuron of its blocks spiking
+// every step of the batch.
+constexpr int NSUB = 64;
 
 // state word: refractory count in bits 0..23, flags above
 constexpr uint32_t CNT_MASK = 0x00FFFFFFu;
@@ -110,11 +116,13 @@ struct Net {
   // spike record of the current batch
   uint32_t *rec_ids;
   T *rec_a;
-  uint32_t *total;          // [1] spikes recorded in this batch
-  uint32_t *cum;            // [K+1] per-step offsets
+  uint32_t *total;          // [NSUB] spikes recorded in this batch, per sub-list
+  uint32_t *cum;            // [K+1][NSUB] per-step offsets within each sub-list
   uint32_t *overflow;       // [1]
-  uint32_t *lcnt;           // [3] fused path: spike count of the lists of steps k%3
-  uint32_t *lbase;          // [3] fused path: record offset of the list of step k%3
+  uint32_t *lcnt;           // [3][NSUB] fused path: spike counts of the lists of steps k%3
+  uint32_t *lbase;          // [3][NSUB] fused path: offsets of the list of step k%3
+  const uint32_t *sub_off;  // [NSUB] region of each sub-list in rec_ids / rec_a
+  const uint32_t *sub_cap;  // [NSUB] its capacity
   uint64_t rec_cap;
   const int64_t *t0;        // [1] first step of the batch
   const int32_t *tmod;      // [2] t0 % D, t0 % DN (slot arithmetic stays 32-bit)
@@ -155,7 +163,9 @@ __device__ __forceinline__ int32_t slot_of(int32_t t0mod, int32_t k, int32_t M) 
 
 // ------------------------------------------------------------------ update
 // One neuron's step t (the reference's Activation) followed by the wave's
-// spike compaction; every lane of the wave must reach it (ballots).  With
+// spike compaction; every thread of the block must reach it (ballots, block
+// barriers); list_ctr[NSUB] and list_base are the sub-list counters and the
+// offset of this step's list within each sub-list.  With
 // gn_given the NMDA conductance comes from the caller (fused step) instead of
 // the gn array written by the separate gather.
 // UNI: the caller guarantees that all 64 lanes of the wave hold the same
@@ -323,16 +333,17 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
   if (mask && lane == leader) woff = atomicAdd(&s_cnt, cntw);
   woff = __shfl(woff, leader);
   __syncthreads();
+  const int sub = (int)(blockIdx.x % NSUB);
   if (threadIdx.x == 0 && s_cnt) {
-    s_base = list_base + atomicAdd(list_ctr, s_cnt);
+    s_base = list_base + atomicAdd(list_ctr + sub, s_cnt);
     atomicAdd(&net.counters[blockIdx.x & (NCTR - 1)], (unsigned long long)s_cnt);
   }
   __syncthreads();
   if (spk) {
     const uint32_t pos = s_base + woff + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    if (pos < net.rec_cap) {
-      net.rec_ids[pos] = (uint32_t)n;
-      net.rec_a[pos] = a_push;
+    if (pos < net.sub_cap[sub]) {
+      net.rec_ids[net.sub_off[sub] + pos] = (uint32_t)n;
+      net.rec_a[net.sub_off[sub] + pos] = a_push;
     } else {
       atomicOr(net.overflow, 1u);
     }
@@ -360,36 +371,49 @@ __device__ __forceinline__ void atomic_add_t(T *p, T v) {
   unsafeAtomicAdd(p, v);
 }
 
-// Push the spikes recorded at [start, end) (emitted at step ts) into the
-// ring slots of their synapses' delays; one wave per spike, strided over
-// `nwaves` waves.  Lane 0 of each wave accumulates the event counters.
-template <typename T>
-__device__ __forceinline__ void push_spikes(const Net<T> &net, uint32_t start, uint32_t end,
-                                            int32_t s0, int gwave, int nwaves, int lane,
-                                            unsigned long long &events,
+// Push the spikes of one step (emitted at step ts, ring slot s0) into the
+// ring slots of their synapses' delays; one wave per spike.  range(s, a, b)
+// gives the step's spikes [a, b) within sub-list s; the waves are spread over
+// the NSUB sub-lists (nwaves / NSUB waves each, or several sub-lists per wave
+// when there are fewer waves than sub-lists).  Lane 0 of each wave
+// accumulates the event counters.
+template <typename T, typename Range>
+__device__ __forceinline__ void push_spikes(const Net<T> &net, Range range, int32_t s0, int gwave,
+                                            int nwaves, int lane, unsigned long long &events,
                                             unsigned long long &nout) {
   const int32_t N = net.N, B = net.B;
   // s0: ring slot of the emission step; delay d lands in s0 + d wrapped once
   // (d <= maxd < D)
-  for (uint32_t i = start + (uint32_t)gwave; i < end; i += (uint32_t)nwaves) {
-    // i is the same in every lane, so the spike and its row are wave-uniform
-    const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)net.rec_ids[i]);
-    const uint32_t jl = j / (uint32_t)B, rep = j - jl * (uint32_t)B;
-    const T a = net.rec_a[i];
-    const T ga = a * T(net.gainr[0 * B + rep]), gg = a * T(net.gainr[1 * B + rep]),
-            gh = a * T(net.gainr[3 * B + rep]), gl = a * T(net.gainr[4 * B + rep]);
-    const uint64_t r0 = ldc(net.rp_p + jl), r1 = ldc(net.rp_p + jl + 1);
-    if (lane == 0) { events += r1 - r0; nout += ldc(net.nmda_out + jl); }
-    for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
-      const uint32_t m = ldc(net.meta_p + e);
-      const uint32_t col = (m & 0x00FFFFFFu) * (uint32_t)B + rep;
-      const uint32_t d = (m >> 24) & 63u;
-      const uint32_t li = m >> 30;
-      const T s = li == 0 ? ga : li == 1 ? gg : li == 2 ? gh : gl;
-      int32_t slot = s0 + (int32_t)d;
-      slot -= slot >= net.D ? net.D : 0;
-      atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(ldc(net.w_p + e)) * s);
+  auto one = [&](int s, uint32_t first, uint32_t stride) {
+    uint32_t a = 0, b = 0;
+    range(s, a, b);
+    const uint32_t off = net.sub_off[s];
+    for (uint32_t i = a + first; i < b; i += stride) {
+      // i is the same in every lane, so the spike and its row are wave-uniform
+      const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)net.rec_ids[off + i]);
+      const uint32_t jl = j / (uint32_t)B, rep = j - jl * (uint32_t)B;
+      const T amp = net.rec_a[off + i];
+      const T ga = amp * T(net.gainr[0 * B + rep]), gg = amp * T(net.gainr[1 * B + rep]),
+              gh = amp * T(net.gainr[3 * B + rep]), gl = amp * T(net.gainr[4 * B + rep]);
+      const uint64_t r0 = ldc(net.rp_p + jl), r1 = ldc(net.rp_p + jl + 1);
+      if (lane == 0) { events += r1 - r0; nout += ldc(net.nmda_out + jl); }
+      for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
+        const uint32_t m = ldc(net.meta_p + e);
+        const uint32_t col = (m & 0x00FFFFFFu) * (uint32_t)B + rep;
+        const uint32_t d = (m >> 24) & 63u;
+        const uint32_t li = m >> 30;
+        const T sc = li == 0 ? ga : li == 1 ? gg : li == 2 ? gh : gl;
+        int32_t slot = s0 + (int32_t)d;
+        slot -= slot >= net.D ? net.D : 0;
+        atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(ldc(net.w_p + e)) * sc);
+      }
     }
+  };
+  if (nwaves >= NSUB) {
+    const int per = nwaves / NSUB;
+    if (gwave < per * NSUB) one(gwave % NSUB, (uint32_t)(gwave / NSUB), (uint32_t)per);
+  } else {
+    for (int s = gwave; s < NSUB; s += nwaves) one(s, 0u, 1u);
   }
 }
 
@@ -464,25 +488,26 @@ template <typename T, bool REPLAY, bool UNI>
 __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
   const int64_t t = *net.t0 + k;
   const int pm = (k + 2) % 3, pc = k % 3, pn = (k + 1) % 3;
-  const uint32_t prev_base = net.lbase[pm], prev_cnt = net.lcnt[pm];
-  const uint32_t base_k = prev_base + prev_cnt;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    net.lbase[pc] = base_k;
-    net.lcnt[pn] = 0u;
-    net.cum[k] = base_k < net.rec_cap ? base_k : (uint32_t)net.rec_cap;
+  const uint32_t *pbase = net.lbase + pm * NSUB, *pcnt = net.lcnt + pm * NSUB;
+  if (blockIdx.x == 0 && threadIdx.x < NSUB) {
+    const int s = (int)threadIdx.x;
+    const uint32_t base = pbase[s] + pcnt[s];
+    net.lbase[pc * NSUB + s] = base;
+    net.lcnt[pn * NSUB + s] = 0u;
+    net.cum[(size_t)k * NSUB + s] = base < net.sub_cap[s] ? base : net.sub_cap[s];
   }
   const int wave = (int)(threadIdx.x >> 6);
   const int lane = (int)(threadIdx.x & 63);
   const int wpb = (int)(blockDim.x >> 6);
   unsigned long long my_events = 0, my_nout = 0;
-  {
-    const uint32_t cap = (uint32_t)net.rec_cap;
-    const uint32_t s0 = prev_base < cap ? prev_base : cap;
-    const uint32_t s1 = base_k < cap ? base_k : cap;
-    push_spikes(net, s0, s1, slot_of(net.tmod[0], k - 1, net.D), (int)blockIdx.x * wpb + wave,
-                (int)gridDim.x * wpb, lane,
-                my_events, my_nout);
-  }
+  // spikes of step t-1: [base, base + count) of each sub-list's list k-1
+  auto prev = [&](int s, uint32_t &a, uint32_t &b) {
+    const uint32_t cap = net.sub_cap[s], x = pbase[s], y = pbase[s] + pcnt[s];
+    a = x < cap ? x : cap;
+    b = y < cap ? y : cap;
+  };
+  push_spikes(net, prev, slot_of(net.tmod[0], k - 1, net.D), (int)blockIdx.x * wpb + wave,
+              (int)gridDim.x * wpb, lane, my_events, my_nout);
   const int L = net.gather_lanes;
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nn = gt / L;
@@ -490,7 +515,9 @@ __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
   T gn = T(0);
   if (net.has_nmda_syn) gn = gather_row<T, UNI>(net, nn, slot_of(net.tmod[1], k - 1, net.DN), L, sl);
   const int32_t n = (sl == 0 && nn < net.N) ? (int32_t)nn : net.N;  // N: inactive lane
-  update_neuron<T, REPLAY, UNI>(net, n, t, k, true, gn, net.lcnt + pc, base_k);
+  const int sub = (int)(blockIdx.x % NSUB);
+  update_neuron<T, REPLAY, UNI>(net, n, t, k, true, gn, net.lcnt + pc * NSUB,
+                                pbase[sub] + pcnt[sub]);
   block_add_events(net, my_events, my_nout);
 }
 
@@ -498,16 +525,22 @@ __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
 // offset of the batch's record.
 template <typename T>
 __global__ void __launch_bounds__(1024) k_push_tail(Net<T> net, int32_t K) {
-  const int64_t t = *net.t0 + (K - 1);
   const int pl = (K - 1) % 3;
-  const uint32_t b = net.lbase[pl], e = b + net.lcnt[pl];
-  const uint32_t cap = (uint32_t)net.rec_cap;
-  if (blockIdx.x == 0 && threadIdx.x == 0) net.cum[K] = e < cap ? e : cap;
+  const uint32_t *lb = net.lbase + pl * NSUB, *lc = net.lcnt + pl * NSUB;
+  auto last = [&](int s, uint32_t &a, uint32_t &b) {
+    const uint32_t cap = net.sub_cap[s], x = lb[s], y = lb[s] + lc[s];
+    a = x < cap ? x : cap;
+    b = y < cap ? y : cap;
+  };
+  if (blockIdx.x == 0 && threadIdx.x < NSUB) {
+    uint32_t a, b;
+    last((int)threadIdx.x, a, b);
+    net.cum[(size_t)K * NSUB + threadIdx.x] = b;
+  }
   const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const int wpb = (int)(blockDim.x >> 6);
   unsigned long long ev = 0, no = 0;
-  push_spikes(net, b < cap ? b : cap, e < cap ? e : cap, slot_of(net.tmod[0], K - 1, net.D),
-              (int)blockIdx.x * wpb + wave,
+  push_spikes(net, last, slot_of(net.tmod[0], K - 1, net.D), (int)blockIdx.x * wpb + wave,
               (int)gridDim.x * wpb, lane, ev, no);
   block_add_events(net, ev, no);
 }
@@ -531,14 +564,21 @@ __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
   const int gwave = (int)blockIdx.x * waves_per_block + wave;
   const int total_waves = (int)gridDim.x * waves_per_block;
   unsigned long long my_events = 0, my_nout = 0;
-  const uint32_t end_raw = *net.total;
+  // spikes of step t: [cum[k][s], total[s]) of each sub-list
+  auto cur = [&](int s, uint32_t &a, uint32_t &b) {
+    const uint32_t cap = net.sub_cap[s], y = net.total[s];
+    a = net.cum[(size_t)k * NSUB + s];
+    b = y < cap ? y : cap;
+  };
+  if (blockIdx.x == 0 && threadIdx.x < NSUB) {
+    uint32_t a, b;
+    cur((int)threadIdx.x, a, b);
+    net.cum[(size_t)(k + 1) * NSUB + threadIdx.x] = b;
+  }
   if (gwave < net.push_waves) {
     // ---- push: one wave per spike of step t
-    const uint32_t start = net.cum[k];
-    const uint32_t end = end_raw < net.rec_cap ? end_raw : (uint32_t)net.rec_cap;
-    if (blockIdx.x == 0 && threadIdx.x == 0) net.cum[k + 1] = end;
-    push_spikes(net, start, end, slot_of(net.tmod[0], k, net.D), gwave, net.push_waves, lane,
-                my_events, my_nout);
+    push_spikes(net, cur, slot_of(net.tmod[0], k, net.D), gwave, net.push_waves, lane, my_events,
+                my_nout);
   } else if (net.has_nmda_syn) {
     // ---- NMDA gather over CSR-by-post, L lanes per row
     const int L = net.gather_lanes;
@@ -560,14 +600,19 @@ __global__ void __launch_bounds__(1024) k_deliver(Net<T> net, int32_t k) {
 // (the fused path's list rotation starts with lists 2 (step -1) and 0 empty)
 __global__ void k_batch_begin(int64_t *t0, int64_t value, uint32_t *total, uint32_t *cum,
                               uint32_t *lcnt, uint32_t *lbase, int32_t *tmod, int32_t D, int32_t DN) {
-  *t0 = value;
-  tmod[0] = (int32_t)(value % D);
-  tmod[1] = (int32_t)(value % DN);
-  *total = 0u;
-  cum[0] = 0u;
-  lcnt[0] = 0u;
-  lcnt[2] = 0u;
-  lbase[2] = 0u;
+  const int s = (int)threadIdx.x;  // NSUB threads
+  if (s == 0) {
+    *t0 = value;
+    tmod[0] = (int32_t)(value % D);
+    tmod[1] = (int32_t)(value % DN);
+  }
+  if (s < NSUB) {
+    total[s] = 0u;
+    cum[s] = 0u;
+    lcnt[s] = 0u;
+    lcnt[2 * NSUB + s] = 0u;
+    lbase[2 * NSUB + s] = 0u;
+  }
 }
 
 template <typename T>
