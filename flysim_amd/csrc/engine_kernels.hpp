@@ -409,11 +409,88 @@ __device__ __forceinline__ void push_spikes(const Net<T> &net, Range range, int3
       }
     }
   };
+  // Replicas in whole waves: spikes recorded by one wave are copies of one
+  // local neuron, contiguous in the record.  A wave takes 64 record entries
+  // at a time, and the lanes holding copies of the same neuron walk its row
+  // together: each atomic instruction then covers up to 64 consecutive
+  // copies of one target (a few cache lines) instead of one copy of up to 64
+  // targets (a line each, shared with the other copies' waves).  Groups of
+  // fewer than 8 copies keep the one-wave-per-spike walk.
+  auto grouped = [&](int s, uint32_t first, uint32_t stride) {
+    uint32_t a = 0, b = 0;
+    range(s, a, b);
+    const uint32_t off = net.sub_off[s];
+    for (uint32_t c = a + 64u * first; c < b; c += 64u * stride) {
+      const uint32_t i = c + (uint32_t)lane;
+      bool act = i < b;
+      uint32_t jl = 0, rep = 0;
+      T amp = T(0);
+      if (act) {
+        const uint32_t j = net.rec_ids[off + i];
+        jl = j / (uint32_t)B;
+        rep = j - jl * (uint32_t)B;
+        amp = net.rec_a[off + i];
+      }
+      uint64_t left = __ballot(act);
+      while (left) {
+        const int ld = __ffsll((unsigned long long)left) - 1;
+        const uint32_t J = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)jl, ld));
+        const bool mine = act && jl == J;
+        const uint64_t mm = __ballot(mine);
+        left &= ~mm;
+        act = act && !mine;
+        const uint32_t cnt = (uint32_t)__popcll(mm);
+        const uint64_t r0 = ldc(net.rp_p + J), r1 = ldc(net.rp_p + J + 1);
+        if (lane == 0) { events += (r1 - r0) * cnt; nout += (uint64_t)ldc(net.nmda_out + J) * cnt; }
+        if (cnt >= 8) {
+          const T sc[4] = {amp * T(net.gainr[0 * B + rep]), amp * T(net.gainr[1 * B + rep]),
+                           amp * T(net.gainr[3 * B + rep]), amp * T(net.gainr[4 * B + rep])};
+          for (uint64_t e = r0; e < r1; ++e) {
+            const uint32_t m = ldc(net.meta_p + e);
+            const float w = ldc(net.w_p + e);
+            const uint32_t li = m >> 30;
+            int32_t slot = s0 + (int32_t)((m >> 24) & 63u);
+            slot -= slot >= net.D ? net.D : 0;
+            if (mine)
+              atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N +
+                               (size_t)(m & 0x00FFFFFFu) * B + rep,
+                           T(w) * sc[li]);
+          }
+        } else {
+          uint64_t g = mm;
+          while (g) {
+            const int q = __ffsll((unsigned long long)g) - 1;
+            g &= g - 1;
+            const uint32_t rq = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)rep, q));
+            const T aq = __shfl(amp, q);
+            const T sq[4] = {aq * T(net.gainr[0 * B + rq]), aq * T(net.gainr[1 * B + rq]),
+                             aq * T(net.gainr[3 * B + rq]), aq * T(net.gainr[4 * B + rq])};
+            for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
+              const uint32_t m = ldc(net.meta_p + e);
+              const uint32_t li = m >> 30;
+              int32_t slot = s0 + (int32_t)((m >> 24) & 63u);
+              slot -= slot >= net.D ? net.D : 0;
+              atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N +
+                               (size_t)(m & 0x00FFFFFFu) * B + rq,
+                           T(ldc(net.w_p + e)) * sq[li]);
+            }
+          }
+        }
+      }
+    }
+  };
+  const bool wave_reps = (B & 63) == 0;
   if (nwaves >= NSUB) {
     const int per = nwaves / NSUB;
-    if (gwave < per * NSUB) one(gwave % NSUB, (uint32_t)(gwave / NSUB), (uint32_t)per);
+    if (gwave < per * NSUB) {
+      if (wave_reps) grouped(gwave % NSUB, (uint32_t)(gwave / NSUB), (uint32_t)per);
+      else one(gwave % NSUB, (uint32_t)(gwave / NSUB), (uint32_t)per);
+    }
   } else {
-    for (int s = gwave; s < NSUB; s += nwaves) one(s, 0u, 1u);
+    for (int s = gwave; s < NSUB; s += nwaves) {
+      if (wave_reps) grouped(s, 0u, 1u);
+      else one(s, 0u, 1u);
+    }
   }
 }
 

@@ -130,7 +130,8 @@ def screen(model: Dict[str, np.ndarray], points: int, seed: int, replicas: int,
     return {"points": np.int64(points), "seed": np.int64(seed), "flagged": flagged,
             "audited": audited, "idx": mine, "rates": rates.astype(np.float16),
             "pred": pred[mine].astype(np.float16),
-            "pred_mean_all": mean.astype(np.float32), "t_predict_s": np.float64(t_pred),
+            "pred_mean_all": (mean if part == 0 else mean[:0]).astype(np.float32),
+            "t_predict_s": np.float64(t_pred),
             "predict_gpu_ms": np.float64(sur.mlp.last_ms), "t_sim_s": np.float64(t_sim),
             "sim_points_per_s": np.float64(mine.size / t_sim if t_sim > 0 else 0.0)}
 
@@ -228,7 +229,11 @@ def main(argv=None):
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "gen":
-        np.savez(a.out, **gen(a.points, a.seed, a.replicas, log=log))
+        d = gen(a.points, a.seed, a.replicas, log=log)
+        np.savez(a.out, **d)
+        meta = {k: float(v) for k, v in d.items() if np.ndim(v) == 0}
+        meta["points"] = int(d["X"].shape[0])
+        json.dump(meta, open(a.out[:-4] + ".json" if a.out.endswith(".npz") else a.out + ".json", "w"))
     elif a.cmd == "train":
         d = np.load(a.data)
         m = train(d["X"], d["Y"], epochs=a.epochs, batch=a.batch, lr=a.lr)
@@ -243,7 +248,11 @@ def main(argv=None):
         log("plain per-instance simulation: %.2f points/s" % rate)
         json.dump({"points": a.points, "points_per_s": rate}, open(a.out, "w"))
     else:
-        tr = np.load(a.train)
+        if a.train.endswith(".json"):
+            meta = json.load(open(a.train))
+            tr = {"X": np.zeros((int(meta["points"]), 0)), "wall_s": meta["wall_s"]}
+        else:
+            tr = np.load(a.train)
         m = dict(np.load(a.model))
         files = sorted(set(f for p in a.screen for f in glob.glob(p)))
         scr = [dict(np.load(f)) for f in files]

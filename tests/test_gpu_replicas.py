@@ -75,3 +75,37 @@ def test_replica_knobs_validate():
         eb.set_replica(3)
     with pytest.raises(A.FlysimError):
         eb.set_ext_rate(3 * len(net.pops), W.AMPA, 100.0)
+
+
+@pytest.mark.parametrize("kind", ["delayed", "zero_delay"])
+def test_whole_wave_replicas_equal_single_runs(kind):
+    """64 copies (one local neuron per wave: scalar-load update and gather,
+    grouped push) equal the same network run alone, copy by copy (FP64)."""
+    net = _nets()[kind]
+    arr = arrays_from_network(net)
+    B, n0 = 64, net.n
+    knobs = [((0.8 + 0.01 * r, 1.3 - 0.01 * r, 0.7 + 0.02 * r, 1.0, 1.0), 0.9 + 0.004 * r, 7 * r + 1)
+             for r in range(B)]
+    print("case whole-wave replicas %s: %d copies of %d neurons, %d steps" % (kind, B, n0, STEPS),
+          flush=True)
+    eb = Engine(arr, precision=A.FS_FP64, replicas=B)
+    eb.set_schedule(gather_lanes=1)
+    for r, (g, eg, key) in enumerate(knobs):
+        eb.set_replica(r, gain=g, ext_gain=eg, stream_key=key)
+    eb.reset(5)
+    rb = eb.raster(STEPS)
+    total = 0
+    for r in (0, 21, 42, 63):
+        g, eg, key = knobs[r]
+        e = Engine(arr, precision=A.FS_FP64)
+        e.set_schedule(gather_lanes=1)
+        e.set_replica(0, gain=g, ext_gain=eg, stream_key=key)
+        e.reset(5)
+        ref = e.raster(STEPS)
+        e.close()
+        mine = rb[rb[:, 1] % B == r]
+        mine = np.stack([mine[:, 0], mine[:, 1] // B], axis=1)
+        assert len(ref) > 20
+        np.testing.assert_array_equal(mine, ref)
+        total += len(ref)
+    assert total > 200
