@@ -596,6 +596,13 @@ struct Engine final : EngineBase {
     const int64_t threads = (int64_t)N * sched.gather_lanes;
     return (int)((threads + sched.update_block - 1) / sched.update_block);
   }
+  // fused step: push blocks (update_block threads each) ahead of the
+  // gather + update blocks, enough waves for the typical spike count
+  int step_push_blocks() const {
+    const int wpb = sched.update_block / 64;
+    const int pw = std::max(64, std::min(4096, N / 64));
+    return (pw + wpb - 1) / wpb;
+  }
   int deliver_grid(int *push_waves) const {
     const int wpb = sched.push_block / 64;
     // push waves: enough for the typical spike count of a step (spikes are
@@ -622,7 +629,8 @@ struct Engine final : EngineBase {
     const bool uni_row = uni && sched.gather_lanes == 1;
     const bool rp = net.ext_mode == FS_EXT_REPLAY;
     if (fused()) {
-      const dim3 g(step_grid()), b(sched.update_block);
+      net.push_blocks = step_push_blocks();
+      const dim3 g(net.push_blocks + step_grid()), b(sched.update_block);
       if (rp && uni_row) hipLaunchKernelGGL((fsk::k_step<T, true, true>), g, b, 0, stream, net, k);
       else if (rp) hipLaunchKernelGGL((fsk::k_step<T, true, false>), g, b, 0, stream, net, k);
       else if (uni_row) hipLaunchKernelGGL((fsk::k_step<T, false, true>), g, b, 0, stream, net, k);

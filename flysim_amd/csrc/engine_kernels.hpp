@@ -131,6 +131,7 @@ struct Net {
                                 // spikers, each spread over NCTR slots by block index
   int32_t gather_lanes;
   int32_t push_waves;       // waves of k_deliver dedicated to the push
+  int32_t push_blocks;      // fused step: leading blocks that push; the rest gather + update
 };
 
 // ------------------------------------------------------------------ RNG
@@ -165,7 +166,8 @@ __device__ __forceinline__ int32_t slot_of(int32_t t0mod, int32_t k, int32_t M) 
 // One neuron's step t (the reference's Activation) followed by the wave's
 // spike compaction; every thread of the block must reach it (ballots, block
 // barriers); list_ctr[NSUB] and list_base are the sub-list counters and the
-// offset of this step's list within each sub-list.  With
+// offset of this step's list within sub-list `sub` (the update block's index
+// modulo NSUB).  With
 // gn_given the NMDA conductance comes from the caller (fused step) instead of
 // the gn array written by the separate gather.
 // UNI: the caller guarantees that all 64 lanes of the wave hold the same
@@ -174,7 +176,7 @@ __device__ __forceinline__ int32_t slot_of(int32_t t0mod, int32_t k, int32_t M) 
 template <typename T, bool REPLAY, bool UNI>
 __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int64_t t, int32_t k,
                                               bool gn_given, T gn_in, uint32_t *list_ctr,
-                                              uint32_t list_base) {
+                                              uint32_t list_base, int sub) {
   const bool active = n < net.N;
   bool spk = false;
   T a_push = T(0);
@@ -333,7 +335,6 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
   if (mask && lane == leader) woff = atomicAdd(&s_cnt, cntw);
   woff = __shfl(woff, leader);
   __syncthreads();
-  const int sub = (int)(blockIdx.x % NSUB);
   if (threadIdx.x == 0 && s_cnt) {
     s_base = list_base + atomicAdd(list_ctr + sub, s_cnt);
     atomicAdd(&net.counters[blockIdx.x & (NCTR - 1)], (unsigned long long)s_cnt);
@@ -362,7 +363,7 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
 template <typename T, bool REPLAY, bool UNI>
 __global__ void __launch_bounds__(1024) k_update(Net<T> net, int32_t k) {
   update_neuron<T, REPLAY, UNI>(net, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x), *net.t0 + k, k,
-                           false, T(0), net.total, 0u);
+                                false, T(0), net.total, 0u, (int)(blockIdx.x % NSUB));
 }
 
 // ------------------------------------------------------------------ delivery
@@ -559,8 +560,10 @@ __device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int32_t h0
 // neuron for step t and appends its spike to the list of step k.  The per-step
 // lists rotate over three counters (k%3): launch k reads the final count of
 // list k-1, appends to list k (zeroed by launch k-1) and zeroes list k+1, so no
-// block waits for another within a launch.  The grid has N*L threads, so
-// small networks still present L times more waves to hide memory latency.
+// block waits for another within a launch.  The first push_blocks blocks do
+// (a), the others (b) and (c) with N*L threads in all, so small networks still
+// present L times more waves to hide memory latency and the push runs beside
+// the gather instead of ahead of it.
 template <typename T, bool REPLAY, bool UNI>
 __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
   const int64_t t = *net.t0 + k;
@@ -576,26 +579,31 @@ __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
   const int wave = (int)(threadIdx.x >> 6);
   const int lane = (int)(threadIdx.x & 63);
   const int wpb = (int)(blockDim.x >> 6);
-  unsigned long long my_events = 0, my_nout = 0;
-  // spikes of step t-1: [base, base + count) of each sub-list's list k-1
-  auto prev = [&](int s, uint32_t &a, uint32_t &b) {
-    const uint32_t cap = net.sub_cap[s], x = pbase[s], y = pbase[s] + pcnt[s];
-    a = x < cap ? x : cap;
-    b = y < cap ? y : cap;
-  };
-  push_spikes(net, prev, slot_of(net.tmod[0], k - 1, net.D), (int)blockIdx.x * wpb + wave,
-              (int)gridDim.x * wpb, lane, my_events, my_nout);
+  if ((int)blockIdx.x < net.push_blocks) {
+    // leading blocks: push the spikes of step t-1, in parallel with the
+    // gather + update blocks (dispatched first, so they never trail them)
+    unsigned long long my_events = 0, my_nout = 0;
+    auto prev = [&](int s, uint32_t &a, uint32_t &b) {
+      const uint32_t cap = net.sub_cap[s], x = pbase[s], y = pbase[s] + pcnt[s];
+      a = x < cap ? x : cap;
+      b = y < cap ? y : cap;
+    };
+    push_spikes(net, prev, slot_of(net.tmod[0], k - 1, net.D), (int)blockIdx.x * wpb + wave,
+                net.push_blocks * wpb, lane, my_events, my_nout);
+    block_add_events(net, my_events, my_nout);
+    return;
+  }
+  const int ub = (int)blockIdx.x - net.push_blocks;  // gather + update block index
   const int L = net.gather_lanes;
-  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gt = (int64_t)ub * blockDim.x + threadIdx.x;
   const int64_t nn = gt / L;
   const int sl = (int)(gt % L);
   T gn = T(0);
   if (net.has_nmda_syn) gn = gather_row<T, UNI>(net, nn, slot_of(net.tmod[1], k - 1, net.DN), L, sl);
   const int32_t n = (sl == 0 && nn < net.N) ? (int32_t)nn : net.N;  // N: inactive lane
-  const int sub = (int)(blockIdx.x % NSUB);
+  const int sub = ub % NSUB;
   update_neuron<T, REPLAY, UNI>(net, n, t, k, true, gn, net.lcnt + pc * NSUB,
-                                pbase[sub] + pcnt[sub]);
-  block_add_events(net, my_events, my_nout);
+                                pbase[sub] + pcnt[sub], sub);
 }
 
 // Push of the last step of a fused batch of K launches; publishes the end
