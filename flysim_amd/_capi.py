@@ -18,6 +18,7 @@ FS_OK = 0
 FS_AMPA, FS_GABA, FS_NMDA, FS_ACH, FS_GCL = 0, 1, 2, 3, 4
 FS_ACT_AMPA, FS_ACT_GABA, FS_ACT_NMDA, FS_ACT_GAP = 1, 2, 4, 8
 FS_ACT_ACH, FS_ACT_GCL, FS_ACT_INJ, FS_ACT_LIF = 32, 64, 128, 256
+FS_ACT_LTP, FS_ACT_STD, FS_ACT_STP = 8192, 16384, 32768
 FS_FP32, FS_FP64 = 0, 1
 FS_EXT_PHILOX, FS_EXT_REPLAY = 0, 1
 DEFAULT_ACT = (FS_ACT_AMPA | FS_ACT_GABA | FS_ACT_NMDA | FS_ACT_ACH | FS_ACT_GCL
@@ -31,7 +32,11 @@ class PopParams(C.Structure):
                 ("vrev", C.c_float * 5), ("dfst", C.c_float * 5), ("dfxt_n", C.c_float),
                 ("efst_n", C.c_float), ("as_n", C.c_float), ("mg_n", C.c_float),
                 ("pb", C.c_float * 5), ("gext", C.c_float * 5), ("act", C.c_uint32),
-                ("inj_mean", C.c_float), ("inj_std", C.c_float), ("taum", C.c_float)]
+                ("inj_mean", C.c_float), ("inj_std", C.c_float), ("taum", C.c_float),
+                ("stp_pv", C.c_float), ("stp_dfD", C.c_float), ("stp_efD", C.c_float),
+                ("stp_aF", C.c_float), ("stp_dfF", C.c_float), ("stp_efF", C.c_float),
+                ("stp_aCap", C.c_float), ("stp_dfCap", C.c_float), ("stp_aCar", C.c_float),
+                ("stp_dfCar", C.c_float), ("stp_efCar", C.c_float)]
 
 
 class PopConf(C.Structure):
@@ -40,7 +45,11 @@ class PopConf(C.Structure):
                 ("spike_delay_ms", C.c_float), ("declared", C.c_int32 * 5),
                 ("tau", C.c_float * 5), ("tau_xt", C.c_float * 5), ("vrev", C.c_float * 5),
                 ("freq_ext", C.c_float * 5), ("ext_eff", C.c_float * 5),
-                ("ext_con", C.c_float * 5)]
+                ("ext_con", C.c_float * 5), ("stp", C.c_float * 8)]
+
+# STP_pv, STP_tD, STP_aF, STP_tF, STP_aCap, STP_tCap, STP_aCar, STP_tCar defaults
+# of the reference parser (code/parser.cpp:229-243)
+STP_DEFAULTS = (0.6, 300.0, 0.0035, 7000.0, 250.0, 2.0, 0.1, 2000.0)
 
 
 class NetDesc(C.Structure):
@@ -56,7 +65,7 @@ class NetDesc(C.Structure):
 class Schedule(C.Structure):
     _fields_ = [("update_block", C.c_int32), ("gather_lanes", C.c_int32),
                 ("push_block", C.c_int32), ("steps_per_graph", C.c_int32),
-                ("use_graph", C.c_int32), ("fused", C.c_int32)]
+                ("use_graph", C.c_int32), ("fused", C.c_int32), ("window", C.c_int32)]
 
 
 class EiBatchDesc(C.Structure):

@@ -64,6 +64,7 @@ struct EngineBase {
   virtual void times(double *u, double *p, double *g, int64_t *n) = 0;
   virtual void counters(int64_t *out) = 0;
   virtual bool fused_active() const = 0;
+  virtual int32_t window_active() const = 0;
   fs_schedule sched{};
   bool timing = false;
   int64_t step = 0;
@@ -86,12 +87,16 @@ struct Engine final : EngineBase {
   std::vector<uint64_t> hskey;  // [B]
   // device buffers
   DevBuf b_npop, b_pops, b_V, b_cs, b_G, b_ring, b_spre, b_xn, b_shist, b_gn, b_elin, b_enx,
-      b_ens, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
-      b_total, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase, b_suboff, b_subcap,
+      b_ens, b_stp, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
+      b_total, b_rec_k, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase, b_suboff, b_subcap,
       b_pbr, b_gain, b_skey, b_tmod;
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
-  bool fused() const { return fused_ok && sched.fused; }
+  bool fused() const { return fused_ok && sched.fused && !window(); }
+  // windowed path: steps per launch, at most 1 + the smallest synaptic delay
+  int32_t win_max = 1;
+  int32_t window() const { return sched.window > 0 ? std::min(sched.window, win_max) : 0; }
   bool fused_active() const override { return fused(); }
+  int32_t window_active() const override { return window(); }
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -103,7 +108,7 @@ struct Engine final : EngineBase {
   int64_t n_graph_steps = 0;
   int64_t n_timed = 0;
   int64_t syn_lin = 0, syn_nmda = 0;
-  std::vector<uint32_t> cum_h, hsub_off, hsub_cap, stage;
+  std::vector<uint32_t> cum_h, hsub_off, hsub_cap, stage, stage_k;
   std::vector<uint32_t> ids_h;
   unsigned long long ctr_last[4] = {0, 0, 0, 0};
 
@@ -136,6 +141,17 @@ struct Engine final : EngineBase {
     else d.efas_n = T(p.efst_n) * T(p.as_n);
     d.inj_mean = T(p.inj_mean);
     d.inj_std = T(p.inj_std);
+    d.stp_pv = T(p.stp_pv);
+    d.stp_dfD = T(p.stp_dfD);
+    d.stp_efD = T(p.stp_efD);
+    d.stp_aF = T(p.stp_aF);
+    d.stp_dfF = T(p.stp_dfF);
+    d.stp_efF = T(p.stp_efF);
+    d.stp_aCap = T(p.stp_aCap);
+    d.stp_dfCap = T(p.stp_dfCap);
+    d.stp_aCar = T(p.stp_aCar);
+    d.stp_dfCar = T(p.stp_dfCar);
+    d.stp_efCar = T(p.stp_efCar);
     d.rfped = p.rfped;
     d.spidy = p.spidy;
     d.act = p.act;
@@ -270,6 +286,7 @@ struct Engine final : EngineBase {
       if (d.receptor[s] == FS_NMDA) nout[d.pre[s]]++;
     PreGates pg{pre_set, pre_set_n, pre_df, pre_dfxt, pre_dfst, pre_ef, pre_as};
     init_pops(pg);
+    win_max = d.n_syn > 0 ? std::min(mind_all + 1, 64) : 64;
     init_dims(maxd_p, maxd_n, d.n_syn > 0 && mind_all >= 1, lin_mask, d.ext_mode);
     const size_t n0 = (size_t)N0;
     b_rp_p.alloc((n0 + 1) * 8);
@@ -442,17 +459,32 @@ struct Engine final : EngineBase {
 
   void init_dims(int32_t maxd_p, int32_t maxd_n, bool fusable, int32_t lin_mask, int32_t ext_mode) {
     net.N = N;
-    net.D = maxd_p + 1;
-    // the fused step reads gate history while writing the current slot
-    net.DN = maxd_n + 2;
+    // the windowed path writes up to win_max steps ahead of the slots other
+    // blocks still read (the fused step needs DN >= max NMDA delay + 2)
+    net.D = maxd_p + 1 + win_max;
+    net.DN = maxd_n + 1 + win_max;
     fused_ok = fusable;
     net.lin_mask = lin_mask;
     net.has_nmda_syn = syn_nmda > 0;
     net.ext_mode = ext_mode;
+    net.stp_mode = stp_mode_of();
     net.words = (N + 31) / 32;
     net.B = B;
     net.N0 = N0;
     net.P0 = P;
+  }
+
+  // The plasticity switch is global in the reference (-STP / -STD set the
+  // solver's ActCtr, code/main.cpp:229-230): every population carries it.
+  int32_t stp_mode_of() const {
+    const uint32_t m = FS_ACT_STP | FS_ACT_STD;
+    const uint32_t a0 = P ? hpop[0].act & m : 0u;
+    for (int32_t i = 1; i < P; ++i)
+      if ((hpop[i].act & m) != a0)
+        throw std::invalid_argument("populations disagree on the STP / STD switch");
+    if (a0 & FS_ACT_STP) return fsk::STP_FULL;  // STP wins over STD (code/EqsGnl2.cpp:1109-1128)
+    if (a0 & FS_ACT_STD) return fsk::STP_STD;
+    return 0;
   }
 
   // state, tables and launch set-up once the synapse tables are on the device
@@ -487,6 +519,7 @@ struct Engine final : EngineBase {
     b_elin.alloc((size_t)nlin * n * sizeof(T));
     b_enx.alloc(n * sizeof(T));
     b_ens.alloc(n * sizeof(T));
+    if (net.stp_mode) b_stp.alloc(4 * n * sizeof(T));
     b_total.alloc(fsk::NSUB * 4);
     b_overflow.alloc(4);
     b_lcnt.alloc(3 * fsk::NSUB * 4);
@@ -511,6 +544,12 @@ struct Engine final : EngineBase {
     net.elin = b_elin.as<T>();
     net.enx = b_enx.as<T>();
     net.ens = b_ens.as<T>();
+    if (net.stp_mode) {
+      net.sD = b_stp.as<T>();
+      net.sF = net.sD + n;
+      net.sCap = net.sF + n;
+      net.sCar = net.sCap + n;
+    }
     net.rp_p = b_rp_p.as<uint64_t>();
     net.meta_p = b_meta_p.as<uint32_t>();
     net.w_p = b_w_p.as<float>();
@@ -539,6 +578,7 @@ struct Engine final : EngineBase {
     sched.steps_per_graph = 100;
     sched.use_graph = 1;
     sched.fused = 1;
+    sched.window = 0;
     apply_schedule();
     reset(0);
   }
@@ -557,7 +597,7 @@ struct Engine final : EngineBase {
     // record buffer: every neuron could hold SPIKE_F every step (sticky
     // case).  Block b of the update grid appends to sub-list b % NSUB, whose
     // region holds its blocks' neurons for K steps.
-    const int64_t npb = fused() ? ub / L : ub;  // neurons per block
+    const int64_t npb = (fused() || window()) ? ub / L : ub;  // neurons per block
     std::vector<uint64_t> cov(fsk::NSUB, 0);
     const int64_t nblk = ((int64_t)N + npb - 1) / npb;
     for (int64_t b = 0; b < nblk; ++b) cov[b % fsk::NSUB] += std::min<int64_t>(npb, N - b * npb);
@@ -573,9 +613,10 @@ struct Engine final : EngineBase {
       hsub_cap[s] = (uint32_t)c;
       acc += c;
     }
-    if (b_rec_ids.bytes < acc * 4 || b_rec_a.bytes < acc * sizeof(T)) {
+    if (b_rec_ids.bytes < acc * 4 || b_rec_a.bytes < acc * sizeof(T) || b_rec_k.bytes < acc * 4) {
       b_rec_ids.alloc(acc * 4);
       b_rec_a.alloc(acc * sizeof(T));
+      b_rec_k.alloc(acc * 4);
     }
     HIPCHECK(hipMemcpy(b_suboff.p, hsub_off.data(), fsk::NSUB * 4, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(b_subcap.p, hsub_cap.data(), fsk::NSUB * 4, hipMemcpyHostToDevice));
@@ -584,6 +625,7 @@ struct Engine final : EngineBase {
     net.rec_cap = acc;
     net.rec_ids = b_rec_ids.as<uint32_t>();
     net.rec_a = b_rec_a.as<T>();
+    net.rec_k = b_rec_k.as<uint32_t>();
     const size_t cum_bytes = (size_t)(K + 1) * fsk::NSUB * 4;
     if (b_cum.bytes < cum_bytes) b_cum.alloc(cum_bytes);
     net.cum = b_cum.as<uint32_t>();
@@ -659,6 +701,21 @@ struct Engine final : EngineBase {
     if (timed) HIPCHECK(hipEventRecord(ev[2], stream));
   }
 
+  // windowed path: steps k0 .. k0+w-1 in one launch
+  void launch_window(int32_t k0, int32_t w, bool timed) {
+    if (timed) HIPCHECK(hipEventRecord(ev[0], stream));
+    const dim3 g(step_grid()), b(sched.update_block);
+    if (net.ext_mode == FS_EXT_REPLAY)
+      hipLaunchKernelGGL((fsk::k_window<T, true>), g, b, 0, stream, net, k0, w);
+    else
+      hipLaunchKernelGGL((fsk::k_window<T, false>), g, b, 0, stream, net, k0, w);
+    HIPCHECK(hipGetLastError());
+    if (timed) {
+      HIPCHECK(hipEventRecord(ev[1], stream));
+      HIPCHECK(hipEventRecord(ev[2], stream));
+    }
+  }
+
   // fused path: the spikes of the last step of a batch of K launches are
   // pushed by this launch
   void launch_tail(int32_t K) {
@@ -674,8 +731,12 @@ struct Engine final : EngineBase {
     if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
     if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
     HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (int32_t k = 1; k < K; ++k) launch_step(k, false);
-    if (fused()) launch_tail(K);
+    if (const int32_t W = window()) {
+      for (int32_t k0 = W; k0 < K; k0 += W) launch_window(k0, std::min(W, K - k0), false);
+    } else {
+      for (int32_t k = 1; k < K; ++k) launch_step(k, false);
+      if (fused()) launch_tail(K);
+    }
     HIPCHECK(hipStreamEndCapture(stream, &graph));
     HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
     graph_k = K;
@@ -700,6 +761,12 @@ struct Engine final : EngineBase {
     fill(b_elin.p, 4 * n, T(0));
     fill(b_enx.p, n, T(0));
     fill(b_ens.p, n, T(0));
+    if (net.stp_mode) {  // VarStp (code/EqsGnl2.cpp:353-362): D = F = 1, Cap = 0, Car = 20 uM
+      fill(net.sD, n, T(1));
+      fill(net.sF, n, T(1));
+      fill(net.sCap, n, T(0));
+      fill(net.sCar, n, T(20.0f));
+    }
     HIPCHECK(hipMemsetAsync(b_popcnt.p, 0, (size_t)P * B * 8, stream));
     HIPCHECK(hipMemsetAsync(b_counters.p, 0, 3 * fsk::NCTR * 8, stream));
     HIPCHECK(hipMemsetAsync(b_overflow.p, 0, 4, stream));
@@ -793,13 +860,18 @@ struct Engine final : EngineBase {
                          b_total.as<uint32_t>(), b_cum.as<uint32_t>(), b_lcnt.as<uint32_t>(),
                          b_lbase.as<uint32_t>(), b_tmod.as<int32_t>(), net.D, net.DN);
       HIPCHECK(hipGetLastError());
-      launch_step(0, timing);
-      if (K > 1) {
+      const int32_t W = window();
+      const int32_t w0 = W ? std::min(W, K) : 1;  // steps of the eager first launch
+      if (W) launch_window(0, w0, timing);
+      else launch_step(0, timing);
+      if (K > w0) {
         if (sched.use_graph) {
           if (graph_dirty || graph_k != K) build_graph(K);
           if (timing) HIPCHECK(hipEventRecord(ev[3], stream));
           HIPCHECK(hipGraphLaunch(gexec, stream));
           if (timing) HIPCHECK(hipEventRecord(ev[4], stream));
+        } else if (W) {
+          for (int32_t k0 = W; k0 < K; k0 += W) launch_window(k0, std::min(W, K - k0), false);
         } else {
           for (int32_t k = 1; k < K; ++k) launch_step(k, false);
           if (fused()) launch_tail(K);
@@ -815,17 +887,52 @@ struct Engine final : EngineBase {
         t_upd += a;
         t_del += b;
         n_timed++;
-        if (K > 1 && sched.use_graph) {
+        if (K > w0 && sched.use_graph) {
           float c = 0;
           HIPCHECK(hipEventElapsedTime(&c, ev[3], ev[4]));
           t_graph += c;
-          n_graph_steps += K - 1;
+          n_graph_steps += K - w0;
         }
       }
       uint32_t ovf = 0;
       HIPCHECK(hipMemcpy(&ovf, b_overflow.p, 4, hipMemcpyDeviceToHost));
       if (ovf) throw std::length_error("device spike record overflow; lower steps_per_graph");
-      if (off || ids) {
+      if ((off || ids) && W) {
+        // windowed path: sub-list s holds total[s] entries (id, step) in
+        // arrival order; bucket them by step, ascending ids within a step
+        const int S = fsk::NSUB;
+        std::vector<uint32_t> tot_s(S);
+        HIPCHECK(hipMemcpy(tot_s.data(), b_total.p, S * 4, hipMemcpyDeviceToHost));
+        uint64_t tot = 0;
+        for (int s = 0; s < S; ++s) tot += std::min(tot_s[s], hsub_cap[s]);
+        stage.resize(tot);
+        stage_k.resize(tot);
+        uint64_t at = 0;
+        for (int s = 0; s < S; ++s) {
+          const uint32_t ns = std::min(tot_s[s], hsub_cap[s]);
+          if (!ns) continue;
+          HIPCHECK(hipMemcpy(stage.data() + at, b_rec_ids.as<uint32_t>() + hsub_off[s], (size_t)ns * 4,
+                             hipMemcpyDeviceToHost));
+          HIPCHECK(hipMemcpy(stage_k.data() + at, b_rec_k.as<uint32_t>() + hsub_off[s], (size_t)ns * 4,
+                             hipMemcpyDeviceToHost));
+          at += ns;
+        }
+        std::vector<int64_t> kst(K + 1, 0);
+        for (uint64_t i = 0; i < tot; ++i) {
+          if (stage_k[i] >= (uint32_t)K) throw std::runtime_error("corrupt spike record");
+          kst[stage_k[i] + 1]++;
+        }
+        for (int32_t k = 0; k < K; ++k) kst[k + 1] += kst[k];
+        if (ids) {
+          if (written + (int64_t)tot > cap) throw std::length_error("spike id buffer too small");
+          std::vector<int64_t> fill(kst.begin(), kst.end() - 1);
+          for (uint64_t i = 0; i < tot; ++i) ids[written + fill[stage_k[i]]++] = stage[i];
+          for (int32_t k = 0; k < K; ++k) std::sort(ids + written + kst[k], ids + written + kst[k + 1]);
+        }
+        if (off)
+          for (int32_t k = 0; k < K; ++k) off[done + k + 1] = written + kst[k + 1];
+        written += (int64_t)tot;
+      } else if (off || ids) {
         // cum[k][s]: offset of step k's spikes within sub-list s
         const int S = fsk::NSUB;
         cum_h.resize((size_t)(K + 1) * S);
@@ -914,7 +1021,7 @@ struct Engine final : EngineBase {
         break;
       }
       case 5:
-        if (fused() && net.has_nmda_syn) {  // the fused step keeps gN in registers
+        if ((fused() || window()) && net.has_nmda_syn) {  // these paths keep gN in registers
           hipLaunchKernelGGL((fsk::k_gather_only<T>), dim3((N + 3) / 4), dim3(256), 0, stream, net,
                              (int32_t)(((step - 1) % net.DN + net.DN) % net.DN));
           HIPCHECK(hipGetLastError());
@@ -932,6 +1039,10 @@ struct Engine final : EngineBase {
       case 11: case 12: case 13: case 14: d2h(b_elin.as<T>() + (size_t)(f - 11) * n); break;
       case 15: d2h(b_enx.p); break;
       case 16: d2h(b_ens.p); break;
+      case 17: case 18:
+        if (!net.stp_mode) throw std::invalid_argument("no STP / STD state: the switch is off");
+        d2h(f == 17 ? net.sD : net.sF);
+        break;
       default: throw std::invalid_argument("unknown state field");
     }
   }
@@ -1100,6 +1211,7 @@ int fs_get_schedule(fs_handle *h, fs_schedule *s) {
   if (!h || !s) return fs::set_error(FS_ERR_ARG, "null argument");
   *s = h->e->sched;
   s->fused = h->e->fused_active() ? 1 : 0;
+  s->window = h->e->window_active();
   return FS_OK;
 }
 

@@ -45,6 +45,7 @@ constexpr uint32_t CNT_MASK = 0x00FFFFFFu;
 constexpr uint32_t F_FIRE = 1u << 24, F_SPIKE = 1u << 25, F_OVF = 1u << 28, F_UDF = 1u << 29;
 constexpr uint32_t ACT_AMPA = 1, ACT_GABA = 2, ACT_NMDA = 4, ACT_ACH = 32, ACT_GCL = 64,
                    ACT_INJ = 128;
+constexpr int32_t STP_STD = 1, STP_FULL = 2;
 // linear receptor slots li = 0..3 map to receptor ids A=0, G=1, H=3, L=4
 __host__ __device__ constexpr int lin_rid(int li) { return li < 2 ? li : li + 1; }
 __host__ __device__ constexpr uint32_t lin_act(int li) {
@@ -62,6 +63,9 @@ struct DevPop {
   T pre_df;             // presynaptic linear gate decay (targets' dfst)
   T pre_dfxt, pre_dfst, pre_efas;  // presynaptic NMDA gate (targets' params)
   T inj_mean, inj_std;
+  // short-term plasticity of the population's outgoing synapses
+  T stp_pv, stp_dfD, stp_efD, stp_aF, stp_dfF, stp_efF, stp_aCap, stp_dfCap, stp_aCar,
+      stp_dfCar, stp_efCar;
   int32_t rfped, spidy;
   uint32_t act;             // only ACT_INJ is read on the device
   uint32_t has_lin_out, has_nmda_out;
@@ -80,6 +84,7 @@ struct Net {
   int32_t ext_mask;     // bit rid set: some population has external drive on rid
   int32_t has_nmda_syn;
   int32_t ext_mode;     // 0 philox, 1 replay
+  int32_t stp_mode;     // 0 none, STP_STD depression only, STP_FULL depression + facilitation
   int32_t words;        // ceil(N/32)
   int32_t B;            // replicas (interleaved: n = local * B + replica)
   int32_t N0;           // neurons of one replica (N = N0 * B)
@@ -100,6 +105,7 @@ struct Net {
   T *gn;                // [N] gathered NMDA conductance of the last step
   T *elin;              // [4][N] external linear gates
   T *enx, *ens;         // [N] external NMDA gate
+  T *sD, *sF, *sCap, *sCar;  // [N] presynaptic short-term plasticity state (stp_mode != 0)
   // push CSR (linear)
   const uint64_t *rp_p;
   const uint32_t *meta_p;   // col | delay<<24 | li<<30
@@ -116,6 +122,7 @@ struct Net {
   // spike record of the current batch
   uint32_t *rec_ids;
   T *rec_a;
+  uint32_t *rec_k;          // windowed path: step of each record entry within the batch
   uint32_t *total;          // [NSUB] spikes recorded in this batch, per sub-list
   uint32_t *cum;            // [K+1][NSUB] per-step offsets within each sub-list
   uint32_t *overflow;       // [1]
@@ -173,15 +180,16 @@ __device__ __forceinline__ int32_t slot_of(int32_t t0mod, int32_t k, int32_t M) 
 // UNI: the caller guarantees that all 64 lanes of the wave hold the same
 // local neuron (replicas in whole waves); its index, population and
 // parameters are then wave-uniform and come through scalar loads.
+// neuron_core: steps (1)-(4b) for neuron n (n >= N: inactive lane, outputs
+// spk = false); update_neuron adds the compaction (5).
 template <typename T, bool REPLAY, bool UNI>
-__device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int64_t t, int32_t k,
-                                              bool gn_given, T gn_in, uint32_t *list_ctr,
-                                              uint32_t list_base, int sub) {
-  const bool active = n < net.N;
-  bool spk = false;
-  T a_push = T(0);
-  uint32_t pid = 0;  // population x replica index of the spike counters
-  if (active) {
+__device__ __forceinline__ void neuron_core(const Net<T> &net, int32_t n, int64_t t, int32_t k,
+                                            bool gn_given, T gn_in, bool &spk, T &a_push,
+                                            uint32_t &pid) {
+  spk = false;
+  a_push = T(0);
+  pid = 0;  // population x replica index of the spike counters
+  if (n < net.N) {
     const int32_t N = net.N;
     int32_t nl = n / net.B;
     if constexpr (UNI) nl = __builtin_amdgcn_readfirstlane(nl);
@@ -303,24 +311,60 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
       if (v > P.vth) { cs |= F_FIRE; firing(); }
     }
     spk = (cs & F_SPIKE) != 0;
-    // (4) presynaptic gates (the reference's per-synapse gate rule, shared)
+    // (4) presynaptic gates (the reference's per-synapse gate rule, shared);
+    // a spike adds 1, or D*F under STP and D under STD, the presynaptic
+    // state before this step's plasticity update (code/EqsGnl2.cpp:505-863)
+    T inc = T(1);
+    if (net.stp_mode) {
+      inc = net.sD[n];
+      if (net.stp_mode == STP_FULL) inc = inc * net.sF[n];
+    }
     if (P.has_lin_out) {
       const T s = net.spre[n];
-      a_push = T(1) + (T(1) - P.pre_df) * s;
-      net.spre[n] = spk ? s + T(1) : s * P.pre_df;
+      a_push = inc + (T(1) - P.pre_df) * s;
+      net.spre[n] = spk ? s + inc : s * P.pre_df;
     }
     if (P.has_nmda_out) {
       T x = net.xn[n];
       const int32_t sp = slot_of(net.tmod[1], k - 1, net.DN);
       const int32_t sc = slot_of(net.tmod[1], k, net.DN);
       const T s0 = net.shist[(size_t)sp * N + n];
-      x = spk ? x + T(1) : x * P.pre_dfxt;
+      x = spk ? x + inc : x * P.pre_dfxt;
       net.xn[n] = x;
       net.shist[(size_t)sc * N + n] = P.pre_dfst * s0 + P.pre_efas * x * (T(1) - s0);
+    }
+    // (4b) EqsGnl2::STP / STD (code/EqsGnl2.cpp:1048-1090), after the gates
+    if (net.stp_mode == STP_FULL) {
+      T d = net.sD[n], f = net.sF[n], cap = net.sCap[n], car = net.sCar[n];
+      if (spk) {
+        car = car + P.stp_aCar;
+        cap = cap + P.stp_aCap;
+        f = f + (cap + car) * (T(1) - f) * P.stp_aF;
+        d = d * (T(1) - P.stp_pv * f);
+      } else {
+        car = P.stp_dfCar * car + P.stp_efCar;
+        cap = P.stp_dfCap * cap;
+        f = P.stp_dfF * f + P.stp_efF * (cap + car) * (T(1) - f) * P.stp_aF;
+        d = P.stp_dfD * d + P.stp_efD;
+      }
+      net.sD[n] = d; net.sF[n] = f; net.sCap[n] = cap; net.sCar[n] = car;
+    } else if (net.stp_mode == STP_STD) {
+      const T d = net.sD[n];
+      net.sD[n] = spk ? d * (T(1) - P.stp_pv) : P.stp_dfD * d + P.stp_efD;
     }
     net.V[n] = v;
     net.cs[n] = (cs & ~CNT_MASK) | (cnt & CNT_MASK);
   }
+}
+
+template <typename T, bool REPLAY, bool UNI>
+__device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int64_t t, int32_t k,
+                                              bool gn_given, T gn_in, uint32_t *list_ctr,
+                                              uint32_t list_base, int sub) {
+  bool spk;
+  T a_push;
+  uint32_t pid;
+  neuron_core<T, REPLAY, UNI>(net, n, t, k, gn_given, gn_in, spk, a_push, pid);
   // (5) spike-list compaction: wave ballot, then one block-level LDS count,
   // so one global atomic per block claims the block's range of the list
   // (a global atomic per wave serialises on one address at high rates)
@@ -604,6 +648,103 @@ __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
   const int sub = ub % NSUB;
   update_neuron<T, REPLAY, UNI>(net, n, t, k, true, gn, net.lcnt + pc * NSUB,
                                 pbase[sub] + pcnt[sub], sub);
+}
+
+// Temporal blocking over the smallest synaptic delay: one launch runs the W
+// consecutive steps k0 .. k0+W-1 of the batch, W <= 1 + the smallest delay of
+// any synapse.  A spike of step u lands in its targets' ring slot u + d, read
+// by their update of step u + d + 1 >= k0 + W, and its NMDA gate of step u
+// enters gathers of steps >= u + d + 1 as well: nothing a block reads during
+// the launch is written by another block during it, so each block runs its
+// W steps with block barriers only and pushes its own spikes right after each
+// step.  The ring (D >= W + max delay + 1 slots) and the gate history
+// (DN >= W + max NMDA delay + 1) are deep enough that no write of the launch
+// lands in a slot some block has still to read.  Record entries carry their
+// step (rec_k); sub-list s of block b = b % NSUB grows by one atomic on
+// total[s] per block and step with spikes.
+template <typename T, bool REPLAY>
+__global__ void __launch_bounds__(1024) k_window(Net<T> net, int32_t k0, int32_t W) {
+  __shared__ uint32_t s_ids[1024];
+  __shared__ T s_amp[1024];
+  __shared__ uint32_t s_cnt, s_base;
+  const int L = net.gather_lanes;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nn = gt / L;
+  const int sl = (int)(gt % L);
+  const int32_t n = (sl == 0 && nn < net.N) ? (int32_t)nn : net.N;  // N: inactive lane
+  const int sub = (int)(blockIdx.x % NSUB);
+  const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int wpb = (int)(blockDim.x >> 6);
+  const int32_t N = net.N, B = net.B;
+  const uint32_t off = net.sub_off[sub], cap = net.sub_cap[sub];
+  unsigned long long events = 0, nout = 0;
+  for (int32_t s = 0; s < W; ++s) {
+    const int32_t k = k0 + s;
+    const int64_t t = *net.t0 + k;
+    T gn = T(0);
+    if (net.has_nmda_syn) gn = gather_row<T, false>(net, nn, slot_of(net.tmod[1], k - 1, net.DN), L, sl);
+    bool spk;
+    T a_push;
+    uint32_t pid;
+    neuron_core<T, REPLAY, false>(net, n, t, k, true, gn, spk, a_push, pid);
+    // the step's spikes of this block: wave ballots, one LDS count
+    if (threadIdx.x == 0) s_cnt = 0u;
+    __syncthreads();
+    const uint64_t mask = __ballot(spk);
+    const int leader = mask ? __ffsll((unsigned long long)mask) - 1 : 0;
+    const uint32_t cntw = (uint32_t)__popcll(mask);
+    uint32_t woff = 0u;
+    if (mask && lane == leader) woff = atomicAdd(&s_cnt, cntw);
+    woff = __shfl(woff, leader);
+    __syncthreads();
+    const uint32_t cnt = s_cnt;
+    if (threadIdx.x == 0 && cnt) {
+      s_base = atomicAdd(net.total + sub, cnt);
+      atomicAdd(&net.counters[blockIdx.x & (NCTR - 1)], (unsigned long long)cnt);
+    }
+    __syncthreads();
+    if (spk) {
+      const uint32_t pos = woff + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+      s_ids[pos] = (uint32_t)n;
+      s_amp[pos] = a_push;
+      const uint32_t r = s_base + pos;
+      if (r < cap) {
+        net.rec_ids[off + r] = (uint32_t)n;
+        net.rec_k[off + r] = (uint32_t)k;
+      } else {
+        atomicOr(net.overflow, 1u);
+      }
+      const uint32_t p0 = __shfl(pid, leader);
+      const bool same = __ballot(spk && pid != p0) == 0ull;
+      if (same) {
+        if (lane == leader) atomicAdd(&net.pop_cnt[p0], (unsigned long long)cntw);
+      } else {
+        atomicAdd(&net.pop_cnt[pid], 1ull);
+      }
+    }
+    __syncthreads();
+    // push of the block's spikes of step k: one wave per spike, lanes over its row
+    const int32_t s0 = slot_of(net.tmod[0], k, net.D);
+    for (uint32_t i = (uint32_t)wave; i < cnt; i += (uint32_t)wpb) {
+      const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ids[i]);
+      const uint32_t jl = j / (uint32_t)B, rep = j - jl * (uint32_t)B;
+      const T amp = s_amp[i];
+      const T ga = amp * T(net.gainr[0 * B + rep]), gg = amp * T(net.gainr[1 * B + rep]),
+              gh = amp * T(net.gainr[3 * B + rep]), gl = amp * T(net.gainr[4 * B + rep]);
+      const uint64_t r0 = ldc(net.rp_p + jl), r1 = ldc(net.rp_p + jl + 1);
+      if (lane == 0) { events += r1 - r0; nout += ldc(net.nmda_out + jl); }
+      for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
+        const uint32_t m = ldc(net.meta_p + e);
+        const uint32_t col = (m & 0x00FFFFFFu) * (uint32_t)B + rep;
+        const uint32_t li = m >> 30;
+        const T sc = li == 0 ? ga : li == 1 ? gg : li == 2 ? gh : gl;
+        int32_t slot = s0 + (int32_t)((m >> 24) & 63u);
+        slot -= slot >= net.D ? net.D : 0;
+        atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(ldc(net.w_p + e)) * sc);
+      }
+    }
+  }
+  block_add_events(net, events, nout);
 }
 
 // Push of the last step of a fused batch of K launches; publishes the end

@@ -32,6 +32,9 @@ static void to_pop_conf(const PopDecl &p, fs_pop_conf &c) {
     c.tau[id] = r.tau; c.tau_xt[id] = r.tau_xt; c.vrev[id] = r.vrev;
     c.freq_ext[id] = r.fext; c.ext_eff[id] = r.ext_eff; c.ext_con[id] = r.ext_con;
   }
+  const float stp[8] = {p.stp.pv, p.stp.tD, p.stp.aF, p.stp.tF, p.stp.aCap, p.stp.tCap,
+                        p.stp.aCar, p.stp.tCar};
+  for (int k = 0; k < 8; ++k) c.stp[k] = stp[k];
 }
 
 // Parameter derivation of code/NetSimGnl2_BulPp.cpp:159-278, in float.
@@ -69,6 +72,18 @@ static void derive(const fs_pop_conf &c, int32_t rfped, int32_t spidy, float dt,
     q.pb[r] = c.freq_ext[r] * dt * 1e-3f;
     q.gext[r] = c.ext_eff[r] * c.ext_con[r];
   }
+  // short-term plasticity (:185-200); CarBase = 20 uM (code/Inc.h:27)
+  q.stp_pv = c.stp[0];
+  q.stp_dfD = std::exp(-dt / c.stp[1]);
+  q.stp_efD = 1.0f - q.stp_dfD;
+  q.stp_aF = c.stp[2];
+  q.stp_dfF = std::exp(-dt / c.stp[3]);
+  q.stp_efF = c.stp[3] * (1.0f - q.stp_dfF);
+  q.stp_aCap = c.stp[4];
+  q.stp_dfCap = std::exp(-dt / c.stp[5]);
+  q.stp_aCar = c.stp[6];
+  q.stp_dfCar = std::exp(-dt / c.stp[7]);
+  q.stp_efCar = (1.0f - q.stp_dfCar) * 20.0f * 1e-6f;
   // MNoiseMean = MNoiseSTD = 0: injection off (:142-148)
   q.act = act & ~(uint32_t)FS_ACT_INJ;
   q.inj_mean = 0.0f;
@@ -125,7 +140,9 @@ void build_network(const Config &cfg, const BuildOptions &opt, BuiltNet &out) {
   // seeds (:80-127)
   const int RpSize = std::max(1, opt.repeats);
   out.rseeds.assign(RpSize, 0);
-  if (opt.udf_seed) out.rseeds[0] = opt.udf_seed_value;
+  // UDFRDSEED_ACT is STD_ACT's bit (code/Inc.h:275,280): -STD alone also
+  // selects the command-line seed, 0 when -udfsed is not given
+  if (opt.udf_seed || (opt.act_ctr & FS_ACT_STD)) out.rseeds[0] = opt.udf_seed_value;
   else if (!cfg.seeds.empty()) out.rseeds[0] = cfg.seeds[0];
   else out.rseeds[0] = std::random_device()();
   std::mt19937 rd(out.rseeds[0]);

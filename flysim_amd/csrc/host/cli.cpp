@@ -32,9 +32,10 @@ void print_help() {
             << "-rp 1                 # repeats\n"
             << "-t 4                  # thread count (kept for connectome parity)\n"
             << "-dt 0.1               # time step (ms)\n"
-            << "-STP / -STD / -LTP    # plasticity switches (not supported by the GPU engine)\n"
+            << "-STP / -STD           # short-term plasticity (facilitation + depression / depression)\n"
+            << "-LTP                  # long-term plasticity (not supported by the GPU engine)\n"
             << "-nmodel GNL2          # neuron model (GNL2 only)\n"
-            << "-udfsed N             # user-defined random seed\n"
+            << "-udfsed N             # user-defined random seed (switches STD on, as the reference does)\n"
             << "-fp64                 # double-precision state (default float)\n"
             << "-ext replay|philox    # external input: the reference's mt19937 stream\n"
             << "                      # (replay) or the counter-based generator (default)\n";
@@ -98,12 +99,12 @@ int cli_main(int argc, const char *const *argv) {
     else help = true;
   }
   if (help) { print_help(); return FS_OK; }
-  if (stp || stdp || ltp || udf) {
-    // -udfsed shares STD_ACT's bit in the reference (code/Inc.h:275,280)
-    return set_error(FS_ERR_ARG, "short/long-term plasticity (-STP/-STD/-LTP, and -udfsed which "
-                                 "implies STD) is not supported by the GPU engine");
-  }
+  if (ltp) return set_error(FS_ERR_ARG, "long-term plasticity (-LTP) is not supported by the GPU engine");
   fs_cli_opts o{};
+  o.stp = stp;
+  o.std_ = stdp;
+  o.udf_seed = udf;
+  o.udf_seed_value = seed;
   o.dt = dt; o.threads = th; o.repeats = rp;
   o.out_mem = om_en ? om.c_str() : nullptr;
   o.out_spi = os_en ? os_.c_str() : nullptr;

@@ -41,6 +41,9 @@ void pop_conf_of(const PopDecl &p, float dt, fs_pop_conf &c) {
     c.tau[id] = r.tau; c.tau_xt[id] = r.tau_xt; c.vrev[id] = r.vrev;
     c.freq_ext[id] = r.fext; c.ext_eff[id] = r.ext_eff; c.ext_con[id] = r.ext_con;
   }
+  const float stp[8] = {p.stp.pv, p.stp.tD, p.stp.aF, p.stp.tF, p.stp.aCap, p.stp.tCap,
+                        p.stp.aCar, p.stp.tCar};
+  for (int k = 0; k < 8; ++k) c.stp[k] = stp[k];
 }
 
 }  // namespace fs

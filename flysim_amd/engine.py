@@ -12,7 +12,7 @@ from .workloads import Network, Population
 
 STATE_FIELDS = {"V": 0, "count": 1, "nstate": 2, "gA": 3, "gG": 4, "gN": 5, "gH": 6, "gL": 7,
                 "s_lin": 8, "x_nmda": 9, "s_nmda": 10, "extA": 11, "extG": 12, "extH": 13,
-                "extL": 14, "ext_xN": 15, "ext_sN": 16}
+                "extL": 14, "ext_xN": 15, "ext_sN": 16, "stp_D": 17, "stp_F": 18}
 
 
 def pop_conf_of(p: Population) -> A.PopConf:
@@ -20,6 +20,8 @@ def pop_conf_of(p: Population) -> A.PopConf:
     c = A.PopConf()
     c.cm, c.taum, c.vth, c.el, c.vreset = p.cm, p.taum, p.vth, p.rest, p.reset
     c.refractory_ms, c.spike_delay_ms = p.refractory_ms, p.spike_delay_ms
+    for k, v in enumerate(getattr(p, "stp", None) or A.STP_DEFAULTS):
+        c.stp[k] = v
     for r in p.receptors:
         k = r.kind
         c.declared[k] = 1
@@ -119,9 +121,15 @@ class Conf:
     """A network.conf / network.pro pair loaded and built by the library
     (parser and builder of the drop-in API)."""
 
-    def __init__(self, conf: str, pro: str, dt: float = 0.1, threads: int = 1, repeats: int = 1):
+    def __init__(self, conf: str, pro: str, dt: float = 0.1, threads: int = 1, repeats: int = 1,
+                 stp: bool = False, std: bool = False, udf_seed: Optional[int] = None):
+        """stp / std / udf_seed: the -STP, -STD and -udfsed switches (code/main.cpp:90-95);
+        -udfsed also switches STD on, and -STD alone seeds with 0, as the reference does."""
         o = A.CliOpts()
         o.dt, o.threads, o.repeats = dt, threads, repeats
+        o.stp, o.std_ = int(stp), int(std)
+        if udf_seed is not None:
+            o.udf_seed, o.udf_seed_value = 1, int(udf_seed)
         h = C.c_void_p()
         A.check(A.lib().fs_conf_load(conf.encode(), pro.encode(), C.byref(o), C.byref(h)))
         self._h = h

@@ -1,11 +1,12 @@
 """Autotuner of the engine's launch schedule.
 
 Times short runs of candidate schedules (neuron-update block size, NMDA
-gather lanes per row, delivery block size, steps per captured graph, fused
-one-launch step on/off) on the network at hand and keeps the fastest.  Results
-are cached in a JSON file keyed by a signature of the network's shape (neuron
-and synapse counts by class, delays, precision, device) so that sweeps of many
-same-shaped networks tune once.
+gather lanes per row, delivery block size, steps per captured graph, and the
+step path: windowed, fused one-launch or two-launch) on the network at hand
+and keeps the fastest.  Results are cached in a JSON file keyed by a
+signature of the network's shape (neuron and synapse counts by class,
+delays, precision, device) so that sweeps of many same-shaped networks tune
+once.
 """
 from __future__ import annotations
 
@@ -29,9 +30,11 @@ def signature(eng: Engine, extra: str = "") -> str:
 
 
 def candidates(replicas: int = 1, has_nmda: bool = True, fused_ok: bool = True) -> List[Dict]:
-    """Block sizes x NMDA lanes per row x fused on/off.  Replicas in whole
-    waves read one CSR row per wave, so one lane per row is the natural
-    choice there; single networks spread a row over 1-16 lanes."""
+    """Block sizes x NMDA lanes per row x step path (windowed: as many steps
+    per launch as the smallest delay allows; fused one launch per step; two
+    launches per step).  Replicas in whole waves read one CSR row per wave,
+    so one lane per row is the natural choice there; single networks spread
+    a row over 1-16 lanes."""
     blocks = [128, 256, 512, 1024]
     if not has_nmda:
         lanes = [8]
@@ -40,10 +43,11 @@ def candidates(replicas: int = 1, has_nmda: bool = True, fused_ok: bool = True) 
     else:
         lanes = [1, 2, 4, 8, 16]
     out = []
+    paths = [(0, 64)] + ([(1, 0)] if fused_ok else []) + [(0, 0)]
     for b, L in itertools.product(blocks, lanes):
-        for f in ([1, 0] if fused_ok else [0]):
+        for f, w in paths:
             out.append(dict(update_block=b, gather_lanes=L, push_block=256, steps_per_graph=100,
-                            use_graph=1, fused=f))
+                            use_graph=1, fused=f, window=w))
     return out
 
 

@@ -29,6 +29,8 @@ import numpy as np
 # RVs/RPs members, code/neuron_affi.h:637-656).
 AMPA, GABA, NMDA, ACH, GCL = 0, 1, 2, 3, 4
 RECEPTOR_NAMES = {AMPA: "AMPA", GABA: "GABA", NMDA: "NMDA", ACH: "Ach", GCL: "GluCl"}
+# network.conf keys of the short-term plasticity parameters (code/parser.cpp:368-383)
+STP_KEYS = ("STP_pv", "STP_tD", "STP_aF", "STP_tF", "STP_aCap", "STP_tCap", "STP_aCar", "STP_tCar")
 
 
 @dataclasses.dataclass
@@ -57,6 +59,9 @@ class Population:
     spike_delay_ms: float = 0.0
     self_connection: bool = False
     receptors: List[Receptor] = dataclasses.field(default_factory=list)
+    # STP_pv, STP_tD, STP_aF, STP_tF, STP_aCap, STP_tCap, STP_aCar, STP_tCar
+    # (None: the reference defaults, code/parser.cpp:229-243)
+    stp: Optional[Sequence[float]] = None
 
 
 @dataclasses.dataclass
@@ -312,6 +317,9 @@ def write_conf(net: Network, path: str) -> None:
                        _fmt(pop.spike_delay_ms)))
         if pop.self_connection:
             out.append("SelfConnection=true\n")
+        if pop.stp is not None:
+            for key, v in zip(STP_KEYS, pop.stp):
+                out.append("%s=%s\n" % (key, _fmt(v)))
         for r in pop.receptors:
             out.append("Receptor:%s\nTau=%s\nTauXT=%s\nRevPot=%s\nFreqExt=%s\n"
                        "MeanExtEff=%s\nMeanExtCon=%s\nEndReceptor\n" % (

@@ -55,6 +55,10 @@ typedef struct fs_pop_params {
   uint32_t act;           /* FS_ACT_* bits of this population */
   float inj_mean, inj_std;/* Gaussian current injection (nA) */
   float taum;             /* ms */
+  /* short-term plasticity of the population's outgoing synapses, used when
+   * act has FS_ACT_STP or FS_ACT_STD (code/NetSimGnl2_BulPp.cpp:185-200) */
+  float stp_pv, stp_dfD, stp_efD, stp_aF, stp_dfF, stp_efF, stp_aCap, stp_dfCap;
+  float stp_aCar, stp_dfCar, stp_efCar;
 } fs_pop_params;
 
 /* Conf-level population values (NetConf + NeuRep, code/parser.h:9-66). */
@@ -63,6 +67,10 @@ typedef struct fs_pop_conf {
   float refractory_ms, spike_delay_ms;
   int32_t declared[5];    /* receptor declared, by receptor id */
   float tau[5], tau_xt[5], vrev[5], freq_ext[5], ext_eff[5], ext_con[5];
+  /* STP_pv, STP_tD, STP_aF, STP_tF, STP_aCap, STP_tCap, STP_aCar, STP_tCar;
+   * reference defaults 0.6, 300, 0.0035, 7000, 250, 2, 0.1, 2000
+   * (code/parser.cpp:229-243) */
+  float stp[8];
 } fs_pop_conf;
 
 /* Network handed to fs_create.  Neurons may be ordered arbitrarily; synapses
@@ -181,7 +189,9 @@ int fs_step_index(fs_handle *h, int64_t *step);
  * 0 V, 1 refractory count, 2 NState flags, 3..7 synaptic conductance of the
  * last Spike phase by receptor A,G,N,H,L, 8 linear presynaptic gate,
  * 9 NMDA presynaptic x, 10 NMDA presynaptic s, 11..14 external gate of
- * AMPA, GABA, ACh, GluCl, 15/16 external NMDA x/s.  Written as double[N]. */
+ * AMPA, GABA, ACh, GluCl, 15/16 external NMDA x/s, 17/18 presynaptic STP
+ * depression D / facilitation F (VarStp, with -STP or -STD only).  Written as
+ * double[N]. */
 int fs_get_state(fs_handle *h, int32_t field, double *out);
 
 /* Per-population spike counts accumulated since fs_reset (uint64[n_pop]). */
@@ -189,11 +199,16 @@ int fs_pop_spike_counts(fs_handle *h, uint64_t *out);
 
 /* Launch schedule (autotuner output): block size of the neuron update,
  * lanes per row of the NMDA gather, block size of the delivery, steps per
- * captured graph, graph on/off, and the one-launch-per-step fused path
- * (used only when every synapse has a delay of at least one step;
- * fs_get_schedule reports fused = 1 only while that path is in effect). */
+ * captured graph, graph on/off, the one-launch-per-step fused path (used
+ * only when every synapse has a delay of at least one step), and window:
+ * steps per launch of the temporal-blocking path (0 = off; clamped to 1 +
+ * the smallest synaptic delay in steps, and to 64).  The windowed path takes
+ * precedence over the fused one.  fs_get_schedule reports the path in
+ * effect: fused = 1 only while the fused path runs, window = the steps per
+ * launch actually used (0 when off). */
 typedef struct fs_schedule {
   int32_t update_block, gather_lanes, push_block, steps_per_graph, use_graph, fused;
+  int32_t window;
 } fs_schedule;
 int fs_get_schedule(fs_handle *h, fs_schedule *s);
 int fs_set_schedule(fs_handle *h, const fs_schedule *s);

@@ -20,7 +20,7 @@ namespace {
 // NState bits (code/Inc.h:225-231) and ActCtr bits (code/Inc.h:253-280).
 constexpr uint32_t FIRE_F = 1, SPIKE_F = 2, MEMDV_OVF = 16, MEMDV_UDF = 32;
 constexpr uint32_t AMPA_ACT = 1, GABA_ACT = 2, NMDA_ACT = 4, ACH_ACT = 32,
-                   GCL_ACT = 64, INJ_ACT = 128;
+                   GCL_ACT = 64, INJ_ACT = 128, STD_ACT = 16384, STP_ACT = 32768;
 // receptor ids of this library: A=0 G=1 N=2 H(ACh)=3 L(GluCl)=4
 constexpr int RA = 0, RG = 1, RN = 2, RH = 3, RL = 4;
 // parser keywords used by ChangeExtFreq events (code/Inc.h:35-47)
@@ -36,6 +36,8 @@ struct NeuPar {
   float pb[5];    // reference pb[] order: A, G, ACh, GluCl, NMDA
   float gext[5];  // same order
   float taum;
+  // short-term plasticity (ParStp2, code/NetSimGnl2_BulPp.cpp:185-200)
+  float pv, dfD, efD, aF, dfF, efF, aCap, dfCap, aCar, dfCar, efCar;
 };
 
 NeuPar derive(const or_pop &p, float dt) {
@@ -75,6 +77,17 @@ NeuPar derive(const or_pop &p, float dt) {
     q.pb[pbslot[r]] = p.freq_ext[r] * dt * 1e-3f;
     q.gext[pbslot[r]] = p.ext_eff[r] * p.ext_con[r];
   }
+  q.pv = p.stp[0];
+  q.dfD = std::exp(-dt / p.stp[1]);
+  q.efD = 1.0f - q.dfD;
+  q.aF = p.stp[2];
+  q.dfF = std::exp(-dt / p.stp[3]);
+  q.efF = p.stp[3] * (1.0f - q.dfF);
+  q.aCap = p.stp[4];
+  q.dfCap = std::exp(-dt / p.stp[5]);
+  q.aCar = p.stp[6];
+  q.dfCar = std::exp(-dt / p.stp[7]);
+  q.efCar = (1.0f - q.dfCar) * 20.0f * 1e-6f;  // CarBase = 20 uM (code/Inc.h:27)
   return q;
 }
 
@@ -189,6 +202,7 @@ struct Sim final : SimBase {
   std::vector<int32_t> cnt;
   std::vector<uint32_t> ns;
   std::vector<T> gst, gxt;
+  std::vector<T> sD, sF, sCap, sCar;  // presynaptic STP state (VarStp)
   // per-gate copies of the postsynaptic decay constants (RPs[PosID])
   std::vector<float> gdf, gdfxt, gefst, gas;
   std::vector<uint8_t> gnm;
@@ -238,6 +252,7 @@ struct Sim final : SimBase {
     for (auto &v : g) v.resize(N);
     for (auto &v : est) v.resize(N);
     ext_xt.resize(N); ext_st.resize(N);
+    sD.assign(N, T(1)); sF.assign(N, T(1)); sCap.assign(N, T(0)); sCar.assign(N, T(20));
     gst.assign(G, T(0)); gxt.assign(G, T(0));
     gdf.resize(G); gdfxt.resize(G); gefst.resize(G); gas.resize(G); gnm.resize(G);
     for (int k = 0; k < G; ++k) {
@@ -261,6 +276,9 @@ struct Sim final : SimBase {
       for (auto &v : est) v[i] = T(0);
       ext_xt[i] = T(0); ext_st[i] = T(0);
       for (int k = 0; k < 5; ++k) par[i].pb[k] = pb0[k][i];
+      // VarStp (code/EqsGnl2.cpp:353-362); under STD only D is initialised
+      // and F stays 1 here (the reference leaves it unset)
+      sD[i] = T(1); sF[i] = T(1); sCap[i] = T(0); sCar[i] = T(20.0f);
     }
     std::fill(gst.begin(), gst.end(), T(0));
     std::fill(gxt.begin(), gxt.end(), T(0));
@@ -385,16 +403,45 @@ struct Sim final : SimBase {
     Isyn[i] = T(0);
   }
 
-  // ThdPGatVar + ThdSynapse gate rule (code/EqsGnl2.cpp:404-502, 866-932)
+  // ThdPGatVar + ThdSynapse gate rule (code/EqsGnl2.cpp:404-502, 866-932),
+  // with STP (D*F, :505-603, :728-794) or STD (D, :605-703, :797-863) in
+  // place of the unit increment on a presynaptic spike
+  inline T increment(uint32_t pre) const {
+    if (act0 & STP_ACT) return sD[pre] * sF[pre];
+    if (act0 & STD_ACT) return sD[pre];
+    return T(1);
+  }
   inline void gate_update(int k) {
     const bool spk = ns[gpre[k]] & SPIKE_F;
     if (gnm[k]) {
-      if (spk) gxt[k] = gxt[k] + T(1);
+      if (spk) gxt[k] = gxt[k] + increment(gpre[k]);
       else gxt[k] = gxt[k] * T(gdfxt[k]);
       gst[k] = T(gdf[k]) * gst[k] + T(gefst[k]) * T(gas[k]) * gxt[k] * (T(1) - gst[k]);
     } else {
-      if (spk) gst[k] = gst[k] + T(1);
+      if (spk) gst[k] = gst[k] + increment(gpre[k]);
       else gst[k] = gst[k] * T(gdf[k]);
+    }
+  }
+
+  // EqsGnl2::STP / STD (code/EqsGnl2.cpp:1048-1090), after all gates
+  inline void stp_update(int i) {
+    const NeuPar &q = par[i];
+    const bool spk = ns[i] & SPIKE_F;
+    if (act0 & STP_ACT) {
+      if (spk) {
+        sCar[i] = sCar[i] + T(q.aCar);
+        sCap[i] = sCap[i] + T(q.aCap);
+        sF[i] = sF[i] + (sCap[i] + sCar[i]) * (T(1) - sF[i]) * T(q.aF);
+        sD[i] = sD[i] * (T(1) - T(q.pv) * sF[i]);
+      } else {
+        sCar[i] = T(q.dfCar) * sCar[i] + T(q.efCar);
+        sCap[i] = T(q.dfCap) * sCap[i];
+        sF[i] = T(q.dfF) * sF[i] + T(q.efF) * (sCap[i] + sCar[i]) * (T(1) - sF[i]) * T(q.aF);
+        sD[i] = T(q.dfD) * sD[i] + T(q.efD);
+      }
+    } else if (act0 & STD_ACT) {
+      if (spk) sD[i] = sD[i] * (T(1) - T(q.pv));
+      else sD[i] = T(q.dfD) * sD[i] + T(q.efD);
     }
   }
 
@@ -416,6 +463,8 @@ struct Sim final : SimBase {
     for (int i = 0; i < N; ++i) { bginit(i); vmeul(i); }  // Activation
     for (int k = 0; k < G; ++k) gate_update(k);             // Spike
     for (int r = 0; r < R; ++r) row_sum(r);
+    if (act0 & (STP_ACT | STD_ACT))
+      for (int i = 0; i < N; ++i) stp_update(i);
   }
 
   int64_t run(int64_t n, int64_t *ss, int32_t *sn, int64_t cap) override {
@@ -447,6 +496,8 @@ struct Sim final : SimBase {
         case 9: case 10: case 11: case 12: v = (double)est[f - 9][i]; break;
         case 14: v = (double)ext_xt[i]; break;
         case 15: v = (double)ext_st[i]; break;
+        case 16: v = (double)sD[i]; break;
+        case 17: v = (double)sF[i]; break;
         default: v = 0;
       }
       o[i] = v;

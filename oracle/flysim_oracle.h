@@ -16,6 +16,7 @@
  *   code/EqsGnl2.cpp:339-401   VarInit
  *   code/EqsGnl2.cpp:404-502   ThdSynapse (single-synapse gates + pull-sum)
  *   code/EqsGnl2.cpp:866-1002  ThdPGatVar / ThdPSynapse (group gates + gather)
+ *   code/EqsGnl2.cpp:505-863, 1048-1090  STP / STD gates and presynaptic state
  *   code/EqsGnl2.cpp:1098-1131 Activation / Spike dispatch
  *   code/NetSimGnl2.cpp:52-141 DoEvt,  :231-330 SimOne / Run seeding
  *   code/NetSimGnl2_BulPp.cpp:159-278 per-neuron parameter derivation
@@ -42,6 +43,8 @@ typedef struct {
   /* receptor params, index AMPA=0 GABA=1 NMDA=2 ACH=3 GCL=4 */
   int32_t declared[5];
   float tau[5], tau_xt[5], vrev[5], freq_ext[5], ext_eff[5], ext_con[5];
+  /* STP_pv, STP_tD, STP_aF, STP_tF, STP_aCap, STP_tCap, STP_aCar, STP_tCar */
+  float stp[8];
 } or_pop;
 
 typedef struct {

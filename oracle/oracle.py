@@ -23,7 +23,7 @@ class OrPop(C.Structure):
                 ("vreset", C.c_float), ("rfped", C.c_int32), ("spidy", C.c_int32),
                 ("declared", C.c_int32 * 5), ("tau", C.c_float * 5), ("tau_xt", C.c_float * 5),
                 ("vrev", C.c_float * 5), ("freq_ext", C.c_float * 5), ("ext_eff", C.c_float * 5),
-                ("ext_con", C.c_float * 5)]
+                ("ext_con", C.c_float * 5), ("stp", C.c_float * 8)]
 
 
 class OrNet(C.Structure):
@@ -70,7 +70,7 @@ def lib():
 
 FIELDS = {"V": 0, "count": 1, "nstate": 2, "gA": 3, "gG": 4, "gN": 5, "gH": 6, "gL": 7,
           "Isyn": 8, "extA": 9, "extG": 10, "extH": 11, "extL": 12, "ext_xN": 14,
-          "ext_sN": 15}
+          "ext_sN": 15, "stp_D": 16, "stp_F": 17}
 PARAMS = {"df": 0, "ef": 1, "gl": 2, "dvmax": 3, "dfst_A": 4, "dfst_G": 5, "dfst_N": 6,
           "dfxt_N": 7, "efst_N": 8, "pb_A": 9, "gext_A": 10}
 
@@ -94,6 +94,8 @@ class Oracle:
             # steps exactly as the reference parser derives them
             o.rfped = int(round(pc.refractory_ms / dt)) if "rfped" not in layout else int(layout["rfped"][i])
             o.spidy = int(round(pc.spike_delay_ms / dt)) if "spidy" not in layout else int(layout["spidy"][i])
+            for r in range(8):
+                o.stp[r] = pc.stp[r]
             for r in range(5):
                 o.declared[r] = pc.declared[r]
                 o.tau[r], o.tau_xt[r], o.vrev[r] = pc.tau[r], pc.tau_xt[r], pc.vrev[r]

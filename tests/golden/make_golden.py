@@ -5,7 +5,7 @@ Runs only where the reference sources exist (this build container).  For
 each case it writes the network in the reference's text format, runs the
 reference with `-t 1 -rp 1` and keeps only small outputs: spike rasters as
 (step, neuron) pairs, synapse orders from network.log, the bytes of small
-output files, and digests.  Usage (repo root):  python tests/golden/make_golden.py
+output files, and digests.  Usage (repo root):  python tests/golden/make_golden.py [plasticity]
 """
 from __future__ import annotations
 
@@ -353,6 +353,51 @@ def example_case(store):
     return meta
 
 
+# STP parameters on ExcA (the other populations keep the parser's defaults)
+STP_KEYS = """STP_pv=0.45
+STP_tD=180
+STP_aF=0.004
+STP_tF=5000
+STP_aCap=200
+STP_tCap=3
+STP_aCar=0.2
+STP_tCar=1500
+"""
+
+
+def stp_conf():
+    return MODULE_CONF.replace("SpikeDly=0.5\n", "SpikeDly=0.5\n" + STP_KEYS, 1)
+
+
+def std_conf():
+    """Under -STD the reference's group NMDA gates add D*F with F never set
+    (code/EqsGnl2.cpp:797-863 against :353-362), so the STD fixture keeps
+    NMDA on single synapses (one target per presynaptic neuron)."""
+    c = stp_conf()
+    a = "TargetReceptor=NMDA\nMeanEff=0.05\nConnectivity=0.3\n"
+    assert a in c
+    # ceil(40 * 0.02f) = 1 target (0.025f would round up to 2)
+    return c.replace(a, "TargetReceptor=NMDA\nMeanEff=0.3\nConnectivity=0.02\n")
+
+
+def plasticity_case(store, files):
+    """-STP, -STD (seed 0: -STD sets the user-seed bit) and -udfsed N (STD on)."""
+    meta = {}
+    for name, conf, args in [("stp", stp_conf(), ["-STP"]), ("std", std_conf(), ["-STD"]),
+                             ("udf", std_conf(), ["-udfsed", "4242"])]:
+        d = tempfile.mkdtemp(prefix="golden_")
+        open(os.path.join(d, "network.conf"), "w").write(conf)
+        open(os.path.join(d, "network.pro"), "w").write(MODULE_PRO)
+        run_ref(d, ["-t", "1"] + args)
+        store[name + "_raster"] = raster(os.path.join(d, "Spikes.dat"))
+        files[name + "_network.conf"] = conf
+        seed = re.search(r"random seed= (\d+)", open(os.path.join(d, "network.log")).read())
+        meta[name] = {"args": args, "spikes": int(len(store[name + "_raster"])),
+                      "seed": int(seed.group(1)) if seed else None}
+        shutil.rmtree(d)
+    return meta
+
+
 def event_digest_lines(log_text):
     i = log_text.index("time event information:")
     j = log_text.index("output files information:")
@@ -388,6 +433,21 @@ def seizure_case():
             hashlib.sha256("\n".join(names).encode()).hexdigest()}
 
 
+def update_plasticity():
+    """Regenerate only the plasticity fixtures, keeping the others as stored."""
+    build_ref()
+    store = dict(np.load(os.path.join(HERE, "reference_runs.npz")))
+    files = json.load(open(os.path.join(HERE, "reference_files.json")))
+    meta = json.load(open(os.path.join(HERE, "reference_meta.json")))
+    meta["plasticity"] = plasticity_case(store, files)
+    np.savez_compressed(os.path.join(HERE, "reference_runs.npz"), **store)
+    with open(os.path.join(HERE, "reference_files.json"), "w") as f:
+        json.dump(files, f, indent=0)
+    with open(os.path.join(HERE, "reference_meta.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta["plasticity"], indent=1))
+
+
 def main():
     build_ref()
     store, files, meta = {}, {}, {}
@@ -398,6 +458,7 @@ def main():
     meta["module"] = module_case(store, files)
     meta["noise"] = noise_case(store, files)
     meta["example"] = example_case(store)
+    meta["plasticity"] = plasticity_case(store, files)
     sz = seizure_case()
     if sz:
         meta["seizure_pro"] = sz
@@ -410,4 +471,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["plasticity"]:
+        update_plasticity()
+    else:
+        main()

@@ -1,0 +1,89 @@
+"""GPU parity tests of short-term plasticity (-STP, -STD, and -udfsed, which
+switches STD on in the reference), through the C ABI.
+
+Reference data: golden rasters of the reference simulator built from source
+with those switches (tests/golden/make_golden.py) and the CPU restatement
+(oracle/).  Bars: the drop-in command line in FP64 with the reference's
+external trains replayed reproduces the reference raster exactly over the
+300 ms module protocol (ChangeExtFreq events included); the presynaptic
+depression / facilitation state follows the FP64 oracle to 1e-9.
+"""
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from flysim_amd import _capi as A
+from flysim_amd import workloads as W
+from flysim_amd.engine import Conf, Engine
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = {"stp": (["-STP"], dict(stp=True)), "std": (["-STD"], dict(std=True)),
+         "udf": (["-udfsed", "4242"], dict(udf_seed=4242))}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_cli_plasticity_fp64_replay_exact_raster(golden, case, tmp_path):
+    files = golden["files"]
+    (tmp_path / "network.conf").write_text(files[case + "_network.conf"])
+    (tmp_path / "network.pro").write_text(files["module_network.pro"])
+    print("case plasticity %s: drop-in CLI, fp64 replay, 3000 steps" % case, flush=True)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        argv = [b"flysim_amd", b"-fp64", b"-ext", b"replay", b"-t", b"1"]
+        argv += [a.encode() for a in CASES[case][0]]
+        arr = (C.c_char_p * len(argv))(*argv)
+        A.check(A.lib().fs_cli_main(len(argv), arr))
+    finally:
+        os.chdir(cwd)
+    s = W.read_spikes(str(tmp_path / "Spikes.dat"))
+    r = np.stack([np.round(s[:, 0] / 1e-4).astype(np.int64), s[:, 1].astype(np.int64)], 1)
+    ref = golden["runs"][case + "_raster"].astype(np.int64)
+    assert len(ref) > 300
+    np.testing.assert_array_equal(r, ref)
+
+
+@pytest.mark.parametrize("case", ["stp", "std"])
+def test_plasticity_state_tracks_fp64_oracle(golden, case):
+    """D (and F under STP) of every neuron after 90 ms, before the first
+    protocol event, against the FP64 oracle fed the same trains."""
+    files = golden["files"]
+    steps = 900
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "network.conf"), "w").write(files[case + "_network.conf"])
+        open(os.path.join(d, "network.pro"), "w").write(files["module_network.pro"])
+        conf = Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"), **CASES[case][1])
+        bits = conf.ext_trains(steps)
+        print("case plasticity state %s: fp64 replay, %d steps" % (case, steps), flush=True)
+        eng = Engine(conf, precision=A.FS_FP64, ext_mode=A.FS_EXT_REPLAY)
+        eng.reset(conf.seed(0))
+        eng.load_ext_trains(bits, 0, steps)
+        orc = Oracle(conf.ref_layout(), conf.n_neuron, precision=1)
+        orc.reset(conf.seed(0))
+        orc.replay(bits, steps)
+        rg, ro = eng.raster(steps), orc.raster(steps)
+        np.testing.assert_array_equal(rg, ro)
+        assert len(ro) > 15
+        dg, do = eng.state("stp_D"), orc.state("stp_D")
+        assert do.min() < 0.9
+        np.testing.assert_allclose(dg, do, rtol=0, atol=1e-9)
+        if case == "stp":
+            np.testing.assert_allclose(eng.state("stp_F"), orc.state("stp_F"), rtol=0, atol=1e-9)
+        np.testing.assert_allclose(eng.state("V"), orc.state("V"), rtol=0, atol=1e-8)
+
+
+def test_plasticity_state_absent_without_switch(golden):
+    files = golden["files"]
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "network.conf"), "w").write(files["stp_network.conf"])
+        open(os.path.join(d, "network.pro"), "w").write(files["module_network.pro"])
+        conf = Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"))
+        eng = Engine(conf, precision=A.FS_FP32)
+        eng.reset(1)
+        with pytest.raises(A.FlysimError):
+            eng.state("stp_D")

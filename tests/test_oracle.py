@@ -50,6 +50,40 @@ def test_fp32_oracle_module_network_with_events(golden):
     np.testing.assert_array_equal(r, golden["runs"]["module_raster"].astype(np.int64))
 
 
+PLASTICITY = {"stp": dict(stp=True), "std": dict(std=True), "udf": dict(udf_seed=4242)}
+
+
+@pytest.mark.parametrize("case", ["stp", "std", "udf"])
+def test_fp32_oracle_short_term_plasticity(golden, case):
+    """-STP (depression D x facilitation F), -STD (D; seeds with 0) and
+    -udfsed 4242 (switches STD on) against the reference built from source."""
+    f = golden["files"]
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, f[case + "_network.conf"], f["module_network.pro"], **PLASTICITY[case])
+        assert c.seed(0) == golden["meta"]["plasticity"][case]["seed"]
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        r = o.raster(3000)
+        d_min = o.state("stp_D").min()
+    ref = golden["runs"][case + "_raster"].astype(np.int64)
+    assert len(ref) > 300
+    np.testing.assert_array_equal(r, ref)
+    assert d_min < 0.9  # depression engaged
+
+
+def test_plasticity_switch_changes_the_raster(golden):
+    """The same conf without the switch gives another raster: the fixture
+    pins the STP terms, not only the network."""
+    f = golden["files"]
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, f["stp_network.conf"], f["module_network.pro"])
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        r = o.raster(3000)
+    ref = golden["runs"]["stp_raster"]
+    assert r.shape != ref.shape or not np.array_equal(r, ref)
+
+
 def test_fp32_oracle_gaussian_injection_ziggurat(golden):
     """ChangeMembraneNoise events drive the Marsaglia-Tsang ziggurat stream."""
     f = golden["files"]
