@@ -592,6 +592,7 @@ struct Engine final : EngineBase {
       throw std::invalid_argument("push_block must be 64..1024, multiple of 64");
     if (sched.steps_per_graph < 1 || sched.steps_per_graph > 100000)
       throw std::invalid_argument("steps_per_graph out of range");
+    if (window() && ub > 512) throw std::invalid_argument("the windowed path takes update_block <= 512");
     net.gather_lanes = L;
     const int K = sched.steps_per_graph;
     // record buffer: every neuron could hold SPIKE_F every step (sticky

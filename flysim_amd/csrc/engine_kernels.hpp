@@ -170,193 +170,329 @@ __device__ __forceinline__ int32_t slot_of(int32_t t0mod, int32_t k, int32_t M) 
 }
 
 // ------------------------------------------------------------------ update
+// Per-neuron state between steps.  The one-step kernels load it, run one
+// step and store it; the windowed kernel keeps it in registers for all the
+// steps of a launch.  Only the fields the network uses are loaded/stored.
+template <typename T>
+struct NState {
+  T v;
+  uint32_t cs;
+  T G[4];          // running linear conductances (lin_mask)
+  T spre;          // linear presynaptic gate (has_lin_out)
+  T xn, sn;        // NMDA presynaptic x and the last step's s (has_nmda_out)
+  T elin[4];       // external linear gates (ext_mask)
+  T enx, ens;      // external NMDA gate
+  T sD, sF, sCap, sCar;  // short-term plasticity (stp_mode)
+};
+
+template <typename T>
+using PopC = const __attribute__((address_space(4))) DevPop<T>;
+
+// the state of neuron n as of the end of step k - 1 (hprev: gate history
+// slot of step k - 1)
+template <typename T>
+__device__ __forceinline__ void load_state(const Net<T> &net, PopC<T> &P, int32_t n, int32_t hprev,
+                                           NState<T> &S) {
+  const size_t N = (size_t)net.N;
+  S.v = net.V[n];
+  S.cs = net.cs[n];
+#pragma unroll
+  for (int li = 0; li < 4; ++li) {
+    if (net.lin_mask & (1 << li)) S.G[li] = net.G[li * N + n];
+    if (net.ext_mask & (1 << lin_rid(li))) S.elin[li] = net.elin[li * N + n];
+  }
+  if (P.has_lin_out) S.spre = net.spre[n];
+  if (P.has_nmda_out) {
+    S.xn = net.xn[n];
+    S.sn = net.shist[(size_t)hprev * N + n];
+  }
+  if (net.ext_mask & (1 << 2)) {
+    S.enx = net.enx[n];
+    S.ens = net.ens[n];
+  }
+  if (net.stp_mode) {
+    S.sD = net.sD[n];
+    if (net.stp_mode == STP_FULL) {
+      S.sF = net.sF[n];
+      S.sCap = net.sCap[n];
+      S.sCar = net.sCar[n];
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_state(const Net<T> &net, PopC<T> &P, int32_t n,
+                                            const NState<T> &S) {
+  const size_t N = (size_t)net.N;
+  net.V[n] = S.v;
+  net.cs[n] = S.cs;
+#pragma unroll
+  for (int li = 0; li < 4; ++li) {
+    if (net.lin_mask & (1 << li)) net.G[li * N + n] = S.G[li];
+    if (net.ext_mask & (1 << lin_rid(li))) net.elin[li * N + n] = S.elin[li];
+  }
+  if (P.has_lin_out) net.spre[n] = S.spre;
+  if (P.has_nmda_out) net.xn[n] = S.xn;  // s goes to the history every step
+  if (net.ext_mask & (1 << 2)) {
+    net.enx[n] = S.enx;
+    net.ens[n] = S.ens;
+  }
+  if (net.stp_mode) {
+    net.sD[n] = S.sD;
+    if (net.stp_mode == STP_FULL) {
+      net.sF[n] = S.sF;
+      net.sCap[n] = S.sCap;
+      net.sCar[n] = S.sCar;
+    }
+  }
+}
+
+// The linear conductance increments delivered for step k - 1 (its ring
+// slot), cleared for reuse.
+template <typename T>
+__device__ __forceinline__ void ring_take(const Net<T> &net, int32_t n, int32_t slot, T inc[4]) {
+#pragma unroll
+  for (int li = 0; li < 4; ++li) {
+    inc[li] = T(0);
+    if (!(net.lin_mask & (1 << li))) continue;
+    T *rs = net.ring + ((size_t)li * net.D + slot) * (size_t)net.N + n;
+    inc[li] = *rs;
+    *rs = T(0);
+  }
+}
+
+// Per-neuron inputs that stay fixed over a batch of steps: external event
+// probabilities (< 0: the receptor's activity bit is cleared), external
+// efficacy gain and the counter-based stream's key and counter (neuron
+// within its network instance; key = seed ^ replica key ^ instance key, so a
+// replica or a batched instance draws the same stream as the same network
+// run alone with that key).
+template <typename T>
+struct NConst {
+  float pb[5];
+  T eg;
+  uint64_t key;
+  uint32_t ctr;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_const(const Net<T> &net, PopC<T> &P, int32_t nl, int32_t rep,
+                                           uint32_t p0, NConst<T> &C) {
+  const float *pbp = net.pbr + (size_t)p0 * net.B + rep;
+  const size_t pbs = (size_t)net.P0 * net.B;  // receptor stride of pbr
+#pragma unroll
+  for (int r = 0; r < 5; ++r) C.pb[r] = (net.ext_mask & (1 << r)) ? pbp[r * pbs] : -1.0f;
+  C.eg = T(net.gainr[5 * net.B + rep]);
+  C.key = net.seed ^ net.skey[rep] ^ (((uint64_t)P.key_hi << 32) | P.key_lo);
+  C.ctr = (uint32_t)(nl - P.first);
+}
+
+// One neuron's step t (the reference's Activation, then its presynaptic
+// gates and plasticity) on state S: (1) linear conductances of the previous
+// Spike phase, (2) external gates, (3) membrane, (4) gates, (4b) STP/STD.
+// gn: NMDA conductance of the previous Spike phase; inc: ring_take's output.
+// Writes the NMDA gate history slot hslot (step t); everything else stays
+// in S.
+template <typename T, bool REPLAY>
+__device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const NConst<T> &C,
+                                            int32_t n, int64_t t, int32_t hslot, T gn,
+                                            const T inc[4], NState<T> &S, bool &spk, T &a_push) {
+  const int32_t N = net.N;
+  uint32_t cs = S.cs;
+  uint32_t cnt = cs & CNT_MASK;
+  T v = S.v;
+  // (1) G = df*G + slot
+  T g[5] = {T(0), T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int li = 0; li < 4; ++li) {
+    if (!(net.lin_mask & (1 << li))) continue;
+    const int rid = lin_rid(li);
+    const T gv = P.dfst[rid] * S.G[li] + inc[li];
+    S.G[li] = gv;
+    g[rid] = gv;
+  }
+  if (net.has_nmda_syn) g[2] = gn;
+  // (2) external Poisson gates, order A, G, ACh, GluCl, NMDA (BgInit);
+  // Philox counter (neuron within its instance, step), key C.key
+  const uint32_t act = P.act;
+  const uint64_t key = C.key;
+  const uint32_t ctr = C.ctr;
+  uint32_t rnd[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  if ((!REPLAY && net.ext_mask) || (act & ACT_INJ)) {
+    uint32_t c[4] = {ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0u};
+    philox10(c, (uint32_t)key, (uint32_t)(key >> 32));
+    rnd[0] = c[0]; rnd[1] = c[1]; rnd[2] = c[2]; rnd[3] = c[3];
+    if ((net.ext_mask & (1 << 4)) || (act & ACT_INJ)) {
+      uint32_t e[4] = {ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 1u};
+      philox10(e, (uint32_t)key, (uint32_t)(key >> 32));
+      rnd[4] = e[0]; rnd[5] = e[1]; rnd[6] = e[2]; rnd[7] = e[3];
+    }
+  }
+  auto draw = [&](int rid, float pb) -> bool {
+    if (REPLAY) {
+      const int64_t s = t - net.ext_first;
+      if (s < 0 || s >= net.ext_steps) return false;
+      return (net.ext_bits[((size_t)s * 5 + rid) * net.words + (n >> 5)] >> (n & 31)) & 1u;
+    }
+    return pb > u24(rnd[rid]);
+  };
+  const T eg = C.eg;
+#pragma unroll
+  for (int li = 0; li < 4; ++li) {
+    const int rid = lin_rid(li);
+    if (!(net.ext_mask & (1 << rid))) continue;
+    const float pb = C.pb[rid];
+    if (pb < 0.0f) continue;
+    T e = S.elin[li];
+    e = draw(rid, pb) ? e + T(1) : e * P.dfst[rid];
+    S.elin[li] = e;
+    g[rid] = g[rid] + P.gext[rid] * eg * e;
+  }
+  if (net.ext_mask & (1 << 2)) {
+    const float pb = C.pb[2];
+    if (pb >= 0.0f) {
+      T x = S.enx, s = S.ens;
+      x = draw(2, pb) ? x + T(1) : x * P.dfxt_n;
+      s = P.dfst[2] * s + P.efas_n * x * (T(1) - s);
+      S.enx = x;
+      S.ens = s;
+      g[2] = g[2] + P.gext[2] * eg * s;
+    }
+  }
+  T isyn = T(0);
+  if (act & ACT_INJ) {
+    // Box-Muller on the second Philox block (statistically the reference's
+    // Gaussian injection; the reference draws from a sequential ziggurat)
+    const float u1 = ((float)(rnd[6] >> 8) + 1.0f) * (1.0f / 16777216.0f);
+    const float u2 = u24(rnd[7]);
+    const float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+    isyn = P.inj_mean + P.inj_std * T(z);
+  }
+  // (3) membrane (VmEul / firing)
+  auto firing = [&]() {
+    if ((int32_t)cnt < P.rfped) {
+      v = cnt == 0 ? T(0) : P.vreset;
+      if ((int32_t)cnt == P.spidy) cs |= F_SPIKE;
+      else cs &= ~F_SPIKE;
+      cnt++;
+    } else {
+      cnt = 0;
+      cs &= ~F_FIRE;
+    }
+  };
+  if (cs & F_FIRE) {
+    firing();
+  } else {
+    const T i0 = g[0] * (P.vrev[0] - v);
+    const T i1 = g[1] * (P.vrev[1] - v);
+    T ex;
+    if constexpr (sizeof(T) == 4) ex = expf(T(-0.062f) * v);
+    else ex = exp(T(-0.062f) * v);
+    const T i2 = g[2] * (P.vrev[2] - v) / (T(1) + P.mg * ex / T(3.57f));
+    const T i3 = g[3] * (P.vrev[3] - v);
+    const T i4 = g[4] * (P.vrev[4] - v);
+    T I = i0 + i1 + i2 + i3 + i4 + isyn * T(1e3f) + P.leak_drive;
+    I = T(1e-3f) * I / P.cm;
+    cs &= ~(F_OVF | F_UDF);
+    if (I > P.dvmax) { I = P.dvmax; cs |= F_OVF; }
+    else if (I < -P.dvmax) { I = -P.dvmax; cs |= F_UDF; }
+    v = P.df * v + P.ef * I;
+    if (v > P.vth) { cs |= F_FIRE; firing(); }
+  }
+  spk = (cs & F_SPIKE) != 0;
+  // (4) presynaptic gates (the reference's per-synapse gate rule, shared);
+  // a spike adds 1, or D*F under STP and D under STD, the presynaptic
+  // state before this step's plasticity update (code/EqsGnl2.cpp:505-863)
+  T one = T(1);
+  if (net.stp_mode) {
+    one = S.sD;
+    if (net.stp_mode == STP_FULL) one = one * S.sF;
+  }
+  a_push = T(0);
+  if (P.has_lin_out) {
+    const T s = S.spre;
+    a_push = one + (T(1) - P.pre_df) * s;
+    S.spre = spk ? s + one : s * P.pre_df;
+  }
+  if (P.has_nmda_out) {
+    T x = S.xn;
+    const T s0 = S.sn;
+    x = spk ? x + one : x * P.pre_dfxt;
+    S.xn = x;
+    const T s1 = P.pre_dfst * s0 + P.pre_efas * x * (T(1) - s0);
+    S.sn = s1;
+    net.shist[(size_t)hslot * N + n] = s1;
+  }
+  // (4b) EqsGnl2::STP / STD (code/EqsGnl2.cpp:1048-1090), after the gates
+  if (net.stp_mode == STP_FULL) {
+    T d = S.sD, f = S.sF, cap = S.sCap, car = S.sCar;
+    if (spk) {
+      car = car + P.stp_aCar;
+      cap = cap + P.stp_aCap;
+      f = f + (cap + car) * (T(1) - f) * P.stp_aF;
+      d = d * (T(1) - P.stp_pv * f);
+    } else {
+      car = P.stp_dfCar * car + P.stp_efCar;
+      cap = P.stp_dfCap * cap;
+      f = P.stp_dfF * f + P.stp_efF * (cap + car) * (T(1) - f) * P.stp_aF;
+      d = P.stp_dfD * d + P.stp_efD;
+    }
+    S.sD = d; S.sF = f; S.sCap = cap; S.sCar = car;
+  } else if (net.stp_mode == STP_STD) {
+    const T d = S.sD;
+    S.sD = spk ? d * (T(1) - P.stp_pv) : P.stp_dfD * d + P.stp_efD;
+  }
+  S.v = v;
+  S.cs = (cs & ~CNT_MASK) | (cnt & CNT_MASK);
+}
+
+// Local neuron, replica and population of global neuron n.
+template <bool UNI>
+__device__ __forceinline__ void locate(const uint32_t *npop, int32_t B, int32_t n, int32_t &nl,
+                                       int32_t &rep, uint32_t &p0) {
+  nl = n / B;
+  if constexpr (UNI) nl = __builtin_amdgcn_readfirstlane(nl);
+  rep = n - nl * B;
+  p0 = ldc(npop + nl);
+  if constexpr (UNI) p0 = __builtin_amdgcn_readfirstlane(p0);
+}
+
+// One step of neuron n from and back to memory (n >= N: inactive lane,
+// spk = false); gn_given: the NMDA conductance comes from the caller (fused
+// step) instead of the gn array written by the separate gather.
+template <typename T, bool REPLAY, bool UNI>
+__device__ __forceinline__ void neuron_step_mem(const Net<T> &net, int32_t n, int64_t t, int32_t k,
+                                                bool gn_given, T gn_in, bool &spk, T &a_push,
+                                                uint32_t &pid) {
+  spk = false;
+  a_push = T(0);
+  pid = 0;  // population x replica index of the spike counters
+  if (n >= net.N) return;
+  int32_t nl, rep;
+  uint32_t p0;
+  locate<UNI>(net.npop, net.B, n, nl, rep, p0);
+  pid = p0 * (uint32_t)net.B + (uint32_t)rep;
+  PopC<T> &P = *(PopC<T> *)(net.pops + p0);
+  NState<T> S;
+  load_state(net, P, n, slot_of(net.tmod[1], k - 1, net.DN), S);
+  NConst<T> C;
+  load_const(net, P, nl, rep, p0, C);
+  T inc[4];
+  ring_take(net, n, slot_of(net.tmod[0], k - 1, net.D), inc);
+  const T gn = net.has_nmda_syn ? (gn_given ? gn_in : net.gn[n]) : T(0);
+  neuron_core<T, REPLAY>(net, P, C, n, t, slot_of(net.tmod[1], k, net.DN), gn, inc, S, spk, a_push);
+  store_state(net, P, n, S);
+}
+
 // One neuron's step t (the reference's Activation) followed by the wave's
 // spike compaction; every thread of the block must reach it (ballots, block
 // barriers); list_ctr[NSUB] and list_base are the sub-list counters and the
 // offset of this step's list within sub-list `sub` (the update block's index
-// modulo NSUB).  With
-// gn_given the NMDA conductance comes from the caller (fused step) instead of
-// the gn array written by the separate gather.
+// modulo NSUB).  With gn_given the NMDA conductance comes from the caller
+// (fused step) instead of the gn array written by the separate gather.
 // UNI: the caller guarantees that all 64 lanes of the wave hold the same
 // local neuron (replicas in whole waves); its index, population and
 // parameters are then wave-uniform and come through scalar loads.
-// neuron_core: steps (1)-(4b) for neuron n (n >= N: inactive lane, outputs
-// spk = false); update_neuron adds the compaction (5).
-template <typename T, bool REPLAY, bool UNI>
-__device__ __forceinline__ void neuron_core(const Net<T> &net, int32_t n, int64_t t, int32_t k,
-                                            bool gn_given, T gn_in, bool &spk, T &a_push,
-                                            uint32_t &pid) {
-  spk = false;
-  a_push = T(0);
-  pid = 0;  // population x replica index of the spike counters
-  if (n < net.N) {
-    const int32_t N = net.N;
-    int32_t nl = n / net.B;
-    if constexpr (UNI) nl = __builtin_amdgcn_readfirstlane(nl);
-    const int32_t rep = n - nl * net.B;
-    uint32_t p0 = ldc(net.npop + nl);
-    if constexpr (UNI) p0 = __builtin_amdgcn_readfirstlane(p0);
-    pid = p0 * (uint32_t)net.B + (uint32_t)rep;
-    const __attribute__((address_space(4))) DevPop<T> &P =
-        *(const __attribute__((address_space(4))) DevPop<T> *)(net.pops + p0);
-    uint32_t cs = net.cs[n];
-    uint32_t cnt = cs & CNT_MASK;
-    T v = net.V[n];
-    // (1) linear conductances of the previous Spike phase: G = df*G + slot
-    T g[5] = {T(0), T(0), T(0), T(0), T(0)};
-    const int32_t slot = slot_of(net.tmod[0], k - 1, net.D);
-#pragma unroll
-    for (int li = 0; li < 4; ++li) {
-      if (!(net.lin_mask & (1 << li))) continue;
-      const int rid = lin_rid(li);
-      T *rs = net.ring + ((size_t)li * net.D + slot) * N + n;
-      const T inc = *rs;
-      *rs = T(0);
-      T *gp = net.G + (size_t)li * N + n;
-      const T gv = P.dfst[rid] * *gp + inc;
-      *gp = gv;
-      g[rid] = gv;
-    }
-    if (net.has_nmda_syn) g[2] = gn_given ? gn_in : net.gn[n];
-    // (2) external Poisson gates, order A, G, ACh, GluCl, NMDA (BgInit)
-    // (counter = neuron within its network instance and step, key = seed ^
-    // replica key ^ instance key: a replica or a batched instance draws the
-    // same stream as the same network run alone with that key)
-    const uint32_t act = P.act;
-    const uint64_t key = net.seed ^ net.skey[rep] ^ (((uint64_t)P.key_hi << 32) | P.key_lo);
-    const uint32_t ctr = (uint32_t)(nl - P.first);
-    uint32_t rnd[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-    if ((!REPLAY && net.ext_mask) || (act & ACT_INJ)) {
-      uint32_t c[4] = {ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0u};
-      philox10(c, (uint32_t)key, (uint32_t)(key >> 32));
-      rnd[0] = c[0]; rnd[1] = c[1]; rnd[2] = c[2]; rnd[3] = c[3];
-      if ((net.ext_mask & (1 << 4)) || (act & ACT_INJ)) {
-        uint32_t e[4] = {ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 1u};
-        philox10(e, (uint32_t)key, (uint32_t)(key >> 32));
-        rnd[4] = e[0]; rnd[5] = e[1]; rnd[6] = e[2]; rnd[7] = e[3];
-      }
-    }
-    const float *pbp = net.pbr + (size_t)p0 * net.B + rep;
-    const size_t pbs = (size_t)net.P0 * net.B;  // receptor stride of pbr
-    auto draw = [&](int rid, float pb) -> bool {
-      if (REPLAY) {
-        const int64_t s = t - net.ext_first;
-        if (s < 0 || s >= net.ext_steps) return false;
-        return (net.ext_bits[((size_t)s * 5 + rid) * net.words + (n >> 5)] >> (n & 31)) & 1u;
-      }
-      return pb > u24(rnd[rid]);
-    };
-    const T eg = T(net.gainr[5 * net.B + rep]);
-#pragma unroll
-    for (int li = 0; li < 4; ++li) {
-      const int rid = lin_rid(li);
-      if (!(net.ext_mask & (1 << rid))) continue;
-      const float pb = pbp[rid * pbs];
-      if (pb < 0.0f) continue;
-      T *ep = net.elin + (size_t)li * N + n;
-      T e = *ep;
-      e = draw(rid, pb) ? e + T(1) : e * P.dfst[rid];
-      *ep = e;
-      g[rid] = g[rid] + P.gext[rid] * eg * e;
-    }
-    if (net.ext_mask & (1 << 2)) {
-      const float pb = pbp[2 * pbs];
-      if (pb >= 0.0f) {
-        T x = net.enx[n], s = net.ens[n];
-        x = draw(2, pb) ? x + T(1) : x * P.dfxt_n;
-        s = P.dfst[2] * s + P.efas_n * x * (T(1) - s);
-        net.enx[n] = x;
-        net.ens[n] = s;
-        g[2] = g[2] + P.gext[2] * eg * s;
-      }
-    }
-    T isyn = T(0);
-    if (act & ACT_INJ) {
-      // Box-Muller on the second Philox block (statistically the reference's
-      // Gaussian injection; the reference draws from a sequential ziggurat)
-      const float u1 = ((float)(rnd[6] >> 8) + 1.0f) * (1.0f / 16777216.0f);
-      const float u2 = u24(rnd[7]);
-      const float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
-      isyn = P.inj_mean + P.inj_std * T(z);
-    }
-    // (3) membrane (VmEul / firing)
-    auto firing = [&]() {
-      if ((int32_t)cnt < P.rfped) {
-        v = cnt == 0 ? T(0) : P.vreset;
-        if ((int32_t)cnt == P.spidy) cs |= F_SPIKE;
-        else cs &= ~F_SPIKE;
-        cnt++;
-      } else {
-        cnt = 0;
-        cs &= ~F_FIRE;
-      }
-    };
-    if (cs & F_FIRE) {
-      firing();
-    } else {
-      const T i0 = g[0] * (P.vrev[0] - v);
-      const T i1 = g[1] * (P.vrev[1] - v);
-      T ex;
-      if constexpr (sizeof(T) == 4) ex = expf(T(-0.062f) * v);
-      else ex = exp(T(-0.062f) * v);
-      const T i2 = g[2] * (P.vrev[2] - v) / (T(1) + P.mg * ex / T(3.57f));
-      const T i3 = g[3] * (P.vrev[3] - v);
-      const T i4 = g[4] * (P.vrev[4] - v);
-      T I = i0 + i1 + i2 + i3 + i4 + isyn * T(1e3f) + P.leak_drive;
-      I = T(1e-3f) * I / P.cm;
-      cs &= ~(F_OVF | F_UDF);
-      if (I > P.dvmax) { I = P.dvmax; cs |= F_OVF; }
-      else if (I < -P.dvmax) { I = -P.dvmax; cs |= F_UDF; }
-      v = P.df * v + P.ef * I;
-      if (v > P.vth) { cs |= F_FIRE; firing(); }
-    }
-    spk = (cs & F_SPIKE) != 0;
-    // (4) presynaptic gates (the reference's per-synapse gate rule, shared);
-    // a spike adds 1, or D*F under STP and D under STD, the presynaptic
-    // state before this step's plasticity update (code/EqsGnl2.cpp:505-863)
-    T inc = T(1);
-    if (net.stp_mode) {
-      inc = net.sD[n];
-      if (net.stp_mode == STP_FULL) inc = inc * net.sF[n];
-    }
-    if (P.has_lin_out) {
-      const T s = net.spre[n];
-      a_push = inc + (T(1) - P.pre_df) * s;
-      net.spre[n] = spk ? s + inc : s * P.pre_df;
-    }
-    if (P.has_nmda_out) {
-      T x = net.xn[n];
-      const int32_t sp = slot_of(net.tmod[1], k - 1, net.DN);
-      const int32_t sc = slot_of(net.tmod[1], k, net.DN);
-      const T s0 = net.shist[(size_t)sp * N + n];
-      x = spk ? x + inc : x * P.pre_dfxt;
-      net.xn[n] = x;
-      net.shist[(size_t)sc * N + n] = P.pre_dfst * s0 + P.pre_efas * x * (T(1) - s0);
-    }
-    // (4b) EqsGnl2::STP / STD (code/EqsGnl2.cpp:1048-1090), after the gates
-    if (net.stp_mode == STP_FULL) {
-      T d = net.sD[n], f = net.sF[n], cap = net.sCap[n], car = net.sCar[n];
-      if (spk) {
-        car = car + P.stp_aCar;
-        cap = cap + P.stp_aCap;
-        f = f + (cap + car) * (T(1) - f) * P.stp_aF;
-        d = d * (T(1) - P.stp_pv * f);
-      } else {
-        car = P.stp_dfCar * car + P.stp_efCar;
-        cap = P.stp_dfCap * cap;
-        f = P.stp_dfF * f + P.stp_efF * (cap + car) * (T(1) - f) * P.stp_aF;
-        d = P.stp_dfD * d + P.stp_efD;
-      }
-      net.sD[n] = d; net.sF[n] = f; net.sCap[n] = cap; net.sCar[n] = car;
-    } else if (net.stp_mode == STP_STD) {
-      const T d = net.sD[n];
-      net.sD[n] = spk ? d * (T(1) - P.stp_pv) : P.stp_dfD * d + P.stp_efD;
-    }
-    net.V[n] = v;
-    net.cs[n] = (cs & ~CNT_MASK) | (cnt & CNT_MASK);
-  }
-}
-
 template <typename T, bool REPLAY, bool UNI>
 __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int64_t t, int32_t k,
                                               bool gn_given, T gn_in, uint32_t *list_ctr,
@@ -364,7 +500,7 @@ __device__ __forceinline__ void update_neuron(const Net<T> &net, int32_t n, int6
   bool spk;
   T a_push;
   uint32_t pid;
-  neuron_core<T, REPLAY, UNI>(net, n, t, k, gn_given, gn_in, spk, a_push, pid);
+  neuron_step_mem<T, REPLAY, UNI>(net, n, t, k, gn_given, gn_in, spk, a_push, pid);
   // (5) spike-list compaction: wave ballot, then one block-level LDS count,
   // so one global atomic per block claims the block's range of the list
   // (a global atomic per wave serialises on one address at high rates)
@@ -557,9 +693,64 @@ __device__ __forceinline__ void block_add_events(const Net<T> &net, unsigned lon
   }
 }
 
+// Sum of the n entries of one lane's share of a row in a fixed order:
+// quadruples of consecutive entries into four partial sums (entry 4j+q into
+// sum q), the last n % 4 entries into the first, then (s0 + s1) + (s2 + s3).
+// head(k, m, w) gives entry k < 4 (k a constant after unrolling), tail(i, m,
+// w) entry i from memory; val(m, w) is the entry's term.  The entries of
+// quadruple j + 1 are fetched while the gate loads of quadruple j are in
+// flight, and the remainder's entries are fetched together (a missing one is
+// replaced by a present one whose term is then dropped).
+template <typename T, typename Head, typename Tail, typename Val>
+__device__ __forceinline__ T row_terms(int32_t n, Head head, Tail tail, Val val) {
+  T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+  const int32_t q = n >> 2;
+  uint32_t m0 = 0u, m1 = 0u, m2 = 0u, m3 = 0u;
+  float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f, w3 = 0.0f;
+  if (q > 0) {
+    head(0, m0, w0);
+    head(1, m1, w1);
+    head(2, m2, w2);
+    head(3, m3, w3);
+  }
+  for (int32_t j = 0; j < q; ++j) {
+    const T x0 = val(m0, w0), x1 = val(m1, w1), x2 = val(m2, w2), x3 = val(m3, w3);
+    if (j + 1 < q) {
+      const int32_t i = 4 * (j + 1);
+      tail(i, m0, w0);
+      tail(i + 1, m1, w1);
+      tail(i + 2, m2, w2);
+      tail(i + 3, m3, w3);
+    }
+    a0 += x0;
+    a1 += x1;
+    a2 += x2;
+    a3 += x3;
+  }
+  const int32_t r = n - 4 * q;
+  if (r > 0) {
+    if (q == 0) {
+      head(0, m0, w0);
+      head(r > 1 ? 1 : 0, m1, w1);
+      head(r > 2 ? 2 : 0, m2, w2);
+    } else {
+      const int32_t i = 4 * q;
+      tail(i, m0, w0);
+      tail(r > 1 ? i + 1 : i, m1, w1);
+      tail(r > 2 ? i + 2 : i, m2, w2);
+    }
+    const T x0 = val(m0, w0), x1 = val(m1, w1), x2 = val(m2, w2);
+    a0 += x0;
+    a0 += r > 1 ? x1 : T(0);
+    a0 += r > 2 ? x2 : T(0);
+  }
+  return (a0 + a1) + (a2 + a3);
+}
+
 // NMDA conductance of one row: sum_j w * s_j(ts - d) over CSR-by-post row p,
-// with L lanes of a group striding the row and a shuffle reduction (result
-// valid in every lane of the group).  All lanes of the group must call it.
+// with L lanes of a group striding the row (lane sl takes entries
+// r0 + sl + i*L, summed by row_terms) and a shuffle reduction (result valid
+// in every lane of the group).  All lanes of the group must call it.
 // UNI: all lanes of the wave gather copies of one local row (L == 1,
 // replicas in whole waves), so the row's columns and weights are scalar loads.
 template <typename T, bool UNI>
@@ -573,24 +764,19 @@ __device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int32_t h0
     const uint64_t r0 = ldc(net.rp_n + pl), r1 = ldc(net.rp_n + pl + 1);
     const int32_t DN = net.DN;  // h0: history slot of ts; the slot of ts - d wraps once
     const T *sh = net.shist + rep;
-    auto term = [&](uint64_t e) -> T {
-      const uint32_t m = ldc(net.meta_n + e);
+    const uint64_t e0 = r0 + (uint64_t)sl;
+    const int32_t n = e0 < r1 ? (int32_t)((r1 - e0 + (uint64_t)L - 1) / (uint64_t)L) : 0;
+    auto tail = [&](int32_t i, uint32_t &m, float &w) {
+      const uint64_t e = e0 + (uint64_t)i * (uint64_t)L;
+      m = ldc(net.meta_n + e);
+      w = ldc(net.w_n + e);
+    };
+    auto val = [&](uint32_t m, float w) -> T {
       int32_t hs = h0 - (int32_t)(m >> 24);
       hs += hs < 0 ? DN : 0;
-      return T(ldc(net.w_n + e)) * sh[(size_t)hs * net.N + (size_t)(m & 0x00FFFFFFu) * B];
+      return T(w) * sh[(size_t)hs * net.N + (size_t)(m & 0x00FFFFFFu) * B];
     };
-    // four independent partial sums keep four row entries in flight per lane
-    T a1 = T(0), a2 = T(0), a3 = T(0);
-    uint64_t e = r0 + (uint64_t)sl;
-    const uint64_t st = (uint64_t)L;
-    for (; e + 3 * st < r1; e += 4 * st) {
-      acc += term(e);
-      a1 += term(e + st);
-      a2 += term(e + 2 * st);
-      a3 += term(e + 3 * st);
-    }
-    for (; e < r1; e += st) acc += term(e);
-    acc = (acc + a1) + (a2 + a3);
+    acc = row_terms<T>(n, tail, tail, val);
     acc *= T(net.gainr[2 * B + rep]);
   }
   for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
@@ -644,10 +830,91 @@ __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
   const int sl = (int)(gt % L);
   T gn = T(0);
   if (net.has_nmda_syn) gn = gather_row<T, UNI>(net, nn, slot_of(net.tmod[1], k - 1, net.DN), L, sl);
-  const int32_t n = (sl == 0 && nn < net.N) ? (int32_t)nn : net.N;  // N: inactive lane
+  int32_t n = (sl == 0 && nn < net.N) ? (int32_t)nn : net.N;  // N: inactive lane
+  if (L > 1) {
+    // the block's blockDim/L neurons are updated by its first lanes, so the
+    // waves past them skip the update instead of running it for a few lanes
+    __shared__ T s_gn[1024];
+    if (sl == 0) s_gn[threadIdx.x / L] = gn;
+    __syncthreads();
+    const int npb = (int)blockDim.x / L;
+    const int64_t nu = (int64_t)ub * npb + threadIdx.x;
+    const bool up = (int)threadIdx.x < npb && nu < net.N;
+    n = up ? (int32_t)nu : net.N;
+    gn = up ? s_gn[threadIdx.x] : T(0);
+  }
   const int sub = ub % NSUB;
   update_neuron<T, REPLAY, UNI>(net, n, t, k, true, gn, net.lcnt + pc * NSUB,
                                 pbase[sub] + pcnt[sub], sub);
+}
+
+// NMDA row of one lane of a gather group, held across the steps of a window:
+// the first RC entries of the lane's share of the row (entries r0 + sl + i*L)
+// in registers, the rest re-read.  The sum is row_terms', as in gather_row,
+// so both paths give the same bits.
+constexpr int RC = 4;
+template <typename T>
+struct RowCache {
+  uint64_t r0, r1;
+  uint32_t m[RC];
+  float w[RC];
+  int32_t cnt;  // entries of this lane in the row
+  T gain;       // the replica's NMDA gain
+};
+
+template <typename T>
+__device__ __forceinline__ void row_cache_load(const Net<T> &net, int64_t p, int L, int sl,
+                                               RowCache<T> &c) {
+  c.cnt = 0;
+  c.r0 = c.r1 = 0;
+  c.gain = T(0);
+#pragma unroll
+  for (int i = 0; i < RC; ++i) { c.m[i] = 0u; c.w[i] = 0.0f; }
+  if (p >= net.N) return;
+  const uint32_t pl = (uint32_t)p / (uint32_t)net.B;
+  c.gain = T(net.gainr[2 * net.B + ((uint32_t)p - pl * (uint32_t)net.B)]);
+  c.r0 = ldc(net.rp_n + pl);
+  c.r1 = ldc(net.rp_n + pl + 1);
+  const uint64_t e0 = c.r0 + (uint64_t)sl;
+  c.cnt = e0 < c.r1 ? (int32_t)((c.r1 - e0 + (uint64_t)L - 1) / (uint64_t)L) : 0;
+#pragma unroll
+  for (int i = 0; i < RC; ++i) {
+    const uint64_t e = e0 + (uint64_t)i * (uint64_t)L;
+    if (e < c.r1) {
+      c.m[i] = ldc(net.meta_n + e);
+      c.w[i] = ldc(net.w_n + e);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T gather_cached(const Net<T> &net, const RowCache<T> &c, int64_t p,
+                                           int32_t h0, int L, int sl) {
+  T acc = T(0);
+  if (p < net.N) {
+    const uint32_t B = (uint32_t)net.B;
+    const uint32_t rep = (uint32_t)p - ((uint32_t)p / B) * B;
+    const int32_t DN = net.DN;
+    const T *sh = net.shist + rep;
+    auto head = [&](int k, uint32_t &m, float &w) {  // k: a constant after unrolling
+      m = k == 0 ? c.m[0] : k == 1 ? c.m[1] : k == 2 ? c.m[2] : c.m[3];
+      w = k == 0 ? c.w[0] : k == 1 ? c.w[1] : k == 2 ? c.w[2] : c.w[3];
+    };
+    auto tail = [&](int32_t i, uint32_t &m, float &w) {
+      const uint64_t e = c.r0 + (uint64_t)sl + (uint64_t)i * (uint64_t)L;
+      m = ldc(net.meta_n + e);
+      w = ldc(net.w_n + e);
+    };
+    auto val = [&](uint32_t m, float w) -> T {
+      int32_t hs = h0 - (int32_t)(m >> 24);
+      hs += hs < 0 ? DN : 0;
+      return T(w) * sh[(size_t)hs * net.N + (size_t)(m & 0x00FFFFFFu) * B];
+    };
+    acc = row_terms<T>(c.cnt, head, tail, val);
+    acc *= c.gain;
+  }
+  for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+  return acc;
 }
 
 // Temporal blocking over the smallest synaptic delay: one launch runs the W
@@ -659,34 +926,95 @@ __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
 // W steps with block barriers only and pushes its own spikes right after each
 // step.  The ring (D >= W + max delay + 1 slots) and the gate history
 // (DN >= W + max NMDA delay + 1) are deep enough that no write of the launch
-// lands in a slot some block has still to read.  Record entries carry their
-// step (rec_k); sub-list s of block b = b % NSUB grows by one atomic on
-// total[s] per block and step with spikes.
+// lands in a slot some block has still to read.  Across the window each
+// neuron's state, its NMDA row (first RC entries per lane) and its push-row
+// bounds stay in registers; the step's spikes are staged in LDS and written
+// to the record (with their step, rec_k) when the staging area fills and at
+// the end, one atomic on total[b % NSUB] per flush.
 template <typename T, bool REPLAY>
-__global__ void __launch_bounds__(1024) k_window(Net<T> net, int32_t k0, int32_t W) {
-  __shared__ uint32_t s_ids[1024];
-  __shared__ T s_amp[1024];
-  __shared__ uint32_t s_cnt, s_base;
+__global__ void __launch_bounds__(512) k_window(Net<T> net, int32_t k0, int32_t W) {
+  constexpr int WREC = 1024;
+  __shared__ uint32_t s_ids[512], s_deg[512];  // blockDim <= 512
+  __shared__ uint64_t s_r0[512];
+  __shared__ T s_sc[4][512];  // push amplitude x the replica's gain, by linear receptor
+  __shared__ uint32_t w_ids[WREC], w_k[WREC];
+  __shared__ uint32_t s_cnt, s_base, s_wn;
   const int L = net.gather_lanes;
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nn = gt / L;
   const int sl = (int)(gt % L);
-  const int32_t n = (sl == 0 && nn < net.N) ? (int32_t)nn : net.N;  // N: inactive lane
+  // update lanes: the block's first blockDim/L lanes, one neuron each (the
+  // gather groups hand their sums over through LDS), so the waves past them
+  // skip the update
+  const int npb = (int)blockDim.x / L;
+  const int64_t nu = (int64_t)blockIdx.x * npb + threadIdx.x;
+  const bool act = (int)threadIdx.x < npb && nu < net.N;
+  const int32_t n = act ? (int32_t)nu : net.N;
+  __shared__ T s_gn[512];
   const int sub = (int)(blockIdx.x % NSUB);
   const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const int wpb = (int)(blockDim.x >> 6);
   const int32_t N = net.N, B = net.B;
   const uint32_t off = net.sub_off[sub], cap = net.sub_cap[sub];
-  unsigned long long events = 0, nout = 0;
+  // registers across the window
+  const int64_t tb = *net.t0;
+  const int32_t tm0 = net.tmod[0], tm1 = net.tmod[1];
+  int32_t nl = 0, rep = 0;
+  uint32_t p0 = 0;
+  NState<T> S;
+  NConst<T> C;
+  uint64_t pr0 = 0;
+  uint32_t pdeg = 0;
+  T lg[4] = {T(0), T(0), T(0), T(0)};  // the replica's linear gains
+  const DevPop<T> *Pp = net.pops;
+  if (act) {
+    locate<false>(net.npop, B, n, nl, rep, p0);
+    Pp = net.pops + p0;
+    load_state(net, *(PopC<T> *)Pp, n, slot_of(tm1, k0 - 1, net.DN), S);
+    load_const(net, *(PopC<T> *)Pp, nl, rep, p0, C);
+    pr0 = ldc(net.rp_p + nl);
+    pdeg = (uint32_t)(ldc(net.rp_p + nl + 1) - pr0);
+#pragma unroll
+    for (int li = 0; li < 4; ++li) lg[li] = T(net.gainr[lin_rid(li) * B + rep]);
+  }
+  RowCache<T> rc;
+  if (net.has_nmda_syn) row_cache_load(net, nn, L, sl, rc);
+  if (threadIdx.x == 0) s_wn = 0u;
+  unsigned long long events = 0, nout = 0, nspk = 0;
+  auto flush = [&](uint32_t wn) {  // block-uniform call
+    if (threadIdx.x == 0) s_base = atomicAdd(net.total + sub, wn);
+    __syncthreads();
+    const uint32_t base = s_base;
+    for (uint32_t i = threadIdx.x; i < wn; i += blockDim.x) {
+      const uint32_t r = base + i;
+      if (r < cap) {
+        net.rec_ids[off + r] = w_ids[i];
+        net.rec_k[off + r] = w_k[i];
+      } else {
+        atomicOr(net.overflow, 1u);
+      }
+    }
+    __syncthreads();
+  };
   for (int32_t s = 0; s < W; ++s) {
     const int32_t k = k0 + s;
-    const int64_t t = *net.t0 + k;
+    const int64_t t = tb + k;
+    T inc[4] = {T(0), T(0), T(0), T(0)};
+    if (act) ring_take(net, n, slot_of(tm0, k - 1, net.D), inc);
     T gn = T(0);
-    if (net.has_nmda_syn) gn = gather_row<T, false>(net, nn, slot_of(net.tmod[1], k - 1, net.DN), L, sl);
-    bool spk;
-    T a_push;
-    uint32_t pid;
-    neuron_core<T, REPLAY, false>(net, n, t, k, true, gn, spk, a_push, pid);
+    if (net.has_nmda_syn) {
+      gn = gather_cached(net, rc, nn, slot_of(tm1, k - 1, net.DN), L, sl);
+      if (L > 1) {
+        if (sl == 0) s_gn[threadIdx.x / L] = gn;
+        __syncthreads();
+        gn = act ? s_gn[threadIdx.x] : T(0);
+      }
+    }
+    bool spk = false;
+    T a_push = T(0);
+    if (act)
+      neuron_core<T, REPLAY>(net, *(PopC<T> *)Pp, C, n, t, slot_of(tm1, k, net.DN), gn, inc, S, spk,
+                             a_push);
     // the step's spikes of this block: wave ballots, one LDS count
     if (threadIdx.x == 0) s_cnt = 0u;
     __syncthreads();
@@ -698,52 +1026,59 @@ __global__ void __launch_bounds__(1024) k_window(Net<T> net, int32_t k0, int32_t
     woff = __shfl(woff, leader);
     __syncthreads();
     const uint32_t cnt = s_cnt;
-    if (threadIdx.x == 0 && cnt) {
-      s_base = atomicAdd(net.total + sub, cnt);
-      atomicAdd(&net.counters[blockIdx.x & (NCTR - 1)], (unsigned long long)cnt);
+    uint32_t wn = s_wn;
+    if (cnt && wn + cnt > (uint32_t)WREC) {
+      flush(wn);
+      wn = 0u;
     }
-    __syncthreads();
     if (spk) {
       const uint32_t pos = woff + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
       s_ids[pos] = (uint32_t)n;
-      s_amp[pos] = a_push;
-      const uint32_t r = s_base + pos;
-      if (r < cap) {
-        net.rec_ids[off + r] = (uint32_t)n;
-        net.rec_k[off + r] = (uint32_t)k;
-      } else {
-        atomicOr(net.overflow, 1u);
-      }
-      const uint32_t p0 = __shfl(pid, leader);
-      const bool same = __ballot(spk && pid != p0) == 0ull;
+#pragma unroll
+      for (int li = 0; li < 4; ++li) s_sc[li][pos] = a_push * lg[li];
+      s_r0[pos] = pr0;
+      s_deg[pos] = pdeg;
+      w_ids[wn + pos] = (uint32_t)n;
+      w_k[wn + pos] = (uint32_t)k;
+      const uint32_t pid = p0 * (uint32_t)B + (uint32_t)rep;
+      const uint32_t q0 = __shfl(pid, leader);
+      const bool same = __ballot(spk && pid != q0) == 0ull;
       if (same) {
-        if (lane == leader) atomicAdd(&net.pop_cnt[p0], (unsigned long long)cntw);
+        if (lane == leader) atomicAdd(&net.pop_cnt[q0], (unsigned long long)cntw);
       } else {
         atomicAdd(&net.pop_cnt[pid], 1ull);
       }
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+      s_wn = wn + cnt;
+      nspk += cnt;
+    }
     // push of the block's spikes of step k: one wave per spike, lanes over its row
-    const int32_t s0 = slot_of(net.tmod[0], k, net.D);
+    const int32_t s0 = slot_of(tm0, k, net.D);
     for (uint32_t i = (uint32_t)wave; i < cnt; i += (uint32_t)wpb) {
       const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ids[i]);
-      const uint32_t jl = j / (uint32_t)B, rep = j - jl * (uint32_t)B;
-      const T amp = s_amp[i];
-      const T ga = amp * T(net.gainr[0 * B + rep]), gg = amp * T(net.gainr[1 * B + rep]),
-              gh = amp * T(net.gainr[3 * B + rep]), gl = amp * T(net.gainr[4 * B + rep]);
-      const uint64_t r0 = ldc(net.rp_p + jl), r1 = ldc(net.rp_p + jl + 1);
+      const uint32_t jl = j / (uint32_t)B, jr = j - jl * (uint32_t)B;
+      const uint64_t r0 = s_r0[i], r1 = r0 + s_deg[i];
+      const T ga = s_sc[0][i], gg = s_sc[1][i], gh = s_sc[2][i], gl = s_sc[3][i];
       if (lane == 0) { events += r1 - r0; nout += ldc(net.nmda_out + jl); }
       for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
         const uint32_t m = ldc(net.meta_p + e);
-        const uint32_t col = (m & 0x00FFFFFFu) * (uint32_t)B + rep;
+        const float w = ldc(net.w_p + e);
+        const uint32_t col = (m & 0x00FFFFFFu) * (uint32_t)B + jr;
         const uint32_t li = m >> 30;
         const T sc = li == 0 ? ga : li == 1 ? gg : li == 2 ? gh : gl;
         int32_t slot = s0 + (int32_t)((m >> 24) & 63u);
         slot -= slot >= net.D ? net.D : 0;
-        atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(ldc(net.w_p + e)) * sc);
+        atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(w) * sc);
       }
     }
   }
+  __syncthreads();
+  const uint32_t wn = s_wn;
+  if (wn) flush(wn);
+  if (act) store_state(net, *(PopC<T> *)Pp, n, S);
+  if (threadIdx.x == 0 && nspk) atomicAdd(&net.counters[blockIdx.x & (NCTR - 1)], nspk);
   block_add_events(net, events, nout);
 }
 

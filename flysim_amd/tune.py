@@ -35,7 +35,7 @@ def candidates(replicas: int = 1, has_nmda: bool = True, fused_ok: bool = True) 
     launches per step).  Replicas in whole waves read one CSR row per wave,
     so one lane per row is the natural choice there; single networks spread
     a row over 1-16 lanes."""
-    blocks = [128, 256, 512, 1024]
+    blocks = [128, 256, 512, 1024]  # the windowed path takes at most 512
     if not has_nmda:
         lanes = [8]
     elif replicas >= 64:
@@ -46,6 +46,8 @@ def candidates(replicas: int = 1, has_nmda: bool = True, fused_ok: bool = True) 
     paths = [(0, 64)] + ([(1, 0)] if fused_ok else []) + [(0, 0)]
     for b, L in itertools.product(blocks, lanes):
         for f, w in paths:
+            if w and b > 512:
+                continue
             out.append(dict(update_block=b, gather_lanes=L, push_block=256, steps_per_graph=100,
                             use_graph=1, fused=f, window=w))
     return out

@@ -47,11 +47,20 @@ def test_window_equals_fused_path_delayed_synapses():
     assert ref["sched"]["fused"] == 1 and len(ref["raster"]) > 100
     for sched in [dict(window=64, steps_per_graph=50), dict(window=4, steps_per_graph=7),
                   dict(window=3, steps_per_graph=50, use_graph=0, gather_lanes=1, update_block=64),
-                  dict(window=64, steps_per_graph=1000, gather_lanes=16, update_block=1024)]:
+                  dict(window=64, steps_per_graph=1000, gather_lanes=16, update_block=512),
+                  dict(window=64, steps_per_graph=100, gather_lanes=64, update_block=128)]:
         got = _run(arr, sched, 1200)
         assert got["sched"]["window"] == min(sched["window"], dmin + 1)
         assert got["sched"]["fused"] == 0
         _same(got, ref, str(sched))
+
+
+def test_window_rejects_blocks_over_512():
+    net = W.c2_network(seed=4, n=600, n_mod=4, mod_edges=4, t_end_ms=100)
+    eng = Engine(arrays_from_network(net), precision=A.FS_FP32)
+    with pytest.raises(A.FlysimError):
+        eng.set_schedule(window=64, update_block=1024)
+    assert eng.schedule["update_block"] == 256  # the previous schedule stays
 
 
 def test_window_one_step_equals_two_launch_path_zero_delay():
