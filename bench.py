@@ -191,7 +191,13 @@ def main():
     del_bytes = (8 * ctr["spikes"] + 17 * ctr["lin_pushes"] + 12 * ctr["nmda_terms"]
                  + 8 * net.n * n_launch) / n_launch
     upd_ms, del_ms = kt["update_ms"], kt["deliver_ms"]
-    if eng.fused:
+    win = eng.schedule.get("window", 0)
+    if win:
+        # W steps per launch; graphs hold whole windows (steps_per_graph a
+        # multiple of W), so a launch lasts W graph steps
+        dom, b = "k_window", (upd_bytes + del_bytes) * win
+        ms = kt["graph_step_ms"] * win if kt["graph_step_ms"] > 0 else upd_ms
+    elif eng.fused:
         # one kernel per step: its launch time is the graph time per step
         # (the eager first step of a batch also carries its dispatch latency)
         dom, b = "k_step", upd_bytes + del_bytes

@@ -113,6 +113,7 @@ EXPORTS = [
     "fs_derive_params", "fs_create", "fs_reset", "fs_set_ext_rate", "fs_set_injection",
     "fs_set_pop_params", "fs_load_ext_trains", "fs_run", "fs_step_index", "fs_get_state",
     "fs_pop_spike_counts", "fs_get_schedule", "fs_set_schedule", "fs_enable_timing",
+    "fs_record_currents",
     "fs_kernel_times", "fs_run_counters", "fs_destroy", "fs_last_error", "fs_conf_load",
     "fs_conf_free", "fs_conf_desc", "fs_conf_sizes", "fs_conf_seed", "fs_conf_ext_trains",
     "fs_conf_ref_layout", "fs_conf_write_outputs", "fs_cli_main", "fs_create_replicated",
@@ -159,6 +160,7 @@ def lib() -> C.CDLL:
             "fs_get_schedule": [C.c_void_p, P(Schedule)],
             "fs_set_schedule": [C.c_void_p, P(Schedule)],
             "fs_enable_timing": [C.c_void_p, C.c_int32],
+            "fs_record_currents": [C.c_void_p, C.c_int32],
             "fs_kernel_times": [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double),
                                 P(C.c_int64)],
             "fs_run_counters": [C.c_void_p, C.c_void_p],

@@ -65,6 +65,7 @@ struct EngineBase {
   virtual void counters(int64_t *out) = 0;
   virtual bool fused_active() const = 0;
   virtual int32_t window_active() const = 0;
+  virtual void record_currents(bool on) = 0;
   fs_schedule sched{};
   bool timing = false;
   int64_t step = 0;
@@ -87,7 +88,7 @@ struct Engine final : EngineBase {
   std::vector<uint64_t> hskey;  // [B]
   // device buffers
   DevBuf b_npop, b_pops, b_V, b_cs, b_G, b_ring, b_spre, b_xn, b_shist, b_gn, b_elin, b_enx,
-      b_ens, b_stp, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
+      b_ens, b_stp, b_irep, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
       b_total, b_rec_k, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase, b_suboff, b_subcap,
       b_pbr, b_gain, b_skey, b_tmod;
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
@@ -571,6 +572,7 @@ struct Engine final : EngineBase {
     net.ext_bits = nullptr;
     net.ext_first = 0;
     net.ext_steps = 0;
+    net.irep = nullptr;
     // default schedule (the autotuner may replace it)
     sched.update_block = 256;
     sched.gather_lanes = 8;
@@ -578,7 +580,10 @@ struct Engine final : EngineBase {
     sched.steps_per_graph = 100;
     sched.use_graph = 1;
     sched.fused = 1;
-    sched.window = 0;
+    // the windowed path when synapses have delays (several steps per
+    // launch), with graphs of whole windows
+    sched.window = win_max >= 2 ? 64 : 0;
+    if (sched.window) sched.steps_per_graph = 10 * std::min(sched.window, win_max);
     apply_schedule();
     reset(0);
   }
@@ -744,6 +749,19 @@ struct Engine final : EngineBase {
     graph_dirty = false;
   }
 
+  // Irep recording (IsynSeparatedSum output): a [5][N] buffer the update
+  // writes each non-refractory step
+  void record_currents(bool on) override {
+    if (on && !net.irep) {
+      b_irep.alloc((size_t)5 * N * sizeof(T));
+      HIPCHECK(hipMemset(b_irep.p, 0, (size_t)5 * N * sizeof(T)));
+      net.irep = b_irep.as<T>();
+    } else if (!on) {
+      net.irep = nullptr;
+    }
+    graph_dirty = true;  // the buffer is a kernel argument
+  }
+
   void reset(uint64_t seed) override {
     const size_t n = (size_t)N;
     auto fill = [&](void *p, size_t count, T v) {
@@ -762,6 +780,7 @@ struct Engine final : EngineBase {
     fill(b_elin.p, 4 * n, T(0));
     fill(b_enx.p, n, T(0));
     fill(b_ens.p, n, T(0));
+    if (net.irep) fill(net.irep, (size_t)5 * n, T(0));  // Irep = 0 (code/NetSimGnl2_BulPp.cpp:204-207)
     if (net.stp_mode) {  // VarStp (code/EqsGnl2.cpp:353-362): D = F = 1, Cap = 0, Car = 20 uM
       fill(net.sD, n, T(1));
       fill(net.sF, n, T(1));
@@ -1044,6 +1063,10 @@ struct Engine final : EngineBase {
         if (!net.stp_mode) throw std::invalid_argument("no STP / STD state: the switch is off");
         d2h(f == 17 ? net.sD : net.sF);
         break;
+      case 19: case 20: case 21: case 22: case 23:
+        if (!net.irep) throw std::invalid_argument("receptor currents are not recorded (fs_record_currents)");
+        d2h(net.irep + (size_t)(f - 19) * n);
+        break;
       default: throw std::invalid_argument("unknown state field");
     }
   }
@@ -1230,6 +1253,11 @@ int fs_set_schedule(fs_handle *h, const fs_schedule *s) {
       throw;
     }
   })
+}
+
+int fs_record_currents(fs_handle *h, int32_t on) {
+  if (!h) return fs::set_error(FS_ERR_ARG, "null handle");
+  ENGINE_CALL(h->e->record_currents(on != 0))
 }
 
 int fs_enable_timing(fs_handle *h, int32_t on) {

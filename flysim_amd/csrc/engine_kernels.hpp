@@ -106,6 +106,8 @@ struct Net {
   T *elin;              // [4][N] external linear gates
   T *enx, *ens;         // [N] external NMDA gate
   T *sD, *sF, *sCap, *sCar;  // [N] presynaptic short-term plasticity state (stp_mode != 0)
+  T *irep;              // [5][N] synaptic current by receptor of the last non-refractory
+                        // step (pA), written only when recording is on (else null)
   // push CSR (linear)
   const uint64_t *rp_p;
   const uint32_t *meta_p;   // col | delay<<24 | li<<30
@@ -391,6 +393,14 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
     const T i2 = g[2] * (P.vrev[2] - v) / (T(1) + P.mg * ex / T(3.57f));
     const T i3 = g[3] * (P.vrev[3] - v);
     const T i4 = g[4] * (P.vrev[4] - v);
+    if (net.irep) {  // Nets[].Irep (code/EqsGnl2.cpp:196-200)
+      T *ir = net.irep + n;
+      ir[0] = i0;
+      ir[(size_t)N] = i1;
+      ir[2 * (size_t)N] = i2;
+      ir[3 * (size_t)N] = i3;
+      ir[4 * (size_t)N] = i4;
+    }
     T I = i0 + i1 + i2 + i3 + i4 + isyn * T(1e3f) + P.leak_drive;
     I = T(1e-3f) * I / P.cm;
     cs &= ~(F_OVF | F_UDF);

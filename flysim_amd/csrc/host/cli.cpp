@@ -155,8 +155,26 @@ int cli_main(int argc, const char *const *argv) {
     OutputWriters out(c->cfg, c->net, r, "");
     for (const std::string &u : out.unsupported())
       std::cerr << "warning: output type " << u << " is not written by the GPU engine\n";
-    std::vector<double> vbuf(out.needs_potential() ? N : 0);
-    const int64_t batch_max = out.needs_potential() ? 1 : 1000;
+    // outputs that print per-step state run one step per fs_run
+    const unsigned need = out.needs();
+    if (need & OutputWriters::NEED_IREP) {
+      rc = fs_record_currents(h, 1);
+      if (rc != FS_OK) return rc;
+    }
+    std::vector<double> vbuf((need & OutputWriters::NEED_V) ? N : 0);
+    std::vector<double> fbuf((need & OutputWriters::NEED_FLAGS) ? N : 0);
+    std::vector<double> gbuf((need & OutputWriters::NEED_GATES) ? 3 * (size_t)N : 0);
+    std::vector<double> ibuf((need & OutputWriters::NEED_IREP) ? 5 * (size_t)N : 0);
+    StepState st;
+    if (!vbuf.empty()) st.v = vbuf.data();
+    if (!fbuf.empty()) st.nstate = fbuf.data();
+    if (!gbuf.empty()) {
+      st.spre = gbuf.data();
+      st.xn = gbuf.data() + N;
+      st.sn = gbuf.data() + 2 * (size_t)N;
+    }
+    for (int r = 0; r < 5 && !ibuf.empty(); ++r) st.irep[r] = ibuf.data() + (size_t)r * N;
+    const int64_t batch_max = need ? 1 : 1000;
     std::vector<int64_t> off;
     std::vector<uint32_t> ids;
     uint64_t s = 0;
@@ -177,13 +195,17 @@ int cli_main(int argc, const char *const *argv) {
       ids.resize((size_t)N * K);
       rc = fs_run(h, K, off.data(), ids.data(), (int64_t)ids.size());
       if (rc != FS_OK) return rc;
-      if (out.needs_potential()) {
-        rc = fs_get_state(h, 0, vbuf.data());
-        if (rc != FS_OK) return rc;
+      // fs_get_state fields: 0 V, 2 NState, 8/9/10 presynaptic gates, 19.. Irep
+      if (st.v && (rc = fs_get_state(h, 0, vbuf.data())) != FS_OK) return rc;
+      if (st.nstate && (rc = fs_get_state(h, 2, fbuf.data())) != FS_OK) return rc;
+      if (st.spre) {
+        for (int f = 0; f < 3; ++f)
+          if ((rc = fs_get_state(h, 8 + f, gbuf.data() + (size_t)f * N)) != FS_OK) return rc;
       }
+      for (int r = 0; r < 5 && st.irep[0]; ++r)
+        if ((rc = fs_get_state(h, 19 + r, ibuf.data() + (size_t)r * N)) != FS_OK) return rc;
       for (int64_t k = 0; k < K; ++k)
-        out.step(s + (uint64_t)k, ids.data() + off[k], off[k + 1] - off[k],
-                 out.needs_potential() ? vbuf.data() : nullptr);
+        out.step(s + (uint64_t)k, ids.data() + off[k], off[k + 1] - off[k], st);
       s = e;
     }
     const float evlast = (float)(unsigned)(endtime ? endtime - 1 : 0) * dt;

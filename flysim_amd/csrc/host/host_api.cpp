@@ -190,12 +190,14 @@ int fs_conf_write_outputs(fs_conf *c, int32_t repeat, int64_t n_steps, const int
   if (!c || !offsets || (!ids && n_steps > 0 && offsets[n_steps] > 0) || n_steps < 0)
     return fs::set_error(FS_ERR_ARG, "bad argument");
   fs::OutputWriters w(c->cfg, c->net, repeat, dir ? dir : "");
-  if (w.needs_potential())
-    return fs::set_error(FS_ERR_ARG, "MemPot outputs need membrane potentials (use the CLI)");
+  if (w.needs())
+    return fs::set_error(FS_ERR_ARG, "MemPot / DeltaMemPotLimit / IsynSeparatedSum / GatingVariable "
+                                     "outputs need per-step state (use the CLI)");
+  const fs::StepState none;
   for (int64_t k = 0; k < n_steps; ++k) {
     for (int64_t q = offsets[k]; q < offsets[k + 1]; ++q)
       if (ids[q] >= c->net.n_neuron) return fs::set_error(FS_ERR_ARG, "spike id out of range");
-    w.step((uint64_t)k, ids + offsets[k], offsets[k + 1] - offsets[k], nullptr);
+    w.step((uint64_t)k, ids + offsets[k], offsets[k + 1] - offsets[k], none);
   }
   return FS_OK;
   FS_TRY_END

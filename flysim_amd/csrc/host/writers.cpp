@@ -13,7 +13,8 @@ OutputWriters::OutputWriters(const Config &cfg, const BuiltNet &net, int repeat,
   one_ms_ = (unsigned)(1u / cfg.dt);  // OneMsStep=(unsigned int)1/dt (code/NetSimGnl2.cpp:272)
   if (one_ms_ == 0) one_ms_ = 1;
   for (const OutCtrl &oc : cfg.outs) {
-    if (oc.type_id != OUT_SPIKE && oc.type_id != OUT_FRATE && oc.type_id != OUT_MEMPOT) {
+    if (oc.type_id != OUT_SPIKE && oc.type_id != OUT_FRATE && oc.type_id != OUT_MEMPOT &&
+        oc.type_id != OUT_DVOVF && oc.type_id != OUT_ISYNSEP && oc.type_id != OUT_GATVAR) {
       unsupported_.push_back(oc.type);
       continue;
     }
@@ -30,11 +31,21 @@ OutputWriters::OutputWriters(const Config &cfg, const BuiltNet &net, int repeat,
       if (oc.rwin == 0 || oc.fpnt == 0) throw std::runtime_error("FiringRate window/print step must be positive");
       w.spirwin.assign(oc.pops.size() * (oc.rwin + 2), 0u);
     }
-    need_v_ |= oc.type_id == OUT_MEMPOT;
+    if (oc.type_id == OUT_MEMPOT) need_ |= NEED_V;
+    if (oc.type_id == OUT_DVOVF) need_ |= NEED_FLAGS;
+    if (oc.type_id == OUT_ISYNSEP) need_ |= NEED_IREP;
+    if (oc.type_id == OUT_GATVAR) need_ |= NEED_GATES;
+  }
+  if (need_ & NEED_GATES) {
+    gate_cnt_.assign((size_t)net.n_neuron * 5, 0u);
+    for (size_t g = 0; g < net.gate_pre.size(); ++g)
+      if (!net.gate_group[g] && net.gate_pre[g] < net.n_neuron && net.gate_rec[g] < 5)
+        gate_cnt_[(size_t)net.gate_pre[g] * 5 + net.gate_rec[g]]++;
   }
 }
 
-void OutputWriters::step(uint64_t step, const uint32_t *ids, int64_t n_ids, const double *v) {
+void OutputWriters::step(uint64_t step, const uint32_t *ids, int64_t n_ids, const StepState &st) {
+  const double *v = st.v;
   const float evtime = (float)(unsigned)step * cfg_.dt;  // EvTime=i*dt
   for (int64_t q = 0; q < n_ids; ++q) spiked_[ids[q]] = 1;
   for (W &w : ws_) {
@@ -95,6 +106,51 @@ void OutputWriters::step(uint64_t step, const uint32_t *ids, int64_t n_ids, cons
       for (uint32_t m : oc.pops)
         for (uint32_t nid = net_.pop_first[m]; nid < net_.pop_first[m] + net_.pop_n[m]; ++nid)
           o << " " << (float)v[nid] * 1e-3f;
+      w.f << o.str() << std::endl;
+    } else if (oc.type_id == OUT_DVOVF) {  // code/NetSimGnl2_info.cpp:589-622
+      if (!st.nstate) throw std::runtime_error("DeltaMemPotLimit output needs the NState flags");
+      std::ostringstream o;
+      unsigned n = 0, kk = 0;
+      for (uint32_t m : oc.pops) {
+        for (uint32_t nid = net_.pop_first[m]; nid < net_.pop_first[m] + net_.pop_n[m]; ++nid) {
+          const unsigned f = (unsigned)st.nstate[nid];
+          const unsigned id = ((oc.status & OF_ALIGN) ? n : nid) + 1;
+          if (f & 16u) {  // MEMDV_OVF
+            o << evtime * 1e-3f << " " << id << "\n";
+            ++kk;
+          } else if (f & 32u) {  // MEMDV_UDF
+            o << evtime * 1e-3f << " -" << id << "\n";
+            ++kk;
+          }
+          ++n;
+        }
+      }
+      if (kk) w.f << o.str() << std::flush;
+    } else if (oc.type_id == OUT_ISYNSEP) {  // code/NetSimGnl2_info.cpp:559-573
+      for (int r = 0; r < 5; ++r)
+        if (!st.irep[r]) throw std::runtime_error("IsynSeparatedSum output needs the receptor currents");
+      std::ostringstream o;
+      o << evtime * 1e-3f;
+      for (uint32_t m : oc.pops)
+        for (uint32_t nid = net_.pop_first[m]; nid < net_.pop_first[m] + net_.pop_n[m]; ++nid)
+          for (int r = 0; r < 5; ++r) o << " " << (float)st.irep[r][nid] * 1e-12f;
+      w.f << o.str() << std::endl;
+    } else if (oc.type_id == OUT_GATVAR) {  // code/NetSimGnl2_info.cpp:701-752
+      if (!st.spre || !st.xn || !st.sn)
+        throw std::runtime_error("GatingVariable output needs the presynaptic gates");
+      // a single synapse's gate equals its presynaptic neuron's gate (one
+      // decay per presynaptic population), printed once per synapse in the
+      // receptor order AMPA, GABA, ACh, GluCl, then NMDA x and s
+      std::ostringstream o;
+      o << evtime * 1e-3f << " ";
+      static const int lin[4] = {0, 1, 3, 4};
+      for (uint32_t m : oc.pops)
+        for (uint32_t nid = net_.pop_first[m]; nid < net_.pop_first[m] + net_.pop_n[m]; ++nid) {
+          const uint32_t *c = &gate_cnt_[(size_t)nid * 5];
+          for (int r : lin)
+            for (uint32_t q = 0; q < c[r]; ++q) o << (float)st.spre[nid] << " ";
+          for (uint32_t q = 0; q < c[2]; ++q) o << (float)st.xn[nid] << " " << (float)st.sn[nid] << " ";
+        }
       w.f << o.str() << std::endl;
     }
   }

@@ -12,7 +12,8 @@ from .workloads import Network, Population
 
 STATE_FIELDS = {"V": 0, "count": 1, "nstate": 2, "gA": 3, "gG": 4, "gN": 5, "gH": 6, "gL": 7,
                 "s_lin": 8, "x_nmda": 9, "s_nmda": 10, "extA": 11, "extG": 12, "extH": 13,
-                "extL": 14, "ext_xN": 15, "ext_sN": 16, "stp_D": 17, "stp_F": 18}
+                "extL": 14, "ext_xN": 15, "ext_sN": 16, "stp_D": 17, "stp_F": 18,
+                "IrepA": 19, "IrepG": 20, "IrepN": 21, "IrepH": 22, "IrepL": 23}
 
 
 def pop_conf_of(p: Population) -> A.PopConf:
@@ -360,6 +361,11 @@ class Engine:
         cur.update(kw)
         s = A.Schedule(**cur)
         A.check(A.lib().fs_set_schedule(self._h, C.byref(s)))
+
+    def record_currents(self, on: bool = True) -> None:
+        """Keep each neuron's per-receptor synaptic currents (state fields
+        IrepA..IrepL, pA) of its last non-refractory step."""
+        A.check(A.lib().fs_record_currents(self._h, int(on)))
 
     def enable_timing(self, on: bool = True) -> None:
         A.check(A.lib().fs_enable_timing(self._h, int(on)))

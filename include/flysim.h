@@ -190,9 +190,16 @@ int fs_step_index(fs_handle *h, int64_t *step);
  * last Spike phase by receptor A,G,N,H,L, 8 linear presynaptic gate,
  * 9 NMDA presynaptic x, 10 NMDA presynaptic s, 11..14 external gate of
  * AMPA, GABA, ACh, GluCl, 15/16 external NMDA x/s, 17/18 presynaptic STP
- * depression D / facilitation F (VarStp, with -STP or -STD only).  Written as
- * double[N]. */
+ * depression D / facilitation F (VarStp, with -STP or -STD only), 19..23
+ * synaptic current by receptor A,G,N,H,L of the last non-refractory step in
+ * pA (Nets[].Irep, with fs_record_currents on).  Written as double[N]. */
 int fs_get_state(fs_handle *h, int32_t field, double *out);
+
+/* Keep the per-receptor synaptic currents of each neuron's last
+ * non-refractory step (Nets[].Irep, code/EqsGnl2.cpp:196-200; printed by the
+ * IsynSeparatedSum output, code/NetSimGnl2_info.cpp:559-573) in a device
+ * buffer of 5 x n_neuron values; off by default.  Zeroed by fs_reset. */
+int fs_record_currents(fs_handle *h, int32_t on);
 
 /* Per-population spike counts accumulated since fs_reset (uint64[n_pop]). */
 int fs_pop_spike_counts(fs_handle *h, uint64_t *out);

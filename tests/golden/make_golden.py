@@ -398,6 +398,103 @@ def plasticity_case(store, files):
     return meta
 
 
+BLAST_POP = """
+NeuralPopulation: Blast
+N=5
+C=0.5
+Taum=20
+Threshold=-50
+RestPot=-70
+ResetPot=-55
+RefractoryPeriod=2
+Receptor:AMPA
+Tau=2
+RevPot=0
+FreqExt=4000
+MeanExtEff=300
+EndReceptor
+Receptor:GABA
+Tau=5
+RevPot=-70
+EndReceptor
+TargetPopulation: Inh
+TargetReceptor=AMPA
+MeanEff=0.7
+Connectivity=0.09
+EndTargetPopulation
+TargetPopulation: ExcA
+TargetReceptor=NMDA
+MeanEff=0.2
+Connectivity=0.02
+EndTargetPopulation
+EndNeuralPopulation
+"""
+
+OUTPUTS_PRO = """DefineMacro
+RandomSeeds: 7 11 EndRandomSeeds
+EndDefineMacro
+EventTime 150.0
+Type=EndTrial
+EndEvent
+OutControl
+FileName:Spikes.dat
+Type=Spike
+population:AllPopulation
+EndOutputFile
+FileName:dV.dat
+Type=DeltaMemPotLimit
+population:AllPopulation
+EndOutputFile
+FileName:dVal.dat
+Type=DeltaMemPotLimit
+Align=true
+population:Blast
+EndOutputFile
+FileName:Isyn.dat
+Type=IsynSeparatedSum
+population:Inh
+EndOutputFile
+FileName:Gate.dat
+Type=GatingVariable
+population:ExcA
+EndOutputFile
+FileName:Vm.dat
+Type=MemPot
+population:Inh
+EndOutputFile
+EndOutControl
+"""
+
+
+# lines kept of each output file: every n-th (1 = the whole file)
+OUTPUT_SAMPLE = {"Spikes.dat": 1, "dV.dat": 10, "dVal.dat": 10, "Isyn.dat": 30, "Gate.dat": 30,
+                 "Vm.dat": 30}
+
+
+def outputs_case(store, files):
+    """Every per-step output type on the module network plus a population
+    driven hard enough to hit the dV limit, whose one-target links take the
+    single-synapse path (PreID = its population index, 3), so that neuron 3
+    has single-synapse gates to print (150 ms, -t 1).  Keeps the lines
+    OUTPUT_SAMPLE names."""
+    d = tempfile.mkdtemp(prefix="golden_")
+    conf = MODULE_CONF + BLAST_POP
+    open(os.path.join(d, "network.conf"), "w").write(conf)
+    open(os.path.join(d, "network.pro"), "w").write(OUTPUTS_PRO)
+    run_ref(d, ["-t", "1"])
+    files["outputs_network.conf"] = conf
+    files["outputs_network.pro"] = OUTPUTS_PRO
+    meta = {}
+    for fn in ["Spikes.dat", "dV.dat", "dVal.dat", "Isyn.dat", "Gate.dat", "Vm.dat"]:
+        txt = open(os.path.join(d, fn)).read()
+        lines = txt.split("\n")
+        meta[fn] = {"lines": len(lines) - 1, "sha256": hashlib.sha256(txt.encode()).hexdigest()}
+        every = OUTPUT_SAMPLE[fn]
+        files["outputs_" + fn] = "\n".join(lines[i] for i in range(0, len(lines) - 1, every)) + "\n"
+    shutil.rmtree(d)
+    return meta
+
+
 def event_digest_lines(log_text):
     i = log_text.index("time event information:")
     j = log_text.index("output files information:")
@@ -440,12 +537,14 @@ def update_plasticity():
     files = json.load(open(os.path.join(HERE, "reference_files.json")))
     meta = json.load(open(os.path.join(HERE, "reference_meta.json")))
     meta["plasticity"] = plasticity_case(store, files)
+    meta["outputs"] = outputs_case(store, files)
     np.savez_compressed(os.path.join(HERE, "reference_runs.npz"), **store)
     with open(os.path.join(HERE, "reference_files.json"), "w") as f:
         json.dump(files, f, indent=0)
     with open(os.path.join(HERE, "reference_meta.json"), "w") as f:
         json.dump(meta, f, indent=1)
     print(json.dumps(meta["plasticity"], indent=1))
+    print(json.dumps(meta["outputs"], indent=1))
 
 
 def main():
@@ -459,6 +558,7 @@ def main():
     meta["noise"] = noise_case(store, files)
     meta["example"] = example_case(store)
     meta["plasticity"] = plasticity_case(store, files)
+    meta["outputs"] = outputs_case(store, files)
     sz = seizure_case()
     if sz:
         meta["seizure_pro"] = sz

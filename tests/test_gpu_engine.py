@@ -201,7 +201,7 @@ def test_fused_step_matches_two_kernel_path():
     out = []
     for fused in (1, 0):
         eng = Engine(arr, precision=A.FS_FP64)
-        eng.set_schedule(fused=fused, steps_per_graph=50)
+        eng.set_schedule(fused=fused, window=0, steps_per_graph=50)
         eng.reset(7)
         r = eng.raster(1200)
         out.append((r, eng.state("V"), eng.state("gN"), eng.state("gA"), eng.counters()))

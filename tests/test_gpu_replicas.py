@@ -32,12 +32,20 @@ def _nets():
     return {"delayed": delayed, "zero_delay": plain}
 
 
-@pytest.mark.parametrize("kind", ["delayed", "zero_delay"])
-def test_replicas_equal_single_runs(kind):
+# step paths: the default (windowed for delayed synapses, two launches for
+# zero delays) and the alternative (fused one launch per step; windowed W = 1)
+PATHS = {("delayed", "default"): {}, ("delayed", "alt"): {"window": 0, "fused": 1},
+         ("zero_delay", "default"): {}, ("zero_delay", "alt"): {"window": 1}}
+
+
+@pytest.mark.parametrize("kind,path", sorted(PATHS))
+def test_replicas_equal_single_runs(kind, path):
     net = _nets()[kind]
     arr = arrays_from_network(net)
     B, P0, n0 = len(KNOBS), len(net.pops), net.n
-    print("case replicas %s: %d copies of %d neurons, %d steps" % (kind, B, n0, STEPS), flush=True)
+    sched = PATHS[(kind, path)]
+    print("case replicas %s/%s: %d copies of %d neurons, %d steps" % (kind, path, B, n0, STEPS),
+          flush=True)
     singles = []
     for r, (g, eg, key) in enumerate(KNOBS):
         e = Engine(arr, precision=A.FS_FP64)
@@ -48,6 +56,8 @@ def test_replicas_equal_single_runs(kind):
         singles.append((e.raster(STEPS), e.pop_counts(), e.state("V")))
         e.close()
     eb = Engine(arr, precision=A.FS_FP64, replicas=B)
+    if sched:
+        eb.set_schedule(**sched)
     for r, (g, eg, key) in enumerate(KNOBS):
         eb.set_replica(r, gain=g, ext_gain=eg, stream_key=key)
     eb.set_ext_rate(EXT_CHANGE[0] * P0 + EXT_CHANGE[1], EXT_CHANGE[2], EXT_CHANGE[3])
@@ -55,7 +65,10 @@ def test_replicas_equal_single_runs(kind):
     rb = eb.raster(STEPS)
     counts = eb.pop_counts().reshape(B, P0)
     vb = eb.state("V").reshape(n0, B)
-    assert eb.fused == (kind == "delayed")
+    s = eb.schedule
+    expect = {("delayed", "default"): (0, 11), ("delayed", "alt"): (1, 0),
+              ("zero_delay", "default"): (0, 0), ("zero_delay", "alt"): (0, 1)}[(kind, path)]
+    assert (s["fused"], s["window"]) == expect
     for r in range(B):
         mine = rb[rb[:, 1] % B == r]
         mine = np.stack([mine[:, 0], mine[:, 1] // B], axis=1)
