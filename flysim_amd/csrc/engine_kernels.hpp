@@ -941,8 +941,11 @@ __device__ __forceinline__ T gather_cached(const Net<T> &net, const RowCache<T> 
 // bounds stay in registers; the step's spikes are staged in LDS and written
 // to the record (with their step, rec_k) when the staging area fills and at
 // the end, one atomic on total[b % NSUB] per flush.
+// (three waves per SIMD: 168 registers, so a 20k-neuron network's 625
+// blocks of 256 threads are all resident at once)
 template <typename T, bool REPLAY>
-__global__ void __launch_bounds__(512) k_window(Net<T> net, int32_t k0, int32_t W) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3)))
+k_window(Net<T> net, int32_t k0, int32_t W) {
   constexpr int WREC = 1024;
   __shared__ uint32_t s_ids[512], s_deg[512];  // blockDim <= 512
   __shared__ uint64_t s_r0[512];
