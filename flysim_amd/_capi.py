@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libflysim.so")
 
 FS_OK = 0
 FS_AMPA, FS_GABA, FS_NMDA, FS_ACH, FS_GCL = 0, 1, 2, 3, 4
+FS_GAP = 5
 FS_ACT_AMPA, FS_ACT_GABA, FS_ACT_NMDA, FS_ACT_GAP = 1, 2, 4, 8
 FS_ACT_ACH, FS_ACT_GCL, FS_ACT_INJ, FS_ACT_LIF = 32, 64, 128, 256
 FS_ACT_LTP, FS_ACT_STD, FS_ACT_STP = 8192, 16384, 32768
@@ -106,7 +107,9 @@ class RefLayout(C.Structure):
                 ("evt_time_s", C.POINTER(C.c_float)), ("evt_var1", C.POINTER(C.c_float)),
                 ("evt_var2", C.POINTER(C.c_float)), ("evt_type", C.POINTER(C.c_int32)),
                 ("evt_pop", C.POINTER(C.c_int32)), ("evt_sub", C.POINTER(C.c_int32)),
-                ("act_ctr", C.c_uint32)]
+                ("act_ctr", C.c_uint32), ("n_gap", C.c_int64),
+                ("gap_pre", C.POINTER(C.c_uint32)), ("gap_post", C.POINTER(C.c_uint32)),
+                ("gap_g", C.POINTER(C.c_float))]
 
 
 EXPORTS = [

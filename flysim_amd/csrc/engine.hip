@@ -88,7 +88,7 @@ struct Engine final : EngineBase {
   std::vector<uint64_t> hskey;  // [B]
   // device buffers
   DevBuf b_npop, b_pops, b_V, b_cs, b_G, b_ring, b_spre, b_xn, b_shist, b_gn, b_elin, b_enx,
-      b_ens, b_stp, b_irep, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
+      b_ens, b_stp, b_irep, b_rp_g, b_col_g, b_w_g, b_igap, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
       b_total, b_rec_k, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase, b_suboff, b_subcap,
       b_pbr, b_gain, b_skey, b_tmod;
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
@@ -216,7 +216,7 @@ struct Engine final : EngineBase {
     for (int32_t i = 0; i < N0; ++i)
       if (d.neuron_pop[i] >= (uint32_t)P) throw std::invalid_argument("neuron_pop out of range");
     // ---------------- synapses: classify, validate, CSR by pre / by post
-    std::vector<uint64_t> cnt_p(N0 + 1, 0), cnt_n(N0 + 1, 0);
+    std::vector<uint64_t> cnt_p(N0 + 1, 0), cnt_n(N0 + 1, 0), cnt_g(N0 + 1, 0);
     int32_t maxd_p = 0, maxd_n = 0, mind_all = 1 << 30;
     int32_t lin_mask = 0;
     // presynaptic gate parameters per population (targets' receptor params)
@@ -228,7 +228,13 @@ struct Engine final : EngineBase {
       const uint8_t r = d.receptor[s];
       const int32_t dl = d.delay ? d.delay[s] : 0;
       if (a >= (uint32_t)N0 || b >= (uint32_t)N0) throw std::invalid_argument("synapse endpoint out of range");
-      if (r > FS_GCL) throw std::invalid_argument("synapse receptor id out of range");
+      if (r > FS_GCL && r != FS_GAP) throw std::invalid_argument("synapse receptor id out of range");
+      if (r == FS_GAP) {  // electrical: both endpoints, no delay
+        if (dl != 0) throw std::invalid_argument("gap junctions take no delay");
+        cnt_g[a + 1]++;
+        cnt_g[b + 1]++;
+        continue;
+      }
       const uint32_t pp = d.neuron_pop[a];
       const fs_pop_params &q = hpop[d.neuron_pop[b]];
       mind_all = std::min(mind_all, dl);
@@ -259,18 +265,32 @@ struct Engine final : EngineBase {
               "one presynaptic linear gate per population is supported");
       }
     }
-    for (int32_t i = 0; i < N0; ++i) { cnt_p[i + 1] += cnt_p[i]; cnt_n[i + 1] += cnt_n[i]; }
+    for (int32_t i = 0; i < N0; ++i) {
+      cnt_p[i + 1] += cnt_p[i];
+      cnt_n[i + 1] += cnt_n[i];
+      cnt_g[i + 1] += cnt_g[i];
+    }
+    const int64_t gap_ends = (int64_t)cnt_g[N0];  // two per gap junction
+    std::vector<uint32_t> col_g(gap_ends);
+    std::vector<float> w_g(gap_ends);
     syn_lin = (int64_t)cnt_p[N0];
     syn_nmda = (int64_t)cnt_n[N0];
     std::vector<uint32_t> meta_p(syn_lin), meta_n(syn_nmda);
     std::vector<float> w_p(syn_lin), w_n(syn_nmda);
     {
-      std::vector<uint64_t> fp(cnt_p.begin(), cnt_p.end() - 1), fn(cnt_n.begin(), cnt_n.end() - 1);
+      std::vector<uint64_t> fp(cnt_p.begin(), cnt_p.end() - 1), fn(cnt_n.begin(), cnt_n.end() - 1),
+          fg(cnt_g.begin(), cnt_g.end() - 1);
       for (int64_t s = 0; s < d.n_syn; ++s) {
         const uint32_t a = d.pre[s], b = d.post[s];
         const uint8_t r = d.receptor[s];
         const uint32_t dl = d.delay ? d.delay[s] : 0u;
-        if (r == FS_NMDA) {
+        if (r == FS_GAP) {  // each end sums g (V_other - V_self)
+          const uint64_t e1 = fg[b]++, e2 = fg[a]++;
+          col_g[e1] = a;
+          w_g[e1] = d.w[s];
+          col_g[e2] = b;
+          w_g[e2] = d.w[s];
+        } else if (r == FS_NMDA) {
           const uint64_t e = fn[b]++;
           meta_n[e] = a | (dl << 24);
           w_n[e] = d.w[s];
@@ -287,8 +307,12 @@ struct Engine final : EngineBase {
       if (d.receptor[s] == FS_NMDA) nout[d.pre[s]]++;
     PreGates pg{pre_set, pre_set_n, pre_df, pre_dfxt, pre_dfst, pre_ef, pre_as};
     init_pops(pg);
-    win_max = d.n_syn > 0 ? std::min(mind_all + 1, 64) : 64;
-    init_dims(maxd_p, maxd_n, d.n_syn > 0 && mind_all >= 1, lin_mask, d.ext_mode);
+    const int64_t n_chem = d.n_syn - gap_ends / 2;
+    win_max = n_chem > 0 ? std::min(mind_all + 1, 64) : 64;
+    // gap currents use every neuron's potential of the step before: one
+    // step per launch, after the whole update (the two-launch path)
+    if (gap_ends) win_max = 0;
+    init_dims(maxd_p, maxd_n, n_chem > 0 && mind_all >= 1 && !gap_ends, lin_mask, d.ext_mode);
     const size_t n0 = (size_t)N0;
     b_rp_p.alloc((n0 + 1) * 8);
     HIPCHECK(hipMemcpy(b_rp_p.p, cnt_p.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
@@ -309,6 +333,21 @@ struct Engine final : EngineBase {
     b_nmda_out.alloc(n0 * 4);
     HIPCHECK(hipMemcpy(b_nmda_out.p, nout.data(), n0 * 4, hipMemcpyHostToDevice));
     finish(d.neuron_pop);
+    if (gap_ends) {
+      b_rp_g.alloc((n0 + 1) * 8);
+      HIPCHECK(hipMemcpy(b_rp_g.p, cnt_g.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
+      b_col_g.alloc(gap_ends * 4);
+      b_w_g.alloc(gap_ends * 4);
+      HIPCHECK(hipMemcpy(b_col_g.p, col_g.data(), gap_ends * 4, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(b_w_g.p, w_g.data(), gap_ends * 4, hipMemcpyHostToDevice));
+      b_igap.alloc((size_t)N * sizeof(T));
+      net.rp_g = b_rp_g.as<uint64_t>();
+      net.col_g = b_col_g.as<uint32_t>();
+      net.w_g = b_w_g.as<float>();
+      net.igap = b_igap.as<T>();
+      graph_dirty = true;
+      reset(0);
+    }
   }
 
   // Batched random E/I instances, synapse tables generated on the device
@@ -573,6 +612,10 @@ struct Engine final : EngineBase {
     net.ext_first = 0;
     net.ext_steps = 0;
     net.irep = nullptr;
+    net.rp_g = nullptr;
+    net.col_g = nullptr;
+    net.w_g = nullptr;
+    net.igap = nullptr;
     // default schedule (the autotuner may replace it)
     sched.update_block = 256;
     sched.gather_lanes = 8;
@@ -698,6 +741,10 @@ struct Engine final : EngineBase {
       else hipLaunchKernelGGL((fsk::k_update<T, false, false>), g, b, 0, stream, net, k);
     }
     HIPCHECK(hipGetLastError());
+    if (net.igap) {  // gap currents from this step's potentials, for the next step
+      hipLaunchKernelGGL((fsk::k_gap<T>), dim3((N + 255) / 256), dim3(256), 0, stream, net);
+      HIPCHECK(hipGetLastError());
+    }
     if (timed) HIPCHECK(hipEventRecord(ev[1], stream));
     if (uni_row)
       hipLaunchKernelGGL((fsk::k_deliver<T, true>), dim3(dgrid), dim3(sched.push_block), 0, stream, net, k);
@@ -781,6 +828,7 @@ struct Engine final : EngineBase {
     fill(b_enx.p, n, T(0));
     fill(b_ens.p, n, T(0));
     if (net.irep) fill(net.irep, (size_t)5 * n, T(0));  // Irep = 0 (code/NetSimGnl2_BulPp.cpp:204-207)
+    if (net.igap) fill(net.igap, n, T(0));              // Isyn = 0 (code/EqsGnl2.cpp:346)
     if (net.stp_mode) {  // VarStp (code/EqsGnl2.cpp:353-362): D = F = 1, Cap = 0, Car = 20 uM
       fill(net.sD, n, T(1));
       fill(net.sF, n, T(1));

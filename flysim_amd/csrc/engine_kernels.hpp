@@ -106,6 +106,11 @@ struct Net {
   T *elin;              // [4][N] external linear gates
   T *enx, *ens;         // [N] external NMDA gate
   T *sD, *sF, *sCap, *sCar;  // [N] presynaptic short-term plasticity state (stp_mode != 0)
+  // gap junctions (two-launch path only): both ends of each junction
+  const uint64_t *rp_g; // [N0+1]
+  const uint32_t *col_g;// other end (local neuron)
+  const float *w_g;     // conductance (nS)
+  T *igap;              // [N] gap current for the next step (null: no gap junctions)
   T *irep;              // [5][N] synaptic current by receptor of the last non-refractory
                         // step (pA), written only when recording is on (else null)
   // push CSR (linear)
@@ -361,14 +366,16 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
       g[2] = g[2] + P.gext[2] * eg * s;
     }
   }
-  T isyn = T(0);
+  // Isyn: the gap current of the previous Spike phase, plus the Gaussian
+  // injection drawn now (BgInit adds to Isyn, code/EqsGnl2.cpp:335)
+  T isyn = net.igap ? net.igap[n] : T(0);
   if (act & ACT_INJ) {
     // Box-Muller on the second Philox block (statistically the reference's
     // Gaussian injection; the reference draws from a sequential ziggurat)
     const float u1 = ((float)(rnd[6] >> 8) + 1.0f) * (1.0f / 16777216.0f);
     const float u2 = u24(rnd[7]);
     const float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
-    isyn = P.inj_mean + P.inj_std * T(z);
+    isyn = isyn + (P.inj_mean + P.inj_std * T(z));
   }
   // (3) membrane (VmEul / firing)
   auto firing = [&]() {
@@ -1187,6 +1194,22 @@ __global__ void k_batch_begin(int64_t *t0, int64_t value, uint32_t *total, uint3
     lcnt[2 * NSUB + s] = 0u;
     lbase[2 * NSUB + s] = 0u;
   }
+}
+
+// Gap-junction currents (ThdGapSyn, code/EqsGnl2.cpp:705-725) from the
+// potentials of the step just updated: each neuron sums g (V_other - V_self)
+// over both ends of its junctions; the next update adds it to Isyn.
+template <typename T>
+__global__ void __launch_bounds__(256) k_gap(Net<T> net) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= net.N) return;
+  const uint32_t B = (uint32_t)net.B;
+  const uint32_t nl = (uint32_t)n / B, rep = (uint32_t)n - nl * B;
+  const T v = net.V[n];
+  T acc = T(0);
+  for (uint64_t e = net.rp_g[nl]; e < net.rp_g[nl + 1]; ++e)
+    acc += T(net.w_g[e]) * (net.V[(size_t)net.col_g[e] * B + rep] - v);
+  net.igap[n] = acc;
 }
 
 template <typename T>

@@ -239,6 +239,13 @@ void build_network(const Config &cfg, const BuildOptions &opt, BuiltNet &out) {
     std::sort(v.begin(), v.end(), [](const Single &a, const Single &b) { return a.post < b.post; });
 
   out.gap_synapses = tpsyn[R_GAP].size() + syn1[R_GAP].size();
+  // the solver couples only single-synapse gap junctions (ThdGapSyn,
+  // code/EqsGnl2.cpp:705-725); group gap links are built but never used
+  for (const Single &s : syn1[R_GAP]) {
+    out.gap_pre.push_back(s.pre);
+    out.gap_post.push_back(s.post);
+    out.gap_g.push_back(s.g);
+  }
   out.total_synapses = 0;
   for (int r = 0; r < 6; ++r) out.total_synapses += tpsyn[r].size() + syn1[r].size();
 
@@ -298,6 +305,12 @@ void build_network(const Config &cfg, const BuildOptions &opt, BuiltNet &out) {
       out.w.push_back(s.g);
       out.rec.push_back(r);
     }
+  }
+  for (size_t q = 0; q < out.gap_pre.size(); ++q) {
+    out.pre.push_back(out.gap_pre[q]);
+    out.post.push_back(out.gap_post[q]);
+    out.w.push_back(out.gap_g[q]);
+    out.rec.push_back(R_GAP);
   }
 }
 

@@ -43,6 +43,10 @@ struct BuiltNet {
   // pull-sum rows in execution order (group rows, then single rows)
   std::vector<uint32_t> row_post, row_beg, term;
   std::vector<uint8_t> row_rec;
+  // gap junctions of the single-synapse path (SynCnd1 after its sort by
+  // PosID, code/NetSimGnl2_BulPp.cpp:683-712); PreID is the population index
+  std::vector<uint32_t> gap_pre, gap_post;
+  std::vector<float> gap_g;
   std::vector<uint32_t> rseeds;      // one per repeat
   uint64_t total_synapses = 0;       // TolSynp as the reference reports it
   uint64_t gap_synapses = 0;

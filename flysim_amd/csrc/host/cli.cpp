@@ -114,7 +114,6 @@ int cli_main(int argc, const char *const *argv) {
   if (rc != FS_OK) return rc;
   std::unique_ptr<fs_conf, int (*)(fs_conf *)> cg(c, fs_conf_free);
   for (const std::string &w : c->cfg.warnings) std::cerr << "warning: " << w << "\n";
-  if (c->net.gap_synapses) return set_error(FS_ERR_ARG, "gap junctions are not supported by the GPU engine");
   if (c->cfg.events.empty()) return set_error(FS_ERR_ARG, "the protocol has no events (no end time)");
   const Event &last = c->cfg.events.back();
   if (last.nolimit) return set_error(FS_ERR_ARG, "EventTime NoLimit is not supported");

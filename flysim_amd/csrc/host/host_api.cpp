@@ -230,6 +230,10 @@ int fs_conf_ref_layout(fs_conf *c, fs_ref_layout *o) {
   o->evt_pop = c->evt_pop.data();
   o->evt_sub = c->evt_sub.data();
   o->act_ctr = c->opt.act_ctr;
+  o->n_gap = (int64_t)n.gap_pre.size();
+  o->gap_pre = n.gap_pre.data();
+  o->gap_post = n.gap_post.data();
+  o->gap_g = n.gap_g.data();
   return FS_OK;
 }
 

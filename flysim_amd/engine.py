@@ -207,6 +207,9 @@ class Conf:
             "evt_pop": arr(L.evt_pop, L.n_evt, np.int32),
             "evt_sub": arr(L.evt_sub, L.n_evt, np.int32),
             "act_ctr": int(L.act_ctr),
+            "gap_pre": arr(L.gap_pre, L.n_gap, np.uint32),
+            "gap_post": arr(L.gap_post, L.n_gap, np.uint32),
+            "gap_g": arr(L.gap_g, L.n_gap, np.float32),
         }
 
 

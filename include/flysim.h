@@ -28,8 +28,11 @@ enum {
   FS_ERR_STATE = -6      /* call out of sequence */
 };
 
-/* receptor ids (member order of the reference's RVs, code/neuron_affi.h:648) */
-enum { FS_AMPA = 0, FS_GABA = 1, FS_NMDA = 2, FS_ACH = 3, FS_GCL = 4, FS_NREC = 5 };
+/* receptor ids (member order of the reference's RVs, code/neuron_affi.h:648);
+ * FS_GAP marks an electrical synapse (gap junction, code/EqsGnl2.cpp:705-725):
+ * symmetric, g (nS) times the voltage difference of the step before enters
+ * both neurons' synaptic current; no delay */
+enum { FS_AMPA = 0, FS_GABA = 1, FS_NMDA = 2, FS_ACH = 3, FS_GCL = 4, FS_NREC = 5, FS_GAP = 5 };
 
 /* activity bits, same values as the reference's ActCtr (code/Inc.h:253-280) */
 enum {
@@ -83,7 +86,7 @@ typedef struct fs_net_desc {
   int64_t n_syn;
   const uint32_t *pre, *post;     /* [n_syn] */
   const float *w;                 /* [n_syn] efficacy, nS */
-  const uint8_t *receptor;        /* [n_syn] FS_AMPA..FS_GCL */
+  const uint8_t *receptor;        /* [n_syn] FS_AMPA..FS_GCL, FS_GAP */
   const uint8_t *delay;           /* [n_syn] extra delay in steps, NULL = 0 */
   float dt;                       /* ms */
   int32_t precision;              /* FS_FP32 | FS_FP64 */
@@ -294,6 +297,10 @@ typedef struct fs_ref_layout {
   const float *evt_time_s, *evt_var1, *evt_var2;
   const int32_t *evt_type, *evt_pop, *evt_sub;
   uint32_t act_ctr;
+  /* single-synapse gap junctions in SynCnd1 order (sorted by post) */
+  int64_t n_gap;
+  const uint32_t *gap_pre, *gap_post;
+  const float *gap_g;
 } fs_ref_layout;
 int fs_conf_ref_layout(fs_conf *c, fs_ref_layout *out);
 

@@ -7,6 +7,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -203,6 +204,8 @@ struct Sim final : SimBase {
   std::vector<uint32_t> ns;
   std::vector<T> gst, gxt;
   std::vector<T> sD, sF, sCap, sCar;  // presynaptic STP state (VarStp)
+  std::vector<uint32_t> gap_pre, gap_post, gap_by_pre;
+  std::vector<float> gap_g;
   // per-gate copies of the postsynaptic decay constants (RPs[PosID])
   std::vector<float> gdf, gdfxt, gefst, gas;
   std::vector<uint8_t> gnm;
@@ -224,6 +227,15 @@ struct Sim final : SimBase {
     term.assign(n->term_gate, n->term_gate + (R ? n->row_beg[R] : 0));
     act0 = n->act_ctr;
     dt = n->dt;
+    gap_pre.assign(n->gap_pre, n->gap_pre + n->n_gap);
+    gap_post.assign(n->gap_post, n->gap_post + n->n_gap);
+    gap_g.assign(n->gap_g, n->gap_g + n->n_gap);
+    // PreMapCnd1: synapse indices grouped by increasing PreID, in SynCnd1
+    // order within a neuron (BulDPreNeuMap, code/Eqs.h:1168-1202)
+    gap_by_pre.resize(gap_pre.size());
+    for (size_t q = 0; q < gap_by_pre.size(); ++q) gap_by_pre[q] = (uint32_t)q;
+    std::stable_sort(gap_by_pre.begin(), gap_by_pre.end(),
+                     [&](uint32_t a, uint32_t b) { return gap_pre[a] < gap_pre[b]; });
     et.assign(n->evt_time_s, n->evt_time_s + n->n_evt);
     etype.assign(n->evt_type, n->evt_type + n->n_evt);
     epop.assign(n->evt_pop, n->evt_pop + n->n_evt);
@@ -423,6 +435,31 @@ struct Sim final : SimBase {
     }
   }
 
+  // ThdGapSyn (code/EqsGnl2.cpp:705-725): each post's current gathers
+  // g (V_pre - V_post) over its synapses (SynCnd1 order, grouped by post),
+  // then each pre's current loses g (V_pre - V_post) over its synapses
+  // (PreMapCnd1 order); the currents enter the next step's VmEul
+  inline void gap_update() {
+    const size_t S = gap_pre.size();
+    for (size_t q = 0; q < S;) {
+      const uint32_t p = gap_post[q];
+      T acc = Isyn[p];
+      const T vp = V[p];
+      for (; q < S && gap_post[q] == p; ++q) acc = acc + T(gap_g[q]) * (V[gap_pre[q]] - vp);
+      Isyn[p] = acc;
+    }
+    for (size_t q = 0; q < S;) {
+      const uint32_t a = gap_pre[gap_by_pre[q]];
+      T acc = Isyn[a];
+      const T va = V[a];
+      for (; q < S && gap_pre[gap_by_pre[q]] == a; ++q) {
+        const uint32_t s = gap_by_pre[q];
+        acc = acc - T(gap_g[s]) * (va - V[gap_post[s]]);
+      }
+      Isyn[a] = acc;
+    }
+  }
+
   // EqsGnl2::STP / STD (code/EqsGnl2.cpp:1048-1090), after all gates
   inline void stp_update(int i) {
     const NeuPar &q = par[i];
@@ -463,6 +500,7 @@ struct Sim final : SimBase {
     for (int i = 0; i < N; ++i) { bginit(i); vmeul(i); }  // Activation
     for (int k = 0; k < G; ++k) gate_update(k);             // Spike
     for (int r = 0; r < R; ++r) row_sum(r);
+    gap_update();  // unconditional in Spike (code/EqsGnl2.cpp:1125)
     if (act0 & (STP_ACT | STD_ACT))
       for (int i = 0; i < N; ++i) stp_update(i);
   }

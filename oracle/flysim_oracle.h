@@ -17,6 +17,7 @@
  *   code/EqsGnl2.cpp:404-502   ThdSynapse (single-synapse gates + pull-sum)
  *   code/EqsGnl2.cpp:866-1002  ThdPGatVar / ThdPSynapse (group gates + gather)
  *   code/EqsGnl2.cpp:505-863, 1048-1090  STP / STD gates and presynaptic state
+ *   code/EqsGnl2.cpp:705-725  ThdGapSyn (gap junction currents)
  *   code/EqsGnl2.cpp:1098-1131 Activation / Spike dispatch
  *   code/NetSimGnl2.cpp:52-141 DoEvt,  :231-330 SimOne / Run seeding
  *   code/NetSimGnl2_BulPp.cpp:159-278 per-neuron parameter derivation
@@ -72,6 +73,10 @@ typedef struct {
   const int32_t *evt_pop;
   const int32_t *evt_sub;        /* receptor keyword KW_* for ChangeExtFreq */
   const float *evt_var1, *evt_var2;
+  /* single-synapse gap junctions in SynCnd1 order (sorted by post) */
+  int32_t n_gap;
+  const uint32_t *gap_pre, *gap_post;
+  const float *gap_g;
 } or_net;
 
 typedef struct or_sim or_sim;

@@ -37,7 +37,9 @@ class OrNet(C.Structure):
                 ("act_ctr", C.c_uint32), ("dt", C.c_float), ("n_evt", C.c_int32),
                 ("evt_time_s", C.POINTER(C.c_float)), ("evt_type", C.POINTER(C.c_int32)),
                 ("evt_pop", C.POINTER(C.c_int32)), ("evt_sub", C.POINTER(C.c_int32)),
-                ("evt_var1", C.POINTER(C.c_float)), ("evt_var2", C.POINTER(C.c_float))]
+                ("evt_var1", C.POINTER(C.c_float)), ("evt_var2", C.POINTER(C.c_float)),
+                ("n_gap", C.c_int32), ("gap_pre", C.POINTER(C.c_uint32)),
+                ("gap_post", C.POINTER(C.c_uint32)), ("gap_g", C.POINTER(C.c_float))]
 
 
 _lib = None
@@ -131,6 +133,12 @@ class Oracle:
         net.evt_sub = _p(k["evt_sub"], C.c_int32)
         net.evt_var1 = _p(k["evt_var1"], C.c_float)
         net.evt_var2 = _p(k["evt_var2"], C.c_float)
+        for name, dt_ in [("gap_pre", np.uint32), ("gap_post", np.uint32), ("gap_g", np.float32)]:
+            k[name] = np.ascontiguousarray(layout.get(name, np.zeros(0)), dtype=dt_)
+        net.n_gap = int(k["gap_pre"].shape[0])
+        net.gap_pre = _p(k["gap_pre"], C.c_uint32)
+        net.gap_post = _p(k["gap_post"], C.c_uint32)
+        net.gap_g = _p(k["gap_g"], C.c_float)
         k["net"] = net
         self._h = lib().or_create(C.byref(net), precision)
 

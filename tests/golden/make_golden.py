@@ -380,6 +380,41 @@ def std_conf():
     return c.replace(a, "TargetReceptor=NMDA\nMeanEff=0.3\nConnectivity=0.02\n")
 
 
+GAP_LINKS = """TargetPopulation: Inh
+TargetReceptor=GAP
+MeanEff=2.0
+Connectivity=0.09
+EndTargetPopulation
+"""
+
+
+def gap_conf():
+    """The module network with one-target gap links (the solver couples only
+    single-synapse gap junctions; PreID is the population index): from Solo
+    (population 2, so neuron 2) and Inh (population 1, neuron 1) onto Inh
+    neurons."""
+    c = MODULE_CONF
+    a = "EndTargetPopulation\nEndNeuralPopulation\n\nNeuralPopulation: Solo"
+    assert a in c
+    c = c.replace(a, "EndTargetPopulation\n" + GAP_LINKS + "EndNeuralPopulation\n\nNeuralPopulation: Solo")
+    a = "TargetPopulation: Inh\nTargetReceptor=AMPA\nMeanEff=0.7\nConnectivity=0.1\nEndTargetPopulation\n"
+    assert a in c
+    return c.replace(a, a + GAP_LINKS.replace("MeanEff=2.0", "MeanEff=3.5"))
+
+
+def gap_case(store, files):
+    d = tempfile.mkdtemp(prefix="golden_")
+    conf = gap_conf()
+    open(os.path.join(d, "network.conf"), "w").write(conf)
+    open(os.path.join(d, "network.pro"), "w").write(MODULE_PRO)
+    out = run_ref(d, ["-t", "1"])
+    store["gap_raster"] = raster(os.path.join(d, "Spikes.dat"))
+    files["gap_network.conf"] = conf
+    meta = {"spikes": int(len(store["gap_raster"])), "total_synapses": total_synapses(out)}
+    shutil.rmtree(d)
+    return meta
+
+
 def plasticity_case(store, files):
     """-STP, -STD (seed 0: -STD sets the user-seed bit) and -udfsed N (STD on)."""
     meta = {}
@@ -538,6 +573,7 @@ def update_plasticity():
     meta = json.load(open(os.path.join(HERE, "reference_meta.json")))
     meta["plasticity"] = plasticity_case(store, files)
     meta["outputs"] = outputs_case(store, files)
+    meta["gap"] = gap_case(store, files)
     np.savez_compressed(os.path.join(HERE, "reference_runs.npz"), **store)
     with open(os.path.join(HERE, "reference_files.json"), "w") as f:
         json.dump(files, f, indent=0)
@@ -545,6 +581,7 @@ def update_plasticity():
         json.dump(meta, f, indent=1)
     print(json.dumps(meta["plasticity"], indent=1))
     print(json.dumps(meta["outputs"], indent=1))
+    print(json.dumps(meta["gap"], indent=1))
 
 
 def main():
@@ -559,6 +596,7 @@ def main():
     meta["example"] = example_case(store)
     meta["plasticity"] = plasticity_case(store, files)
     meta["outputs"] = outputs_case(store, files)
+    meta["gap"] = gap_case(store, files)
     sz = seizure_case()
     if sz:
         meta["seizure_pro"] = sz

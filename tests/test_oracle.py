@@ -84,6 +84,32 @@ def test_plasticity_switch_changes_the_raster(golden):
     assert r.shape != ref.shape or not np.array_equal(r, ref)
 
 
+def test_fp32_oracle_gap_junctions(golden):
+    """Single-synapse gap junctions (ThdGapSyn: posts gather, then pres lose
+    the same current) against the reference built from source."""
+    f = golden["files"]
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, f["gap_network.conf"], f["module_network.pro"])
+        lay = c.ref_layout()
+        assert len(lay["gap_pre"]) > 0
+        o = Oracle(lay, c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        r = o.raster(3000)
+    ref = golden["runs"]["gap_raster"].astype(np.int64)
+    assert len(ref) > 1000
+    np.testing.assert_array_equal(r, ref)
+    # without the gap links the raster differs
+    lay2 = dict(lay)
+    for k in ("gap_pre", "gap_post", "gap_g"):
+        lay2[k] = lay[k][:0]
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, f["gap_network.conf"], f["module_network.pro"])
+        o = Oracle(lay2, c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        r2 = o.raster(3000)
+    assert r2.shape != ref.shape or not np.array_equal(r2, ref)
+
+
 def test_fp32_oracle_gaussian_injection_ziggurat(golden):
     """ChangeMembraneNoise events drive the Marsaglia-Tsang ziggurat stream."""
     f = golden["files"]
