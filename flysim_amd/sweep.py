@@ -146,6 +146,55 @@ def plain_rate(points: int, seed: int) -> float:
     return sim.points_per_s
 
 
+def band_design(screens, t_fixed_s: float, plain_pts_s: float, lo: float = 45.0, hi: float = 60.0,
+                audit_frac: float = 0.01) -> Dict:
+    """The threshold-locating variant of the screen, evaluated on the data of
+    the conservative one: simulate only the points whose predicted mean rate
+    lies in the uncertain band (lo, hi] around the 50 Hz threshold, plus an
+    audit of `audit_frac` of the others, and let the surrogate classify the
+    rest (hyperactive above the band, quiet below).  Every point the
+    conservative screen flagged (predicted above the threshold minus its
+    margin) was simulated, so the surrogate's misclassifications above the
+    band are counted exactly; below it they are counted where simulated and
+    estimated from the audit elsewhere.  Cost: the fixed part (training data,
+    fit, prediction) plus the band and its audit at the measured
+    verification rate."""
+    points = int(screens[0]["points"])
+    pm = screens[0]["pred_mean_all"].astype(np.float64)
+    if pm.size != points:
+        raise ValueError("the band design needs part 0 of the screen (all predictions)")
+    flag_lo = float(np.min(pm[screens[0]["flagged"]])) if screens[0]["flagged"].size else np.inf
+    idx = np.concatenate([s["idx"] for s in screens])
+    sim = np.full(points, np.nan)
+    sim[idx] = np.concatenate([s["rates"] for s in screens]).astype(np.float64).mean(axis=1)
+    t_sim = sum(float(s["t_sim_s"]) for s in screens)
+    rate = idx.size / t_sim
+    band = (pm > lo) & (pm <= hi)
+    n_band = int(band.sum())
+    n_audit = int(round(audit_frac * (points - n_band)))
+    t_verify = (n_band + n_audit) / rate
+    above = pm > hi
+    seen_above = above & ~np.isnan(sim)
+    fp = int((sim[seen_above] <= THRESHOLD_HZ).sum())
+    below = pm <= lo
+    seen_below = below & ~np.isnan(sim)
+    below_flagged = below & (pm >= flag_lo)            # simulated in full by the conservative screen
+    below_audit = below & ~below_flagged & ~np.isnan(sim)
+    fn_seen = int((sim[below_flagged & seen_below] > THRESHOLD_HZ).sum())
+    aud_rate = float((sim[below_audit] > THRESHOLD_HZ).mean()) if below_audit.any() else 0.0
+    n_below_unflagged = int((below & ~below_flagged).sum())
+    t_total = t_fixed_s + t_verify
+    return {"band_hz": [lo, hi], "band_points": n_band, "band_frac": n_band / points,
+            "audit_points": n_audit, "verify_s": t_verify, "total_s": t_total,
+            "speedup_vs_plain": (points / plain_pts_s) / t_total,
+            "above_band": int(above.sum()), "above_band_simulated": int(seen_above.sum()),
+            "false_hyperactive_above": fp,
+            "below_band": int(below.sum()), "below_band_simulated": int(seen_below.sum()),
+            "missed_hyperactive_below_simulated": fn_seen,
+            "audit_miss_rate_below": aud_rate,
+            "est_missed_below_total": fn_seen + aud_rate * n_below_unflagged}
+
+
 def report(train_d, model, screens, plain_pts_s: float, batched_pts_s: float) -> Dict:
     points = int(screens[0]["points"])
     flagged = screens[0]["flagged"]
@@ -192,6 +241,8 @@ def report(train_d, model, screens, plain_pts_s: float, batched_pts_s: float) ->
         "speedup_vs_batched": t_batched / t_total,
         "effective_points_per_s": points / t_total,
     }
+    if screens[0]["pred_mean_all"].size == points:
+        rep["band_design"] = band_design(screens, t_gen + t_fit + t_pred, plain_pts_s)
     return rep
 
 
