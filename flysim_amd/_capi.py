@@ -37,7 +37,9 @@ class PopParams(C.Structure):
                 ("stp_pv", C.c_float), ("stp_dfD", C.c_float), ("stp_efD", C.c_float),
                 ("stp_aF", C.c_float), ("stp_dfF", C.c_float), ("stp_efF", C.c_float),
                 ("stp_aCap", C.c_float), ("stp_dfCap", C.c_float), ("stp_aCar", C.c_float),
-                ("stp_dfCar", C.c_float), ("stp_efCar", C.c_float)]
+                ("stp_dfCar", C.c_float), ("stp_efCar", C.c_float),
+                ("ltp_dfLP", C.c_float), ("ltp_pos", C.c_float), ("ltp_neg", C.c_float),
+                ("ltp_spidy", C.c_int32)]
 
 
 class PopConf(C.Structure):
@@ -46,11 +48,13 @@ class PopConf(C.Structure):
                 ("spike_delay_ms", C.c_float), ("declared", C.c_int32 * 5),
                 ("tau", C.c_float * 5), ("tau_xt", C.c_float * 5), ("vrev", C.c_float * 5),
                 ("freq_ext", C.c_float * 5), ("ext_eff", C.c_float * 5),
-                ("ext_con", C.c_float * 5), ("stp", C.c_float * 8)]
+                ("ext_con", C.c_float * 5), ("stp", C.c_float * 8), ("ltp", C.c_float * 4)]
 
 # STP_pv, STP_tD, STP_aF, STP_tF, STP_aCap, STP_tCap, STP_aCar, STP_tCar defaults
 # of the reference parser (code/parser.cpp:229-243)
 STP_DEFAULTS = (0.6, 300.0, 0.0035, 7000.0, 250.0, 2.0, 0.1, 2000.0)
+# LTP_tLP (ms), LTP_SpiLPDy (steps), LTP_PosISI, LTP_NegISI defaults (code/parser.cpp:246-253)
+LTP_DEFAULTS = (50.0, 0.0, 1e-5, 1e-5)
 
 
 class NetDesc(C.Structure):

@@ -35,6 +35,10 @@ static void to_pop_conf(const PopDecl &p, fs_pop_conf &c) {
   const float stp[8] = {p.stp.pv, p.stp.tD, p.stp.aF, p.stp.tF, p.stp.aCap, p.stp.tCap,
                         p.stp.aCar, p.stp.tCar};
   for (int k = 0; k < 8; ++k) c.stp[k] = stp[k];
+  c.ltp[0] = p.ltp.tLP;
+  c.ltp[1] = (float)p.ltp.SpiLPDy;
+  c.ltp[2] = p.ltp.PosISI;
+  c.ltp[3] = p.ltp.NegISI;
 }
 
 // Parameter derivation of code/NetSimGnl2_BulPp.cpp:159-278, in float.
@@ -84,6 +88,11 @@ static void derive(const fs_pop_conf &c, int32_t rfped, int32_t spidy, float dt,
   q.stp_aCar = c.stp[6];
   q.stp_dfCar = std::exp(-dt / c.stp[7]);
   q.stp_efCar = (1.0f - q.stp_dfCar) * 20.0f * 1e-6f;
+  // long-term plasticity (:180-183)
+  q.ltp_dfLP = std::exp(-dt / c.ltp[0]);
+  q.ltp_spidy = (int32_t)c.ltp[1];
+  q.ltp_pos = c.ltp[2];
+  q.ltp_neg = c.ltp[3];
   // MNoiseMean = MNoiseSTD = 0: injection off (:142-148)
   q.act = act & ~(uint32_t)FS_ACT_INJ;
   q.inj_mean = 0.0f;

@@ -44,6 +44,10 @@ void pop_conf_of(const PopDecl &p, float dt, fs_pop_conf &c) {
   const float stp[8] = {p.stp.pv, p.stp.tD, p.stp.aF, p.stp.tF, p.stp.aCap, p.stp.tCap,
                         p.stp.aCar, p.stp.tCar};
   for (int k = 0; k < 8; ++k) c.stp[k] = stp[k];
+  c.ltp[0] = p.ltp.tLP;
+  c.ltp[1] = (float)p.ltp.SpiLPDy;
+  c.ltp[2] = p.ltp.PosISI;
+  c.ltp[3] = p.ltp.NegISI;
 }
 
 }  // namespace fs

@@ -23,6 +23,8 @@ def pop_conf_of(p: Population) -> A.PopConf:
     c.refractory_ms, c.spike_delay_ms = p.refractory_ms, p.spike_delay_ms
     for k, v in enumerate(getattr(p, "stp", None) or A.STP_DEFAULTS):
         c.stp[k] = v
+    for k, v in enumerate(A.LTP_DEFAULTS):
+        c.ltp[k] = v
     for r in p.receptors:
         k = r.kind
         c.declared[k] = 1
@@ -123,12 +125,14 @@ class Conf:
     (parser and builder of the drop-in API)."""
 
     def __init__(self, conf: str, pro: str, dt: float = 0.1, threads: int = 1, repeats: int = 1,
-                 stp: bool = False, std: bool = False, udf_seed: Optional[int] = None):
-        """stp / std / udf_seed: the -STP, -STD and -udfsed switches (code/main.cpp:90-95);
-        -udfsed also switches STD on, and -STD alone seeds with 0, as the reference does."""
+                 stp: bool = False, std: bool = False, udf_seed: Optional[int] = None,
+                 ltp: bool = False):
+        """stp / std / ltp / udf_seed: the -STP, -STD, -LTP and -udfsed switches
+        (code/main.cpp:90-97); -udfsed also switches STD on, and -STD alone
+        seeds with 0, as the reference does."""
         o = A.CliOpts()
         o.dt, o.threads, o.repeats = dt, threads, repeats
-        o.stp, o.std_ = int(stp), int(std)
+        o.stp, o.std_, o.ltp = int(stp), int(std), int(ltp)
         if udf_seed is not None:
             o.udf_seed, o.udf_seed_value = 1, int(udf_seed)
         h = C.c_void_p()

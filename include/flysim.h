@@ -62,6 +62,10 @@ typedef struct fs_pop_params {
    * act has FS_ACT_STP or FS_ACT_STD (code/NetSimGnl2_BulPp.cpp:185-200) */
   float stp_pv, stp_dfD, stp_efD, stp_aF, stp_dfF, stp_efF, stp_aCap, stp_dfCap;
   float stp_aCar, stp_dfCar, stp_efCar;
+  /* long-term plasticity of the population's single synapses, used when act
+   * has FS_ACT_LTP (code/NetSimGnl2_BulPp.cpp:180-183) */
+  float ltp_dfLP, ltp_pos, ltp_neg;
+  int32_t ltp_spidy;      /* back-propagation delay of the LTP spike, steps */
 } fs_pop_params;
 
 /* Conf-level population values (NetConf + NeuRep, code/parser.h:9-66). */
@@ -74,6 +78,9 @@ typedef struct fs_pop_conf {
    * reference defaults 0.6, 300, 0.0035, 7000, 250, 2, 0.1, 2000
    * (code/parser.cpp:229-243) */
   float stp[8];
+  /* LTP_tLP (ms), LTP_SpiLPDy (steps), LTP_PosISI, LTP_NegISI; reference
+   * defaults 50, 0, 1e-5, 1e-5 (code/parser.cpp:246-253) */
+  float ltp[4];
 } fs_pop_conf;
 
 /* Network handed to fs_create.  Neurons may be ordered arbitrarily; synapses

@@ -19,9 +19,10 @@
 namespace {
 
 // NState bits (code/Inc.h:225-231) and ActCtr bits (code/Inc.h:253-280).
-constexpr uint32_t FIRE_F = 1, SPIKE_F = 2, MEMDV_OVF = 16, MEMDV_UDF = 32;
+constexpr uint32_t FIRE_F = 1, SPIKE_F = 2, LP_FIRE_F = 4, LP_SPIKE_F = 8, MEMDV_OVF = 16,
+                   MEMDV_UDF = 32;
 constexpr uint32_t AMPA_ACT = 1, GABA_ACT = 2, NMDA_ACT = 4, ACH_ACT = 32,
-                   GCL_ACT = 64, INJ_ACT = 128, STD_ACT = 16384, STP_ACT = 32768;
+                   GCL_ACT = 64, INJ_ACT = 128, LTP_ACT = 8192, STD_ACT = 16384, STP_ACT = 32768;
 // receptor ids of this library: A=0 G=1 N=2 H(ACh)=3 L(GluCl)=4
 constexpr int RA = 0, RG = 1, RN = 2, RH = 3, RL = 4;
 // parser keywords used by ChangeExtFreq events (code/Inc.h:35-47)
@@ -39,6 +40,9 @@ struct NeuPar {
   float taum;
   // short-term plasticity (ParStp2, code/NetSimGnl2_BulPp.cpp:185-200)
   float pv, dfD, efD, aF, dfF, efF, aCap, dfCap, aCar, dfCar, efCar;
+  // long-term plasticity (ParLtp2, code/NetSimGnl2_BulPp.cpp:180-183)
+  float dfLP, lpPos, lpNeg;
+  int32_t lpSpidy;
 };
 
 NeuPar derive(const or_pop &p, float dt) {
@@ -89,6 +93,10 @@ NeuPar derive(const or_pop &p, float dt) {
   q.aCar = p.stp[6];
   q.dfCar = std::exp(-dt / p.stp[7]);
   q.efCar = (1.0f - q.dfCar) * 20.0f * 1e-6f;  // CarBase = 20 uM (code/Inc.h:27)
+  q.dfLP = std::exp(-dt / p.ltp[0]);
+  q.lpSpidy = (int32_t)p.ltp[1];
+  q.lpPos = p.ltp[2];
+  q.lpNeg = p.ltp[3];
   return q;
 }
 
@@ -205,6 +213,10 @@ struct Sim final : SimBase {
   std::vector<T> gst, gxt;
   std::vector<T> sD, sF, sCap, sCar;  // presynaptic STP state (VarStp)
   std::vector<uint32_t> gap_pre, gap_post, gap_by_pre;
+  std::vector<uint8_t> ggroup;
+  std::vector<T> gw;           // gate weights (single synapses change under LTP)
+  std::vector<T> lp;           // NetVLTP.LP
+  std::vector<int32_t> cntlp;  // NetVLTP.CountLP
   std::vector<float> gap_g;
   // per-gate copies of the postsynaptic decay constants (RPs[PosID])
   std::vector<float> gdf, gdfxt, gefst, gas;
@@ -227,6 +239,7 @@ struct Sim final : SimBase {
     term.assign(n->term_gate, n->term_gate + (R ? n->row_beg[R] : 0));
     act0 = n->act_ctr;
     dt = n->dt;
+    ggroup.assign(n->gate_group, n->gate_group + G);
     gap_pre.assign(n->gap_pre, n->gap_pre + n->n_gap);
     gap_post.assign(n->gap_post, n->gap_post + n->n_gap);
     gap_g.assign(n->gap_g, n->gap_g + n->n_gap);
@@ -265,6 +278,9 @@ struct Sim final : SimBase {
     for (auto &v : est) v.resize(N);
     ext_xt.resize(N); ext_st.resize(N);
     sD.assign(N, T(1)); sF.assign(N, T(1)); sCap.assign(N, T(0)); sCar.assign(N, T(20));
+    lp.assign(N, T(0)); cntlp.assign(N, 0);
+    gw.resize(G);
+    for (int k = 0; k < G; ++k) gw[k] = T(gg[k]);
     gst.assign(G, T(0)); gxt.assign(G, T(0));
     gdf.resize(G); gdfxt.resize(G); gefst.resize(G); gas.resize(G); gnm.resize(G);
     for (int k = 0; k < G; ++k) {
@@ -291,7 +307,9 @@ struct Sim final : SimBase {
       // VarStp (code/EqsGnl2.cpp:353-362); under STD only D is initialised
       // and F stays 1 here (the reference leaves it unset)
       sD[i] = T(1); sF[i] = T(1); sCap[i] = T(0); sCar[i] = T(20.0f);
+      lp[i] = T(0); cntlp[i] = 0;  // VarLtp (code/EqsGnl2.cpp:365-369)
     }
+    for (int k = 0; k < G; ++k) gw[k] = T(gg[k]);
     std::fill(gst.begin(), gst.end(), T(0));
     std::fill(gxt.begin(), gxt.end(), T(0));
     rd.seed(seed);
@@ -409,7 +427,29 @@ struct Sim final : SimBase {
       else if (I < -dvm) { I = -dvm; ns[i] &= ~MEMDV_OVF; ns[i] |= MEMDV_UDF; }
       else { ns[i] &= ~MEMDV_OVF; ns[i] &= ~MEMDV_UDF; }
       V[i] = T(q.df) * V[i] + T(q.ef) * I;
-      if (V[i] > T(q.vth)) { ns[i] |= FIRE_F; firing(i); }
+      if (V[i] > T(q.vth)) {
+        ns[i] |= FIRE_F;
+        firing(i);
+        if (act0 & LTP_ACT) ns[i] |= LP_FIRE_F;
+      }
+    }
+    if (act0 & LTP_ACT) {
+      // firLP: the LTP spike follows the onset after SpiLPDy steps (:254-272)
+      if (ns[i] & LP_FIRE_F) {
+        if (cntlp[i] == q.lpSpidy) {
+          ns[i] |= LP_SPIKE_F;
+          ns[i] &= ~LP_FIRE_F;
+          cntlp[i] = 0;
+        } else {
+          ns[i] &= ~LP_SPIKE_F;
+          cntlp[i]++;
+        }
+      } else {
+        ns[i] &= ~LP_SPIKE_F;
+      }
+      // ltpEul (:274-280)
+      if (ns[i] & LP_SPIKE_F) lp[i] = lp[i] + T(1);
+      else lp[i] = T(q.dfLP) * lp[i];
     }
     for (auto &v : g) v[i] = T(0);
     Isyn[i] = T(0);
@@ -460,6 +500,25 @@ struct Sim final : SimBase {
     }
   }
 
+  // EqsGnl2::LTP (code/EqsGnl2.cpp:1005-1044): single synapses only (the
+  // SynFst1 segments and SynSlo1); PreID is what the gate's presynaptic
+  // index holds (the population index on this path)
+  inline void ltp_update() {
+    for (int k = 0; k < G; ++k) {
+      if (ggroup[k]) continue;
+      const uint32_t a = gpre[k], b = gpost0[k];
+      const NeuPar &qa = par[a];
+      if (ns[a] & SPIKE_F) {
+        if (qa.lpNeg > 0.0f) gw[k] = gw[k] + lp[b] * T(qa.lpNeg);
+        else gw[k] = gw[k] * (T(1) + lp[b] * T(qa.lpNeg));
+      }
+      if (ns[b] & SPIKE_F) {
+        if (qa.lpPos > 0.0f) gw[k] = gw[k] + lp[a] * T(qa.lpPos);
+        else gw[k] = gw[k] * (T(1) + lp[a] * T(qa.lpPos));
+      }
+    }
+  }
+
   // EqsGnl2::STP / STD (code/EqsGnl2.cpp:1048-1090), after all gates
   inline void stp_update(int i) {
     const NeuPar &q = par[i];
@@ -489,7 +548,7 @@ struct Sim final : SimBase {
     T acc = g[rec][p];
     for (uint32_t t = rbeg[r]; t < rbeg[r + 1]; ++t) {
       const uint32_t k = term[t];
-      acc = acc + gst[k] * T(gg[k]);
+      acc = acc + gst[k] * gw[k];
     }
     g[rec][p] = acc;
   }
@@ -503,6 +562,7 @@ struct Sim final : SimBase {
     gap_update();  // unconditional in Spike (code/EqsGnl2.cpp:1125)
     if (act0 & (STP_ACT | STD_ACT))
       for (int i = 0; i < N; ++i) stp_update(i);
+    if (act0 & LTP_ACT) ltp_update();
   }
 
   int64_t run(int64_t n, int64_t *ss, int32_t *sn, int64_t cap) override {

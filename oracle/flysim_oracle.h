@@ -18,6 +18,8 @@
  *   code/EqsGnl2.cpp:866-1002  ThdPGatVar / ThdPSynapse (group gates + gather)
  *   code/EqsGnl2.cpp:505-863, 1048-1090  STP / STD gates and presynaptic state
  *   code/EqsGnl2.cpp:705-725  ThdGapSyn (gap junction currents)
+ *   code/EqsGnl2.cpp:200-216, 254-280, 1005-1044  LTP (traces, delayed LTP spike,
+ *                              single-synapse weight rules)
  *   code/EqsGnl2.cpp:1098-1131 Activation / Spike dispatch
  *   code/NetSimGnl2.cpp:52-141 DoEvt,  :231-330 SimOne / Run seeding
  *   code/NetSimGnl2_BulPp.cpp:159-278 per-neuron parameter derivation
@@ -46,6 +48,8 @@ typedef struct {
   float tau[5], tau_xt[5], vrev[5], freq_ext[5], ext_eff[5], ext_con[5];
   /* STP_pv, STP_tD, STP_aF, STP_tF, STP_aCap, STP_tCap, STP_aCar, STP_tCar */
   float stp[8];
+  /* LTP_tLP (ms), LTP_SpiLPDy (steps), LTP_PosISI, LTP_NegISI */
+  float ltp[4];
 } or_pop;
 
 typedef struct {
@@ -73,6 +77,7 @@ typedef struct {
   const int32_t *evt_pop;
   const int32_t *evt_sub;        /* receptor keyword KW_* for ChangeExtFreq */
   const float *evt_var1, *evt_var2;
+  const uint8_t *gate_group;     /* [n_gate] 1: group gate, 0: single synapse (LTP learns these) */
   /* single-synapse gap junctions in SynCnd1 order (sorted by post) */
   int32_t n_gap;
   const uint32_t *gap_pre, *gap_post;

@@ -23,7 +23,7 @@ class OrPop(C.Structure):
                 ("vreset", C.c_float), ("rfped", C.c_int32), ("spidy", C.c_int32),
                 ("declared", C.c_int32 * 5), ("tau", C.c_float * 5), ("tau_xt", C.c_float * 5),
                 ("vrev", C.c_float * 5), ("freq_ext", C.c_float * 5), ("ext_eff", C.c_float * 5),
-                ("ext_con", C.c_float * 5), ("stp", C.c_float * 8)]
+                ("ext_con", C.c_float * 5), ("stp", C.c_float * 8), ("ltp", C.c_float * 4)]
 
 
 class OrNet(C.Structure):
@@ -38,7 +38,7 @@ class OrNet(C.Structure):
                 ("evt_time_s", C.POINTER(C.c_float)), ("evt_type", C.POINTER(C.c_int32)),
                 ("evt_pop", C.POINTER(C.c_int32)), ("evt_sub", C.POINTER(C.c_int32)),
                 ("evt_var1", C.POINTER(C.c_float)), ("evt_var2", C.POINTER(C.c_float)),
-                ("n_gap", C.c_int32), ("gap_pre", C.POINTER(C.c_uint32)),
+                ("gate_group", C.POINTER(C.c_uint8)), ("n_gap", C.c_int32), ("gap_pre", C.POINTER(C.c_uint32)),
                 ("gap_post", C.POINTER(C.c_uint32)), ("gap_g", C.POINTER(C.c_float))]
 
 
@@ -98,6 +98,8 @@ class Oracle:
             o.spidy = int(round(pc.spike_delay_ms / dt)) if "spidy" not in layout else int(layout["spidy"][i])
             for r in range(8):
                 o.stp[r] = pc.stp[r]
+            for r in range(4):
+                o.ltp[r] = pc.ltp[r]
             for r in range(5):
                 o.declared[r] = pc.declared[r]
                 o.tau[r], o.tau_xt[r], o.vrev[r] = pc.tau[r], pc.tau_xt[r], pc.vrev[r]
@@ -133,6 +135,9 @@ class Oracle:
         net.evt_sub = _p(k["evt_sub"], C.c_int32)
         net.evt_var1 = _p(k["evt_var1"], C.c_float)
         net.evt_var2 = _p(k["evt_var2"], C.c_float)
+        k["gate_group"] = np.ascontiguousarray(
+            layout.get("gate_group", np.ones(k["gate_pre"].shape[0])), dtype=np.uint8)
+        net.gate_group = _p(k["gate_group"], C.c_uint8)
         for name, dt_ in [("gap_pre", np.uint32), ("gap_post", np.uint32), ("gap_g", np.float32)]:
             k[name] = np.ascontiguousarray(layout.get(name, np.zeros(0)), dtype=dt_)
         net.n_gap = int(k["gap_pre"].shape[0])

@@ -402,6 +402,45 @@ def gap_conf():
     return c.replace(a, a + GAP_LINKS.replace("MeanEff=2.0", "MeanEff=3.5"))
 
 
+LTP_KEYS = """LTP_tLP=30
+LTP_SpiLPDy=0.2
+LTP_PosISI=0.2
+LTP_NegISI=-0.05
+"""
+
+
+def ltp_conf():
+    """The dV-limit network (module + Blast, whose one-target links take the
+    single-synapse path with PreID 3) plus one-target ExcA -> Solo AMPA links
+    (PreID 0), with LTP parameters on ExcA (the populations of neurons 0 and
+    3, whose parameters the single synapses use) and Inh."""
+    c = MODULE_CONF + BLAST_POP
+    c = c.replace("SpikeDly=0.5\n", "SpikeDly=0.5\n" + LTP_KEYS, 1)
+    a = "TargetPopulation: Solo\nTargetReceptor=AMPA\nMeanEff=0.6\nConnectivity=0.05\nEndTargetPopulation\n"
+    assert a in c
+    c = c.replace(a, a + "TargetPopulation: Solo\nTargetReceptor=AMPA\nMeanEff=2.5\nConnectivity=0.045\n"
+                  "EndTargetPopulation\n")
+    a = "NeuralPopulation: Inh\nN=10\n"
+    assert a in c
+    return c.replace(a, a + LTP_KEYS.replace("PosISI=0.2", "PosISI=0.1"))
+
+
+def ltp_case(store, files):
+    d = tempfile.mkdtemp(prefix="golden_")
+    conf = ltp_conf()
+    open(os.path.join(d, "network.conf"), "w").write(conf)
+    open(os.path.join(d, "network.pro"), "w").write(MODULE_PRO)
+    out = run_ref(d, ["-t", "1", "-LTP"])
+    store["ltp_raster"] = raster(os.path.join(d, "Spikes.dat"))
+    files["ltp_network.conf"] = conf
+    out0 = run_ref(d, ["-t", "1"])
+    r0 = raster(os.path.join(d, "Spikes.dat"))
+    meta = {"spikes": int(len(store["ltp_raster"])), "spikes_without_ltp": int(len(r0)),
+            "total_synapses": total_synapses(out)}
+    shutil.rmtree(d)
+    return meta
+
+
 def gap_case(store, files):
     d = tempfile.mkdtemp(prefix="golden_")
     conf = gap_conf()
@@ -574,6 +613,7 @@ def update_plasticity():
     meta["plasticity"] = plasticity_case(store, files)
     meta["outputs"] = outputs_case(store, files)
     meta["gap"] = gap_case(store, files)
+    meta["ltp"] = ltp_case(store, files)
     np.savez_compressed(os.path.join(HERE, "reference_runs.npz"), **store)
     with open(os.path.join(HERE, "reference_files.json"), "w") as f:
         json.dump(files, f, indent=0)
@@ -582,6 +622,7 @@ def update_plasticity():
     print(json.dumps(meta["plasticity"], indent=1))
     print(json.dumps(meta["outputs"], indent=1))
     print(json.dumps(meta["gap"], indent=1))
+    print(json.dumps(meta["ltp"], indent=1))
 
 
 def main():
@@ -597,6 +638,7 @@ def main():
     meta["plasticity"] = plasticity_case(store, files)
     meta["outputs"] = outputs_case(store, files)
     meta["gap"] = gap_case(store, files)
+    meta["ltp"] = ltp_case(store, files)
     sz = seizure_case()
     if sz:
         meta["seizure_pro"] = sz

@@ -110,6 +110,26 @@ def test_fp32_oracle_gap_junctions(golden):
     assert r2.shape != ref.shape or not np.array_equal(r2, ref)
 
 
+def test_fp32_oracle_long_term_plasticity(golden):
+    """-LTP: delayed LTP spikes, LP traces and the additive / multiplicative
+    weight rules of single synapses, against the reference built from
+    source; without the switch the raster differs."""
+    f = golden["files"]
+    rasters = []
+    for ltp in (True, False):
+        with tempfile.TemporaryDirectory() as d:
+            c = _conf_from_text(d, f["ltp_network.conf"], f["module_network.pro"], ltp=ltp)
+            lay = c.ref_layout()
+            assert (lay["gate_group"] == 0).sum() > 20
+            o = Oracle(lay, c.n_neuron, precision=0)
+            o.reset(c.seed(0))
+            rasters.append(o.raster(3000))
+    ref = golden["runs"]["ltp_raster"].astype(np.int64)
+    assert len(ref) > 1000
+    np.testing.assert_array_equal(rasters[0], ref)
+    assert rasters[1].shape != ref.shape or not np.array_equal(rasters[1], ref)
+
+
 def test_fp32_oracle_gaussian_injection_ziggurat(golden):
     """ChangeMembraneNoise events drive the Marsaglia-Tsang ziggurat stream."""
     f = golden["files"]
