@@ -63,6 +63,7 @@ class NetDesc(C.Structure):
                 ("n_syn", C.c_int64), ("pre", C.POINTER(C.c_uint32)),
                 ("post", C.POINTER(C.c_uint32)), ("w", C.POINTER(C.c_float)),
                 ("receptor", C.POINTER(C.c_uint8)), ("delay", C.POINTER(C.c_uint8)),
+                ("plastic", C.POINTER(C.c_uint8)),
                 ("dt", C.c_float), ("precision", C.c_int32), ("ext_mode", C.c_int32),
                 ("record_spikes", C.c_int32)]
 

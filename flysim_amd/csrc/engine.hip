@@ -88,7 +88,8 @@ struct Engine final : EngineBase {
   std::vector<uint64_t> hskey;  // [B]
   // device buffers
   DevBuf b_npop, b_pops, b_V, b_cs, b_G, b_ring, b_spre, b_xn, b_shist, b_gn, b_elin, b_enx,
-      b_ens, b_stp, b_irep, b_rp_g, b_col_g, b_w_g, b_igap, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
+      b_ens, b_stp, b_irep, b_rp_g, b_col_g, b_w_g, b_igap, b_lp, b_cntlp, b_lpre, b_lpost, b_lrec,
+      b_lw, b_lw0, b_gl, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
       b_total, b_rec_k, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase, b_suboff, b_subcap,
       b_pbr, b_gain, b_skey, b_tmod;
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
@@ -142,6 +143,10 @@ struct Engine final : EngineBase {
     else d.efas_n = T(p.efst_n) * T(p.as_n);
     d.inj_mean = T(p.inj_mean);
     d.inj_std = T(p.inj_std);
+    d.ltp_dfLP = T(p.ltp_dfLP);
+    d.ltp_pos = T(p.ltp_pos);
+    d.ltp_neg = T(p.ltp_neg);
+    d.ltp_spidy = p.ltp_spidy;
     d.stp_pv = T(p.stp_pv);
     d.stp_dfD = T(p.stp_dfD);
     d.stp_efD = T(p.stp_efD);
@@ -217,6 +222,16 @@ struct Engine final : EngineBase {
       if (d.neuron_pop[i] >= (uint32_t)P) throw std::invalid_argument("neuron_pop out of range");
     // ---------------- synapses: classify, validate, CSR by pre / by post
     std::vector<uint64_t> cnt_p(N0 + 1, 0), cnt_n(N0 + 1, 0), cnt_g(N0 + 1, 0);
+    // LTP: the populations' switch (global in the reference, -LTP); the
+    // plastic synapses then leave the push / gather tables for the learning list
+    int32_t ltp = 0;
+    for (int32_t i = 0; i < P; ++i) ltp |= (hpop[i].act & FS_ACT_LTP) ? 1 : 0;
+    for (int32_t i = 0; i < P; ++i)
+      if (((hpop[i].act & FS_ACT_LTP) ? 1 : 0) != ltp)
+        throw std::invalid_argument("populations disagree on the LTP switch");
+    if (ltp && B > 1) throw std::invalid_argument("LTP is not supported with replicas");
+    auto learns = [&](int64_t s) { return ltp && d.plastic && d.plastic[s] && d.receptor[s] != FS_GAP; };
+    int64_t n_learn = 0;
     int32_t maxd_p = 0, maxd_n = 0, mind_all = 1 << 30;
     int32_t lin_mask = 0;
     // presynaptic gate parameters per population (targets' receptor params)
@@ -237,11 +252,14 @@ struct Engine final : EngineBase {
       }
       const uint32_t pp = d.neuron_pop[a];
       const fs_pop_params &q = hpop[d.neuron_pop[b]];
+      const bool lr = learns(s);
+      if (lr && dl != 0) throw std::invalid_argument("plastic synapses take no delay");
+      n_learn += lr ? 1 : 0;
       mind_all = std::min(mind_all, dl);
       if (r == FS_NMDA) {
         if (dl > 255) throw std::invalid_argument("NMDA delay above 255 steps");
         maxd_n = std::max(maxd_n, dl);
-        cnt_n[b + 1]++;
+        if (!lr) cnt_n[b + 1]++;
         if (!pre_set_n[pp]) {
           pre_set_n[pp] = 1;
           pre_dfxt[pp] = q.dfxt_n; pre_dfst[pp] = q.dfst[FS_NMDA];
@@ -255,7 +273,7 @@ struct Engine final : EngineBase {
       } else {
         if (dl > 63) throw std::invalid_argument("linear-receptor delay above 63 steps");
         maxd_p = std::max(maxd_p, dl);
-        cnt_p[a + 1]++;
+        if (!lr) cnt_p[a + 1]++;
         const int li = r < 2 ? r : r - 1;
         lin_mask |= 1 << li;
         if (!pre_set[pp]) { pre_set[pp] = 1; pre_df[pp] = q.dfst[r]; }
@@ -273,6 +291,10 @@ struct Engine final : EngineBase {
     const int64_t gap_ends = (int64_t)cnt_g[N0];  // two per gap junction
     std::vector<uint32_t> col_g(gap_ends);
     std::vector<float> w_g(gap_ends);
+    std::vector<uint32_t> l_pre, l_post;
+    std::vector<uint8_t> l_rec;
+    std::vector<T> l_w;
+    l_pre.reserve(n_learn); l_post.reserve(n_learn); l_rec.reserve(n_learn); l_w.reserve(n_learn);
     syn_lin = (int64_t)cnt_p[N0];
     syn_nmda = (int64_t)cnt_n[N0];
     std::vector<uint32_t> meta_p(syn_lin), meta_n(syn_nmda);
@@ -284,7 +306,12 @@ struct Engine final : EngineBase {
         const uint32_t a = d.pre[s], b = d.post[s];
         const uint8_t r = d.receptor[s];
         const uint32_t dl = d.delay ? d.delay[s] : 0u;
-        if (r == FS_GAP) {  // each end sums g (V_other - V_self)
+        if (learns(s)) {
+          l_pre.push_back(a);
+          l_post.push_back(b);
+          l_rec.push_back(r);
+          l_w.push_back(T(d.w[s]));
+        } else if (r == FS_GAP) {  // each end sums g (V_other - V_self)
           const uint64_t e1 = fg[b]++, e2 = fg[a]++;
           col_g[e1] = a;
           w_g[e1] = d.w[s];
@@ -304,15 +331,18 @@ struct Engine final : EngineBase {
     }
     std::vector<uint32_t> nout(N0, 0u);
     for (int64_t s = 0; s < d.n_syn; ++s)
-      if (d.receptor[s] == FS_NMDA) nout[d.pre[s]]++;
+      if (d.receptor[s] == FS_NMDA && !learns(s)) nout[d.pre[s]]++;
     PreGates pg{pre_set, pre_set_n, pre_df, pre_dfxt, pre_dfst, pre_ef, pre_as};
     init_pops(pg);
     const int64_t n_chem = d.n_syn - gap_ends / 2;
     win_max = n_chem > 0 ? std::min(mind_all + 1, 64) : 64;
     // gap currents use every neuron's potential of the step before: one
     // step per launch, after the whole update (the two-launch path)
-    if (gap_ends) win_max = 0;
-    init_dims(maxd_p, maxd_n, n_chem > 0 && mind_all >= 1 && !gap_ends, lin_mask, d.ext_mode);
+    if (gap_ends || ltp) win_max = 0;
+    init_dims(maxd_p, maxd_n, n_chem > 0 && mind_all >= 1 && !gap_ends && !ltp, lin_mask, d.ext_mode);
+    // the NMDA gather tables may be empty while plastic NMDA synapses still
+    // need the presynaptic gate history
+    net.has_nmda_syn = syn_nmda > 0 || std::count(l_rec.begin(), l_rec.end(), (uint8_t)FS_NMDA) > 0;
     const size_t n0 = (size_t)N0;
     b_rp_p.alloc((n0 + 1) * 8);
     HIPCHECK(hipMemcpy(b_rp_p.p, cnt_p.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
@@ -333,6 +363,34 @@ struct Engine final : EngineBase {
     b_nmda_out.alloc(n0 * 4);
     HIPCHECK(hipMemcpy(b_nmda_out.p, nout.data(), n0 * 4, hipMemcpyHostToDevice));
     finish(d.neuron_pop);
+    if (ltp) {
+      net.ltp_mode = 1;
+      b_lp.alloc((size_t)N * sizeof(T));
+      b_cntlp.alloc((size_t)N * 4);
+      b_gl.alloc((size_t)5 * N * sizeof(T));
+      const size_t L = l_pre.size();
+      b_lpre.alloc(std::max<size_t>(L, 1) * 4);
+      b_lpost.alloc(std::max<size_t>(L, 1) * 4);
+      b_lrec.alloc(std::max<size_t>(L, 1));
+      b_lw.alloc(std::max<size_t>(L, 1) * sizeof(T));
+      b_lw0.alloc(std::max<size_t>(L, 1) * sizeof(T));
+      if (L) {
+        HIPCHECK(hipMemcpy(b_lpre.p, l_pre.data(), L * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(b_lpost.p, l_post.data(), L * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(b_lrec.p, l_rec.data(), L, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(b_lw0.p, l_w.data(), L * sizeof(T), hipMemcpyHostToDevice));
+      }
+      net.lp = b_lp.as<T>();
+      net.cntlp = b_cntlp.as<uint32_t>();
+      net.gl = b_gl.as<T>();
+      net.n_learn = (int64_t)L;
+      net.l_pre = b_lpre.as<uint32_t>();
+      net.l_post = b_lpost.as<uint32_t>();
+      net.l_rec = b_lrec.as<uint8_t>();
+      net.l_w = b_lw.as<T>();
+      graph_dirty = true;
+      reset(0);
+    }
     if (gap_ends) {
       b_rp_g.alloc((n0 + 1) * 8);
       HIPCHECK(hipMemcpy(b_rp_g.p, cnt_g.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
@@ -616,6 +674,14 @@ struct Engine final : EngineBase {
     net.col_g = nullptr;
     net.w_g = nullptr;
     net.igap = nullptr;
+    net.ltp_mode = 0;
+    net.lp = nullptr;
+    net.cntlp = nullptr;
+    net.gl = nullptr;
+    net.n_learn = 0;
+    net.l_pre = net.l_post = nullptr;
+    net.l_rec = nullptr;
+    net.l_w = nullptr;
     // default schedule (the autotuner may replace it)
     sched.update_block = 256;
     sched.gather_lanes = 8;
@@ -745,6 +811,11 @@ struct Engine final : EngineBase {
       hipLaunchKernelGGL((fsk::k_gap<T>), dim3((N + 255) / 256), dim3(256), 0, stream, net);
       HIPCHECK(hipGetLastError());
     }
+    if (net.n_learn) {  // single synapses: sums with the old weights, then the LTP rules
+      hipLaunchKernelGGL((fsk::k_learn<T>), dim3((unsigned)((net.n_learn + 255) / 256)), dim3(256), 0,
+                         stream, net, k);
+      HIPCHECK(hipGetLastError());
+    }
     if (timed) HIPCHECK(hipEventRecord(ev[1], stream));
     if (uni_row)
       hipLaunchKernelGGL((fsk::k_deliver<T, true>), dim3(dgrid), dim3(sched.push_block), 0, stream, net, k);
@@ -829,6 +900,14 @@ struct Engine final : EngineBase {
     fill(b_ens.p, n, T(0));
     if (net.irep) fill(net.irep, (size_t)5 * n, T(0));  // Irep = 0 (code/NetSimGnl2_BulPp.cpp:204-207)
     if (net.igap) fill(net.igap, n, T(0));              // Isyn = 0 (code/EqsGnl2.cpp:346)
+    if (net.ltp_mode) {  // VarLtp (code/EqsGnl2.cpp:365-369); weights back to their initial values
+      fill(net.lp, n, T(0));
+      fill(net.gl, (size_t)5 * n, T(0));
+      HIPCHECK(hipMemsetAsync(net.cntlp, 0, n * 4, stream));
+      if (net.n_learn)
+        HIPCHECK(hipMemcpyAsync(net.l_w, b_lw0.p, (size_t)net.n_learn * sizeof(T),
+                                hipMemcpyDeviceToDevice, stream));
+    }
     if (net.stp_mode) {  // VarStp (code/EqsGnl2.cpp:353-362): D = F = 1, Cap = 0, Car = 20 uM
       fill(net.sD, n, T(1));
       fill(net.sF, n, T(1));
@@ -1071,6 +1150,8 @@ struct Engine final : EngineBase {
             uint32_t s = 0;  // reference NState bits (code/Inc.h:225-231)
             if (c[i] & fsk::F_FIRE) s |= 1;
             if (c[i] & fsk::F_SPIKE) s |= 2;
+            if (c[i] & fsk::F_LPFIRE) s |= 4;
+            if (c[i] & fsk::F_LPSPIKE) s |= 8;
             if (c[i] & fsk::F_OVF) s |= 16;
             if (c[i] & fsk::F_UDF) s |= 32;
             out[i] = (double)s;

@@ -42,7 +42,8 @@ constexpr int NSUB = 64;
 
 // state word: refractory count in bits 0..23, flags above
 constexpr uint32_t CNT_MASK = 0x00FFFFFFu;
-constexpr uint32_t F_FIRE = 1u << 24, F_SPIKE = 1u << 25, F_OVF = 1u << 28, F_UDF = 1u << 29;
+constexpr uint32_t F_FIRE = 1u << 24, F_SPIKE = 1u << 25, F_LPFIRE = 1u << 26, F_LPSPIKE = 1u << 27,
+                   F_OVF = 1u << 28, F_UDF = 1u << 29;
 constexpr uint32_t ACT_AMPA = 1, ACT_GABA = 2, ACT_NMDA = 4, ACT_ACH = 32, ACT_GCL = 64,
                    ACT_INJ = 128;
 constexpr int32_t STP_STD = 1, STP_FULL = 2;
@@ -63,6 +64,8 @@ struct DevPop {
   T pre_df;             // presynaptic linear gate decay (targets' dfst)
   T pre_dfxt, pre_dfst, pre_efas;  // presynaptic NMDA gate (targets' params)
   T inj_mean, inj_std;
+  T ltp_dfLP, ltp_pos, ltp_neg;  // long-term plasticity (single synapses)
+  int32_t ltp_spidy;
   // short-term plasticity of the population's outgoing synapses
   T stp_pv, stp_dfD, stp_efD, stp_aF, stp_dfF, stp_efF, stp_aCap, stp_dfCap, stp_aCar,
       stp_dfCar, stp_efCar;
@@ -85,6 +88,7 @@ struct Net {
   int32_t has_nmda_syn;
   int32_t ext_mode;     // 0 philox, 1 replay
   int32_t stp_mode;     // 0 none, STP_STD depression only, STP_FULL depression + facilitation
+  int32_t ltp_mode;     // 1: LTP traces and learning synapses (two-launch path)
   int32_t words;        // ceil(N/32)
   int32_t B;            // replicas (interleaved: n = local * B + replica)
   int32_t N0;           // neurons of one replica (N = N0 * B)
@@ -111,6 +115,15 @@ struct Net {
   const uint32_t *col_g;// other end (local neuron)
   const float *w_g;     // conductance (nS)
   T *igap;              // [N] gap current for the next step (null: no gap junctions)
+  // long-term plasticity (ltp_mode): per-neuron LP trace and LTP-spike delay
+  // counter, and the learning synapses (the reference's single-synapse path)
+  T *lp;
+  uint32_t *cntlp;
+  int64_t n_learn;
+  const uint32_t *l_pre, *l_post;
+  const uint8_t *l_rec;
+  T *l_w;               // current weights (nS)
+  T *gl;                // [5][N] the learning synapses' conductance of the last Spike phase
   T *irep;              // [5][N] synaptic current by receptor of the last non-refractory
                         // step (pA), written only when recording is on (else null)
   // push CSR (linear)
@@ -319,6 +332,14 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
     g[rid] = gv;
   }
   if (net.has_nmda_syn) g[2] = gn;
+  if (net.ltp_mode) {  // single synapses, summed by k_learn with the weights before its update
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+      T *gp = net.gl + (size_t)r * N + n;
+      g[r] = g[r] + *gp;
+      *gp = T(0);
+    }
+  }
   // (2) external Poisson gates, order A, G, ACh, GluCl, NMDA (BgInit);
   // Philox counter (neuron within its instance, step), key C.key
   const uint32_t act = P.act;
@@ -414,7 +435,11 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
     if (I > P.dvmax) { I = P.dvmax; cs |= F_OVF; }
     else if (I < -P.dvmax) { I = -P.dvmax; cs |= F_UDF; }
     v = P.df * v + P.ef * I;
-    if (v > P.vth) { cs |= F_FIRE; firing(); }
+    if (v > P.vth) {
+      cs |= F_FIRE;
+      firing();
+      if (net.ltp_mode) cs |= F_LPFIRE;
+    }
   }
   spk = (cs & F_SPIKE) != 0;
   // (4) presynaptic gates (the reference's per-synapse gate rule, shared);
@@ -458,6 +483,25 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
   } else if (net.stp_mode == STP_STD) {
     const T d = S.sD;
     S.sD = spk ? d * (T(1) - P.stp_pv) : P.stp_dfD * d + P.stp_efD;
+  }
+  if (net.ltp_mode) {
+    // firLP: the LTP spike follows the onset after SpiLPDy steps
+    // (code/EqsGnl2.cpp:254-272); ltpEul: LP trace (:274-280)
+    uint32_t c = net.cntlp[n];
+    if (cs & F_LPFIRE) {
+      if ((int32_t)c == P.ltp_spidy) {
+        cs |= F_LPSPIKE;
+        cs &= ~F_LPFIRE;
+        c = 0;
+      } else {
+        cs &= ~F_LPSPIKE;
+        c++;
+      }
+    } else {
+      cs &= ~F_LPSPIKE;
+    }
+    net.cntlp[n] = c;
+    net.lp[n] = (cs & F_LPSPIKE) ? net.lp[n] + T(1) : P.ltp_dfLP * net.lp[n];
   }
   S.v = v;
   S.cs = (cs & ~CNT_MASK) | (cnt & CNT_MASK);
@@ -1194,6 +1238,37 @@ __global__ void k_batch_begin(int64_t *t0, int64_t value, uint32_t *total, uint3
     lcnt[2 * NSUB + s] = 0u;
     lbase[2 * NSUB + s] = 0u;
   }
+}
+
+// Learning synapses (LTP, code/EqsGnl2.cpp:1005-1044) after the update of
+// step k of the batch, one thread per synapse: add w * gate of the
+// presynaptic neuron (linear: its shared gate; NMDA: its s of step k) to the
+// post's conductance of this Spike phase, then apply the weight rules with
+// this step's spikes and LP traces: a presynaptic spike adds LP(post) *
+// NegISI (or scales by 1 + LP(post) * NegISI when NegISI <= 0), a
+// postsynaptic spike likewise with LP(pre) and PosISI; both from the
+// presynaptic neuron's parameters.
+template <typename T>
+__global__ void __launch_bounds__(256) k_learn(Net<T> net, int32_t k) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= net.n_learn) return;
+  const int32_t hslot = slot_of(net.tmod[1], k, net.DN);  // gate history slot of step k
+  const uint32_t a = net.l_pre[i], b = net.l_post[i];
+  const int r = net.l_rec[i];
+  const size_t N = (size_t)net.N;
+  T w = net.l_w[i];
+  const T gate = r == 2 ? net.shist[(size_t)hslot * N + a] : net.spre[a];
+  atomic_add_t(net.gl + (size_t)r * N + b, w * gate);
+  const DevPop<T> &Pa = net.pops[net.npop[a]];
+  if (net.cs[a] & F_SPIKE) {
+    if (Pa.ltp_neg > T(0)) w = w + net.lp[b] * Pa.ltp_neg;
+    else w = w * (T(1) + net.lp[b] * Pa.ltp_neg);
+  }
+  if (net.cs[b] & F_SPIKE) {
+    if (Pa.ltp_pos > T(0)) w = w + net.lp[a] * Pa.ltp_pos;
+    else w = w * (T(1) + net.lp[a] * Pa.ltp_pos);
+  }
+  net.l_w[i] = w;
 }
 
 // Gap-junction currents (ThdGapSyn, code/EqsGnl2.cpp:705-725) from the

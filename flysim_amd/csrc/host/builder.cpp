@@ -307,12 +307,14 @@ void build_network(const Config &cfg, const BuildOptions &opt, BuiltNet &out) {
       out.post.push_back(e.post);
       out.w.push_back(g.g);
       out.rec.push_back(r);
+      out.plastic.push_back(0);
     }
     for (const Single &s : syn1[r]) {
       out.pre.push_back(s.pre);
       out.post.push_back(s.post);
       out.w.push_back(s.g);
       out.rec.push_back(r);
+      out.plastic.push_back(1);
     }
   }
   for (size_t q = 0; q < out.gap_pre.size(); ++q) {
@@ -320,6 +322,7 @@ void build_network(const Config &cfg, const BuildOptions &opt, BuiltNet &out) {
     out.post.push_back(out.gap_post[q]);
     out.w.push_back(out.gap_g[q]);
     out.rec.push_back(R_GAP);
+    out.plastic.push_back(0);
   }
 }
 

@@ -36,6 +36,7 @@ struct BuiltNet {
   std::vector<uint32_t> pre, post;
   std::vector<float> w;
   std::vector<uint8_t> rec;
+  std::vector<uint8_t> plastic;      // 1: single-synapse path (LTP learns it)
   // reference layout: gates (group gates first, then single synapses)
   std::vector<uint32_t> gate_pre, gate_post0;
   std::vector<uint8_t> gate_rec, gate_group;

@@ -33,7 +33,7 @@ void print_help() {
             << "-t 4                  # thread count (kept for connectome parity)\n"
             << "-dt 0.1               # time step (ms)\n"
             << "-STP / -STD           # short-term plasticity (facilitation + depression / depression)\n"
-            << "-LTP                  # long-term plasticity (not supported by the GPU engine)\n"
+            << "-LTP                  # long-term plasticity of single synapses\n"
             << "-nmodel GNL2          # neuron model (GNL2 only)\n"
             << "-udfsed N             # user-defined random seed (switches STD on, as the reference does)\n"
             << "-fp64                 # double-precision state (default float)\n"
@@ -99,10 +99,10 @@ int cli_main(int argc, const char *const *argv) {
     else help = true;
   }
   if (help) { print_help(); return FS_OK; }
-  if (ltp) return set_error(FS_ERR_ARG, "long-term plasticity (-LTP) is not supported by the GPU engine");
   fs_cli_opts o{};
   o.stp = stp;
   o.std_ = stdp;
+  o.ltp = ltp;
   o.udf_seed = udf;
   o.udf_seed_value = seed;
   o.dt = dt; o.threads = th; o.repeats = rp;

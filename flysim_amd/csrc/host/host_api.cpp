@@ -128,6 +128,7 @@ int fs_conf_desc(fs_conf *c, fs_net_desc *d) {
   d->w = c->net.w.data();
   d->receptor = c->net.rec.data();
   d->delay = nullptr;
+  d->plastic = c->net.plastic.data();
   d->dt = c->cfg.dt;
   d->precision = FS_FP32;
   d->ext_mode = FS_EXT_PHILOX;

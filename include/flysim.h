@@ -95,6 +95,10 @@ typedef struct fs_net_desc {
   const float *w;                 /* [n_syn] efficacy, nS */
   const uint8_t *receptor;        /* [n_syn] FS_AMPA..FS_GCL, FS_GAP */
   const uint8_t *delay;           /* [n_syn] extra delay in steps, NULL = 0 */
+  const uint8_t *plastic;         /* [n_syn] 1: the weight follows the LTP rules when the
+                                     populations carry FS_ACT_LTP (the reference's
+                                     single-synapse path, code/EqsGnl2.cpp:1005-1044);
+                                     NULL = none */
   float dt;                       /* ms */
   int32_t precision;              /* FS_FP32 | FS_FP64 */
   int32_t ext_mode;               /* FS_EXT_PHILOX | FS_EXT_REPLAY */

@@ -77,6 +77,51 @@ def test_plasticity_state_tracks_fp64_oracle(golden, case):
         np.testing.assert_allclose(eng.state("V"), orc.state("V"), rtol=0, atol=1e-8)
 
 
+def test_cli_ltp_fp64_replay_exact_raster(golden, tmp_path):
+    """-LTP: single-synapse weights learn from delayed LTP spikes and LP
+    traces; the drop-in CLI in FP64 replay reproduces the reference raster."""
+    files = golden["files"]
+    (tmp_path / "network.conf").write_text(files["ltp_network.conf"])
+    (tmp_path / "network.pro").write_text(files["module_network.pro"])
+    print("case plasticity ltp: drop-in CLI, fp64 replay, 3000 steps", flush=True)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        argv = [b"flysim_amd", b"-fp64", b"-ext", b"replay", b"-t", b"1", b"-LTP"]
+        arr = (C.c_char_p * len(argv))(*argv)
+        A.check(A.lib().fs_cli_main(len(argv), arr))
+    finally:
+        os.chdir(cwd)
+    s = W.read_spikes(str(tmp_path / "Spikes.dat"))
+    r = np.stack([np.round(s[:, 0] / 1e-4).astype(np.int64), s[:, 1].astype(np.int64)], 1)
+    ref = golden["runs"]["ltp_raster"].astype(np.int64)
+    assert len(ref) > 1000
+    np.testing.assert_array_equal(r, ref)
+
+
+def test_ltp_state_tracks_fp64_oracle(golden):
+    files = golden["files"]
+    steps = 900
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "network.conf"), "w").write(files["ltp_network.conf"])
+        open(os.path.join(d, "network.pro"), "w").write(files["module_network.pro"])
+        conf = Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"), ltp=True)
+        bits = conf.ext_trains(steps)
+        print("case ltp state: fp64 replay, %d steps" % steps, flush=True)
+        eng = Engine(conf, precision=A.FS_FP64, ext_mode=A.FS_EXT_REPLAY)
+        assert eng.schedule["window"] == 0 and eng.schedule["fused"] == 0
+        eng.reset(conf.seed(0))
+        eng.load_ext_trains(bits, 0, steps)
+        orc = Oracle(conf.ref_layout(), conf.n_neuron, precision=1)
+        orc.reset(conf.seed(0))
+        orc.replay(bits, steps)
+        for _ in range(steps // 100):
+            rg, ro = eng.raster(100), orc.raster(100)
+            np.testing.assert_array_equal(rg, ro)
+            np.testing.assert_allclose(eng.state("V"), orc.state("V"), rtol=0, atol=1e-8)
+        assert int(eng.pop_counts().sum()) > 100
+
+
 def test_plasticity_state_absent_without_switch(golden):
     files = golden["files"]
     with tempfile.TemporaryDirectory() as d:
