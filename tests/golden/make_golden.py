@@ -85,12 +85,12 @@ def total_synapses(stdout):
     return int(m.group(1)) if m else -1
 
 
-def c1_case(name, seed, g_ext, t_ms, store, keep_syn=False, cut_steps=None):
+def c1_case(name, seed, g_ext, t_ms, store, keep_syn=False, cut_steps=None, args=()):
     d = tempfile.mkdtemp(prefix="golden_")
     net = W.c1_network(seed=seed, g_ext=g_ext, t_end_ms=t_ms)
     W.write_conf(net, os.path.join(d, "network.conf"))
     W.write_pro(os.path.join(d, "network.pro"), t_ms, seeds=[seed], spike_file="Spikes.dat")
-    out = run_ref(d)
+    out = run_ref(d, list(args))
     r = raster(os.path.join(d, "Spikes.dat"))
     store[name + "_counts"] = np.bincount(r[:, 1], minlength=net.n).astype(np.int32)
     store[name + "_raster"] = r if cut_steps is None else r[r[:, 0] < cut_steps]
@@ -613,6 +613,7 @@ def update_plasticity():
     meta["plasticity"] = plasticity_case(store, files)
     meta["outputs"] = outputs_case(store, files)
     meta["gap"] = gap_case(store, files)
+    meta["c1_stp"] = c1_case("c1_stp", 1, 1.25, 1000.0, store, args=["-STP"])
     meta["ltp"] = ltp_case(store, files)
     np.savez_compressed(os.path.join(HERE, "reference_runs.npz"), **store)
     with open(os.path.join(HERE, "reference_files.json"), "w") as f:
@@ -638,6 +639,7 @@ def main():
     meta["plasticity"] = plasticity_case(store, files)
     meta["outputs"] = outputs_case(store, files)
     meta["gap"] = gap_case(store, files)
+    meta["c1_stp"] = c1_case("c1_stp", 1, 1.25, 1000.0, store, args=["-STP"])
     meta["ltp"] = ltp_case(store, files)
     sz = seizure_case()
     if sz:

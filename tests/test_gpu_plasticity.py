@@ -122,6 +122,41 @@ def test_ltp_state_tracks_fp64_oracle(golden):
         assert int(eng.pop_counts().sum()) > 100
 
 
+def _c1_stp_conf(d):
+    net = W.c1_network(seed=1, g_ext=1.25, t_end_ms=1000.0)
+    W.write_conf(net, os.path.join(d, "network.conf"))
+    W.write_pro(os.path.join(d, "network.pro"), 1000.0, seeds=[1])
+    return Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"), stp=True)
+
+
+def test_stp_1k_fp64_exact_200ms_and_fp32_rates(golden):
+    """-STP on the 1k E/I network: FP64 replay reproduces the reference raster
+    for 200 ms and its E and I rates over 1 s within 1 %; FP32 replay keeps the E and I rates over 1 s within 2 % and
+    the per-neuron spike counts within a KS p > 0.05 of the reference."""
+    from scipy.stats import ks_2samp
+    ref = golden["runs"]["c1_stp_raster"].astype(np.int64)
+    with tempfile.TemporaryDirectory() as d:
+        conf = _c1_stp_conf(d)
+        bits = conf.ext_trains(10000)
+        print("case c1 stp: fp64 replay 2000 steps, fp32 replay 10000 steps", flush=True)
+        e64 = Engine(conf, precision=A.FS_FP64, ext_mode=A.FS_EXT_REPLAY)
+        e64.reset(conf.seed(0))
+        e64.load_ext_trains(bits, 0, 10000)
+        r64 = e64.raster(10000)
+        np.testing.assert_array_equal(r64[r64[:, 0] < 2000], ref[ref[:, 0] < 2000])
+        e32 = Engine(conf, precision=A.FS_FP32, ext_mode=A.FS_EXT_REPLAY)
+        e32.reset(conf.seed(0))
+        e32.load_ext_trains(bits, 0, 10000)
+        r32 = e32.raster(10000)
+    rates = lambda r: np.array([(r[:, 1] < 800).sum() / 800.0, (r[:, 1] >= 800).sum() / 200.0])
+    print("fp64 rates", rates(r64), "fp32 rates", rates(r32), "reference", rates(ref))
+    np.testing.assert_allclose(rates(r64), rates(ref), rtol=0.01)
+    np.testing.assert_allclose(rates(r32), rates(ref), rtol=0.02)
+    p = ks_2samp(np.bincount(r32[:, 1], minlength=1000), np.bincount(ref[:, 1], minlength=1000)).pvalue
+    print("KS p-value", p)
+    assert p > 0.05
+
+
 def test_plasticity_state_absent_without_switch(golden):
     files = golden["files"]
     with tempfile.TemporaryDirectory() as d:

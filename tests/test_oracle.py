@@ -71,6 +71,23 @@ def test_fp32_oracle_short_term_plasticity(golden, case):
     assert d_min < 0.9  # depression engaged
 
 
+def test_fp32_oracle_stp_1k_network(golden):
+    """-STP on the 1k E/I network (single-neuron populations, so every link
+    is a single synapse): the first 200 ms of the reference's raster."""
+    with tempfile.TemporaryDirectory() as d:
+        net = W.c1_network(seed=1, g_ext=1.25, t_end_ms=1000.0)
+        W.write_conf(net, os.path.join(d, "network.conf"))
+        W.write_pro(os.path.join(d, "network.pro"), 1000.0, seeds=[1])
+        c = Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"), stp=True)
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        r = o.raster(2000)
+    ref = golden["runs"]["c1_stp_raster"].astype(np.int64)
+    ref = ref[ref[:, 0] < 2000]
+    assert len(ref) > 500
+    np.testing.assert_array_equal(r, ref)
+
+
 def test_plasticity_switch_changes_the_raster(golden):
     """The same conf without the switch gives another raster: the fixture
     pins the STP terms, not only the network."""
