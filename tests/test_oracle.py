@@ -73,7 +73,9 @@ def test_fp32_oracle_short_term_plasticity(golden, case):
 
 def test_fp32_oracle_stp_1k_network(golden):
     """-STP on the 1k E/I network (single-neuron populations, so every link
-    is a single synapse): the first 200 ms of the reference's raster."""
+    is a single synapse): the FP32 oracle gives the first 200 ms of the
+    reference's raster exactly (FP64 moves one near-threshold spike, neuron
+    818, from step 905 to 906)."""
     with tempfile.TemporaryDirectory() as d:
         net = W.c1_network(seed=1, g_ext=1.25, t_end_ms=1000.0)
         W.write_conf(net, os.path.join(d, "network.conf"))
@@ -82,10 +84,16 @@ def test_fp32_oracle_stp_1k_network(golden):
         o = Oracle(c.ref_layout(), c.n_neuron, precision=0)
         o.reset(c.seed(0))
         r = o.raster(2000)
+        o64 = Oracle(c.ref_layout(), c.n_neuron, precision=1)
+        o64.reset(c.seed(0))
+        r64 = o64.raster(2000)
     ref = golden["runs"]["c1_stp_raster"].astype(np.int64)
     ref = ref[ref[:, 0] < 2000]
     assert len(ref) > 500
     np.testing.assert_array_equal(r, ref)
+    assert r64.shape == ref.shape
+    bad = np.nonzero((r64 != ref).any(1))[0]
+    assert bad.size == 1 and tuple(ref[bad[0]]) == (905, 818) and tuple(r64[bad[0]]) == (906, 818)
 
 
 def test_plasticity_switch_changes_the_raster(golden):
