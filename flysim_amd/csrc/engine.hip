@@ -91,7 +91,14 @@ struct Engine final : EngineBase {
       b_ens, b_stp, b_irep, b_rp_g, b_col_g, b_w_g, b_igap, b_lp, b_cntlp, b_lpre, b_lpost, b_lrec,
       b_lw, b_lw0, b_gl, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
       b_total, b_rec_k, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase, b_suboff, b_subcap,
-      b_pbr, b_gain, b_skey, b_tmod;
+      b_pbr, b_gain, b_skey, b_tmod, b_zig;
+  // replay mode's Gaussian injection: population ranges and the position in
+  // the reference's ziggurat stream (zig_calls calls before step zig_t0,
+  // zig_a calls per step from there)
+  std::vector<int64_t> pfirst, pcount;
+  bool pops_contiguous = true;
+  uint32_t zig_calls = 0, zig_a = 0;
+  int64_t zig_t0 = 0;
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
   bool fused() const { return fused_ok && sched.fused && !window(); }
   // windowed path: steps per launch, at most 1 + the smallest synaptic delay
@@ -162,6 +169,58 @@ struct Engine final : EngineBase {
     d.spidy = p.spidy;
     d.act = p.act;
     return d;
+  }
+
+  // r4_nor_setup (code/StiExt.h:321-351): the 128 strips of Marsaglia and
+  // Tsang's ziggurat, computed in double and stored as the reference stores them
+  static fsk::ZigTables zig_tables() {
+    fsk::ZigTables z{};
+    const double m1 = 2147483648.0, vn = 9.91256303526217e-3;
+    double dn = 3.442619855899, tn = dn;
+    const double q = vn / std::exp(-0.5 * dn * dn);
+    z.kn[0] = (uint32_t)((dn / q) * m1);
+    z.kn[1] = 0;
+    z.wn[0] = (float)(q / m1);
+    z.wn[127] = (float)(dn / m1);
+    z.fn[0] = 1.0f;
+    z.fn[127] = (float)std::exp(-0.5 * dn * dn);
+    for (int i = 126; i >= 1; --i) {
+      dn = std::sqrt(-2.0 * std::log(vn / dn + std::exp(-0.5 * dn * dn)));
+      z.kn[i + 1] = (uint32_t)((dn / tn) * m1);
+      tn = dn;
+      z.fn[i] = (float)std::exp(-0.5 * dn * dn);
+      z.wn[i] = (float)(dn / m1);
+    }
+    return z;
+  }
+
+  // Replay mode: each step the reference draws one ziggurat call per neuron
+  // with injection on, in index order, from one stream seeded at Run
+  // (code/EqsGnl2.cpp:335, 1098-1105; code/NetSimGnl2.cpp:323).  Called when
+  // the set of injected populations changes (and at reset): the calls up to
+  // the current step are banked, and each population gets the offsets that
+  // make neuron n's call at step t equal t * zig_a + n + zig_b (mod 2^32).
+  void zig_refresh() {
+    if (net.ext_mode != FS_EXT_REPLAY) return;
+    zig_calls += (uint32_t)(step - zig_t0) * zig_a;
+    zig_t0 = step;
+    std::vector<int32_t> on;
+    for (int32_t i = 0; i < P; ++i)
+      if ((hpop[i].act & FS_ACT_INJ) && pcount[i]) on.push_back(i);
+    if (!on.empty() && (B != 1 || !pops_contiguous))
+      throw std::invalid_argument(
+          "replay-mode Gaussian injection needs one instance whose populations are contiguous neuron ranges");
+    std::sort(on.begin(), on.end(), [&](int32_t x, int32_t y) { return pfirst[x] < pfirst[y]; });
+    uint32_t a = 0;
+    for (int32_t i : on) a += (uint32_t)pcount[i];
+    uint32_t rank = 0;
+    for (int32_t i = 0; i < P; ++i) dpop[i].zig_a = a, dpop[i].zig_b = 0;
+    for (int32_t i : on) {
+      dpop[i].zig_b = zig_calls - (uint32_t)zig_t0 * a + rank - (uint32_t)pfirst[i];
+      rank += (uint32_t)pcount[i];
+    }
+    zig_a = a;
+    upload_pops();
   }
 
   void upload_pops() {
@@ -592,6 +651,19 @@ struct Engine final : EngineBase {
     HIPCHECK(hipMemcpy(b_npop.p, neuron_pop, n0 * 4, hipMemcpyHostToDevice));
     b_pops.alloc(sizeof(fsk::DevPop<T>) * P);
     upload_pops();
+    pfirst.assign(P, -1);
+    pcount.assign(P, 0);
+    for (size_t k = 0; k < n0; ++k) {
+      const uint32_t p = neuron_pop[k];
+      if (pfirst[p] < 0) pfirst[p] = (int64_t)k;
+      pcount[p]++;
+    }
+    for (size_t k = 0; k < n0; ++k)
+      if ((int64_t)k >= pfirst[neuron_pop[k]] + pcount[neuron_pop[k]]) pops_contiguous = false;
+    const fsk::ZigTables zt = zig_tables();
+    b_zig.alloc(sizeof(zt));
+    HIPCHECK(hipMemcpy(b_zig.p, &zt, sizeof(zt), hipMemcpyHostToDevice));
+    net.zig = b_zig.as<fsk::ZigTables>();
     // per-replica tables: pb from the population blocks, unit gains, and
     // stream keys spaced by the 64-bit golden ratio (replica 0 keeps the seed)
     hpbr.assign((size_t)5 * P * B, 0.0f);
@@ -923,6 +995,10 @@ struct Engine final : EngineBase {
     net.seed = seed;
     graph_dirty = true;  // the seed is a kernel argument
     step = 0;
+    zig_calls = 0;
+    zig_a = 0;
+    zig_t0 = 0;
+    zig_refresh();
     std::memset(ctr_last, 0, sizeof(ctr_last));
   }
 
@@ -932,10 +1008,12 @@ struct Engine final : EngineBase {
   void set_pop(int32_t g, const fs_pop_params &p) override {
     if (g < 0 || g >= P * B) throw std::invalid_argument("population index out of range");
     const int32_t rep = g / P, i = g - rep * P;
+    const bool inj_switch = ((hpop[i].act ^ p.act) & FS_ACT_INJ) != 0;
     hpop[i] = p;
     dpop[i] = to_dev(p, dpop[i]);
     HIPCHECK(hipMemcpyAsync(b_pops.as<fsk::DevPop<T>>() + i, &dpop[i], sizeof(fsk::DevPop<T>),
                             hipMemcpyHostToDevice, stream));
+    if (inj_switch) zig_refresh();
     pbr_from_pop(i, rep, p);
     for (int r = 0; r < 5; ++r) note_pb(r, pbr(r, i, rep));
     tables_dirty = true;

@@ -53,6 +53,8 @@ __host__ __device__ constexpr uint32_t lin_act(int li) {
   return li == 0 ? ACT_AMPA : li == 1 ? ACT_GABA : li == 2 ? ACT_ACH : ACT_GCL;
 }
 
+struct ZigTables;
+
 // Per-population parameters in the precision of the run.
 template <typename T>
 struct DevPop {
@@ -76,6 +78,9 @@ struct DevPop {
   // seed ^ replica key ^ (key_hi:key_lo), counter = local neuron - first
   uint32_t key_lo, key_hi;
   int32_t first;
+  // replay mode: the Gaussian injection of neuron n at step t is call
+  // t * zig_a + n + zig_b (mod 2^32) of the reference's ziggurat stream
+  uint32_t zig_a, zig_b;
 };
 
 template <typename T>
@@ -139,6 +144,8 @@ struct Net {
   const uint32_t *ext_bits;
   int64_t ext_first, ext_steps;
   uint64_t seed;
+  // replay mode: the reference's ziggurat tables for the Gaussian injection
+  const ZigTables *zig;
   // spike record of the current batch
   uint32_t *rec_ids;
   T *rec_a;
@@ -175,6 +182,88 @@ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1
   }
 }
 __device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// The reference's Gaussian injection (ziggurat::Igauss, code/StiExt.h:214-218,
+// one stream per solver thread): every call advances the LCG
+// jcong' = 69069 jcong + 1234567 once (code/StiExt.h:222-229) and seeds SHR3
+// with it, so call k depends only on the seed and k.  lcg_jump gives the
+// state after m calls directly, and every neuron-step draws independently.
+struct ZigTables {
+  uint32_t kn[128];
+  float fn[128], wn[128];
+};
+
+__device__ __forceinline__ uint32_t lcg_jump(uint32_t seed, uint32_t m) {
+  uint32_t aa = 1u, cc = 0u, a = 69069u, c = 1234567u;  // x -> aa x + cc; step x -> a x + c
+  while (m) {
+    if (m & 1u) {
+      aa *= a;
+      cc = cc * a + c;
+    }
+    c = (a + 1u) * c;
+    a *= a;
+    m >>= 1;
+  }
+  return aa * seed + cc;
+}
+
+// SHR3 step: returns the input plus the shifted state (shr3_seeded / r4_uni,
+// code/StiExt.h:231-259)
+__device__ __forceinline__ uint32_t shr3_next(uint32_t &s) {
+  const uint32_t in = s;
+  s ^= s << 13;
+  s ^= s >> 17;
+  s ^= s << 5;
+  return in + s;
+}
+
+// r4_uni: fmod(0.5 + float(u) / 2^32, 1.0) in double, returned as float
+__device__ __forceinline__ float zig_uni(uint32_t &s) {
+  double d = 0.5 + (double)(float)shr3_next(s) / 65536.0 / 65536.0;
+  if (d >= 1.0) d -= 1.0;  // fmod(d, 1.0) for d in [0.5, 1.5]
+  return (float)d;
+}
+
+// logf, rounded from the double logarithm
+__device__ __forceinline__ float zig_logf(float u) { return (float)log((double)u); }
+
+// r4_nor (code/StiExt.h:262-319), Marsaglia and Tsang's 128-strip ziggurat
+// with r = 3.442620, in the reference's float / double mix and without
+// contraction.  The rejection loops end after 256 rounds (the expected count
+// is about 1.01), where the reference would keep drawing.
+__device__ __noinline__ float zig_normal(uint32_t jsr, const ZigTables *z) {
+#pragma clang fp contract(off)
+  const float r = 3.442620f;
+  int32_t hz = (int32_t)shr3_next(jsr);
+  uint32_t iz = (uint32_t)hz & 127u;
+  if (fabs((double)hz) < (double)z->kn[iz]) return (float)hz * z->wn[iz];
+  for (int round = 0; round < 256; ++round) {
+    if (iz == 0) {
+      float x = 0.0f;
+      for (int j = 0; j < 256; ++j) {
+        x = (float)(-0.2904764 * (double)zig_logf(zig_uni(jsr)));
+        const float y = -zig_logf(zig_uni(jsr));
+        if (x * x <= y + y) break;
+      }
+      return hz <= 0 ? -r - x : r + x;
+    }
+    const float x = (float)hz * z->wn[iz];
+    const float f = z->fn[iz] + zig_uni(jsr) * (z->fn[iz - 1] - z->fn[iz]);
+    if ((double)f < exp(-0.5 * (double)x * (double)x)) return x;
+    hz = (int32_t)shr3_next(jsr);
+    iz = (uint32_t)hz & 127u;
+    if (fabs((double)hz) < (double)z->kn[iz]) return (float)hz * z->wn[iz];
+  }
+  return (float)hz * z->wn[iz];
+}
+
+// Igauss for call k of the stream seeded with seed: mean + sd * r4_nor
+__device__ __forceinline__ float zig_igauss(uint32_t seed, uint32_t k, float mean, float sd,
+                                            const ZigTables *z) {
+#pragma clang fp contract(off)
+  const float v = zig_normal(lcg_jump(seed, k + 1u), z);
+  return mean + sd * v;
+}
 
 // Load through the constant address space: for tables no kernel writes
 // (CSR, population parameters) a wave-uniform address then becomes a scalar
@@ -346,7 +435,7 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
   const uint64_t key = C.key;
   const uint32_t ctr = C.ctr;
   uint32_t rnd[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-  if ((!REPLAY && net.ext_mask) || (act & ACT_INJ)) {
+  if (!REPLAY && (net.ext_mask || (act & ACT_INJ))) {
     uint32_t c[4] = {ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0u};
     philox10(c, (uint32_t)key, (uint32_t)(key >> 32));
     rnd[0] = c[0]; rnd[1] = c[1]; rnd[2] = c[2]; rnd[3] = c[3];
@@ -391,12 +480,18 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
   // injection drawn now (BgInit adds to Isyn, code/EqsGnl2.cpp:335)
   T isyn = net.igap ? net.igap[n] : T(0);
   if (act & ACT_INJ) {
-    // Box-Muller on the second Philox block (statistically the reference's
-    // Gaussian injection; the reference draws from a sequential ziggurat)
-    const float u1 = ((float)(rnd[6] >> 8) + 1.0f) * (1.0f / 16777216.0f);
-    const float u2 = u24(rnd[7]);
-    const float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
-    isyn = isyn + (P.inj_mean + P.inj_std * T(z));
+    if (REPLAY) {
+      // the reference's own ziggurat stream (single-thread order: neurons in
+      // index order, only those with injection on draw)
+      const uint32_t k = (uint32_t)t * P.zig_a + (uint32_t)n + P.zig_b;
+      isyn = isyn + T(zig_igauss((uint32_t)net.seed, k, (float)P.inj_mean, (float)P.inj_std, net.zig));
+    } else {
+      // Box-Muller on the second Philox block (the counter-based stream)
+      const float u1 = ((float)(rnd[6] >> 8) + 1.0f) * (1.0f / 16777216.0f);
+      const float u2 = u24(rnd[7]);
+      const float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+      isyn = isyn + (P.inj_mean + P.inj_std * T(z));
+    }
   }
   // (3) membrane (VmEul / firing)
   auto firing = [&]() {

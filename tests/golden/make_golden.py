@@ -5,7 +5,7 @@ Runs only where the reference sources exist (this build container).  For
 each case it writes the network in the reference's text format, runs the
 reference with `-t 1 -rp 1` and keeps only small outputs: spike rasters as
 (step, neuron) pairs, synapse orders from network.log, the bytes of small
-output files, and digests.  Usage (repo root):  python tests/golden/make_golden.py [plasticity]
+output files, and digests.  Usage (repo root):  python tests/golden/make_golden.py [plasticity|noise]
 """
 from __future__ import annotations
 
@@ -329,6 +329,71 @@ EndOutControl
 """
 
 
+NOISE2_PRO = """DefineMacro
+RandomSeeds: 9 EndRandomSeeds
+EndDefineMacro
+EventTime 10.0
+Type=ChangeMembraneNoise
+Population: Inh
+GaussMean=0.1
+GaussSTD=0.5
+EndEvent
+EventTime 30.0
+Type=ChangeMembraneNoise
+Population: Solo
+GaussMean=0.2
+GaussSTD=0.8
+EndEvent
+EventTime 80.0
+Type=ChangeMembraneNoise
+Population: ExcA
+GaussMean=0.05
+GaussSTD=1.0
+EndEvent
+EventTime 150.0
+Type=ChangeMembraneNoise
+Population: Inh
+GaussMean=0
+GaussSTD=0
+EndEvent
+EventTime 200.0
+Type=ChangeMembraneNoise
+Population: Solo
+GaussMean=0.3
+GaussSTD=0.3
+EndEvent
+EventTime 250.0
+Type=ChangeMembraneNoise
+Population: ExcA
+GaussMean=0
+GaussSTD=0
+EndEvent
+EventTime 300.0
+Type=EndTrial
+EndEvent
+OutControl
+FileName:Spikes.dat
+Type=Spike
+population:AllPopulation
+EndOutputFile
+EndOutControl
+"""
+
+
+def noise2_case(store, files):
+    """Injection switched on for three populations in turn (the middle one
+    ahead of the others in neuron order), switched off for two, and changed
+    in place for one: the stream's per-step draw count changes five times."""
+    d = tempfile.mkdtemp(prefix="golden_")
+    open(os.path.join(d, "network.conf"), "w").write(MODULE_CONF)
+    open(os.path.join(d, "network.pro"), "w").write(NOISE2_PRO)
+    run_ref(d)
+    store["noise2_raster"] = raster(os.path.join(d, "Spikes.dat"))
+    files["noise2_network.pro"] = NOISE2_PRO
+    shutil.rmtree(d)
+    return {"spikes": int(len(store["noise2_raster"]))}
+
+
 def noise_case(store, files):
     d = tempfile.mkdtemp(prefix="golden_")
     open(os.path.join(d, "network.conf"), "w").write(MODULE_CONF)
@@ -604,6 +669,22 @@ def seizure_case():
             hashlib.sha256("\n".join(names).encode()).hexdigest()}
 
 
+def update_noise():
+    """Regenerate only the injection fixtures, keeping the others as stored."""
+    build_ref()
+    store = dict(np.load(os.path.join(HERE, "reference_runs.npz")))
+    files = json.load(open(os.path.join(HERE, "reference_files.json")))
+    meta = json.load(open(os.path.join(HERE, "reference_meta.json")))
+    meta["noise"] = noise_case(store, files)
+    meta["noise2"] = noise2_case(store, files)
+    np.savez_compressed(os.path.join(HERE, "reference_runs.npz"), **store)
+    with open(os.path.join(HERE, "reference_files.json"), "w") as f:
+        json.dump(files, f, indent=0)
+    with open(os.path.join(HERE, "reference_meta.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(meta["noise"], meta["noise2"])
+
+
 def update_plasticity():
     """Regenerate only the plasticity fixtures, keeping the others as stored."""
     build_ref()
@@ -635,6 +716,7 @@ def main():
     meta["c1_hyper"] = c1_case("c1_hyper", 1, 1.18, 300.0, store, cut_steps=2500)
     meta["module"] = module_case(store, files)
     meta["noise"] = noise_case(store, files)
+    meta["noise2"] = noise2_case(store, files)
     meta["example"] = example_case(store)
     meta["plasticity"] = plasticity_case(store, files)
     meta["outputs"] = outputs_case(store, files)
@@ -655,5 +737,7 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["plasticity"]:
         update_plasticity()
+    elif sys.argv[1:] == ["noise"]:
+        update_noise()
     else:
         main()

@@ -1,11 +1,21 @@
-"""GPU test of Gaussian current injection (ChangeMembraneNoise,
-code/NetSimGnl2.cpp:120-133).  The engine draws the Gaussian with Philox +
-Box-Muller where the reference draws a sequential Marsaglia-Tsang ziggurat
-stream (restated exactly by the oracle, pinned against the reference in
-tests/test_oracle.py), so the bar is statistical: after 300 ms of noise-only
-drive the membrane potentials of 4000 non-spiking neurons have the same
-distribution as the oracle's (two-sample KS p > 1e-3, means within 4 standard
-errors, standard deviations within 5 %)."""
+"""GPU tests of Gaussian current injection (ChangeMembraneNoise,
+code/NetSimGnl2.cpp:120-133).
+
+Replay mode draws the reference's own stream: every Igauss call reseeds SHR3
+from the next state of one LCG (code/StiExt.h:214-229), so the device jumps
+the LCG to each neuron-step's call index and runs Marsaglia and Tsang's
+ziggurat there.  Bars: the drop-in command line in FP64 with replayed draws
+reproduces the reference rasters of two injection protocols exactly (the
+second switches injection on and off for populations in and out of neuron
+order); 4000 injected neurons keep the FP64 oracle's potentials to 1e-6 mV
+over 300 ms (12 M draws).
+
+The counter-based mode draws with Philox + Box-Muller instead, so its bar is
+statistical: after 300 ms of noise-only drive the membrane potentials of 4000
+non-spiking neurons have the same distribution as the oracle's (two-sample
+KS p > 1e-3, means within 4 standard errors, standard deviations within 5 %).
+"""
+import ctypes as C
 import os
 import tempfile
 
@@ -13,6 +23,7 @@ import numpy as np
 import pytest
 
 from flysim_amd import _capi as A
+from flysim_amd import workloads as W
 from flysim_amd.engine import Conf, Engine
 from oracle.oracle import Oracle
 
@@ -80,3 +91,47 @@ def test_injection_membrane_distribution_matches_oracle():
     assert abs(vg.mean() - vo.mean()) < 4 * se
     assert abs(vg.std() / vo.std() - 1) < 0.05
     assert ks_2samp(vg, vo).pvalue > 1e-3
+
+
+def test_injection_replay_follows_the_reference_stream():
+    steps = 3000
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "network.conf"), "w").write(CONF)
+        open(os.path.join(d, "network.pro"), "w").write(PRO)
+        conf = Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"))
+        bits = conf.ext_trains(steps)
+        print("case injection replay: 4000 neurons, %d steps, FP64" % steps, flush=True)
+        orc = Oracle(conf.ref_layout(), conf.n_neuron, precision=1)
+        orc.reset(conf.seed(0))
+        orc.replay(bits, steps)
+        eng = Engine(conf, precision=A.FS_FP64, ext_mode=A.FS_EXT_REPLAY)
+        eng.reset(conf.seed(0))
+        eng.load_ext_trains(bits, 0, steps)
+        eng.set_injection(0, 0.05, 0.4)      # the t = 0 event, as DoEvt applies it
+        for _ in range(3):
+            assert len(eng.raster(steps // 3)) == 0
+            orc.raster(steps // 3)
+            vg, vo = eng.state("V"), orc.state("V")
+            assert vo.std() > 0.5
+            np.testing.assert_allclose(vg, vo, rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("case,steps", [("noise", 1200), ("noise2", 3000)])
+def test_cli_injection_fp64_replay_exact_raster(golden, case, steps, tmp_path):
+    files = golden["files"]
+    (tmp_path / "network.conf").write_text(files["module_network.conf"])
+    (tmp_path / "network.pro").write_text(files[case + "_network.pro"])
+    print("case injection %s: drop-in CLI, fp64 replay, %d steps" % (case, steps), flush=True)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        argv = [b"flysim_amd", b"-fp64", b"-ext", b"replay", b"-t", b"1"]
+        arr = (C.c_char_p * len(argv))(*argv)
+        A.check(A.lib().fs_cli_main(len(argv), arr))
+    finally:
+        os.chdir(cwd)
+    s = W.read_spikes(str(tmp_path / "Spikes.dat"))
+    r = np.stack([np.round(s[:, 0] / 1e-4).astype(np.int64), s[:, 1].astype(np.int64)], 1)
+    ref = golden["runs"][case + "_raster"].astype(np.int64)
+    assert len(ref) > 300
+    np.testing.assert_array_equal(r, ref)

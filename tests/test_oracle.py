@@ -166,6 +166,20 @@ def test_fp32_oracle_gaussian_injection_ziggurat(golden):
     np.testing.assert_array_equal(r, golden["runs"]["noise_raster"].astype(np.int64))
 
 
+def test_fp32_oracle_injection_switched_on_and_off(golden):
+    """Injection switched on, off and changed for populations in and out of
+    neuron order (300 ms): the stream keeps the reference's draw order."""
+    f = golden["files"]
+    with tempfile.TemporaryDirectory() as d:
+        c = _conf_from_text(d, f["module_network.conf"], f["noise2_network.pro"])
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        r = o.raster(3000)
+    ref = golden["runs"]["noise2_raster"].astype(np.int64)
+    assert len(ref) > 500
+    np.testing.assert_array_equal(r, ref)
+
+
 def test_fp64_oracle_with_replayed_trains_matches_reference_200ms(golden):
     """FP64 restatement fed the reference's draw outcomes keeps the raster."""
     with tempfile.TemporaryDirectory() as d:
