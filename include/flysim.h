@@ -42,6 +42,12 @@ enum {
 };
 
 enum { FS_FP32 = 0, FS_FP64 = 1 };
+/* Random streams.  FS_EXT_PHILOX: counter-based (Philox4x32-10) Poisson draws
+ * and Box-Muller Gaussian injection.  FS_EXT_REPLAY: the reference's streams
+ * -- external draws loaded with fs_load_ext_trains (its mt19937 outcomes,
+ * code/EqsGnl2.cpp:284-332) and Gaussian injection from its ziggurat stream
+ * seeded at fs_reset (code/StiExt.h:214-319, code/NetSimGnl2.cpp:323), in
+ * single-thread draw order. */
 enum { FS_EXT_PHILOX = 0, FS_EXT_REPLAY = 1 };
 
 /* Derived per-population parameters, the values the reference stores per
@@ -177,7 +183,9 @@ int fs_set_ext_rate(fs_handle *h, int32_t pop, int32_t receptor, float hz);
 /* The same for every population at once: hz[n_pop] (sweep drivers). */
 int fs_set_ext_rates(fs_handle *h, int32_t receptor, const float *hz, int32_t n_pop);
 
-/* ChangeMembraneNoise / CurrentInjec semantics (code/NetSimGnl2.cpp:120-133). */
+/* ChangeMembraneNoise / CurrentInjec semantics (code/NetSimGnl2.cpp:120-133).
+ * In FS_EXT_REPLAY mode the handle must hold one instance whose populations
+ * are contiguous neuron ranges (FS_ERR_ARG otherwise). */
 int fs_set_injection(fs_handle *h, int32_t pop, float mean, float std_dev);
 
 /* Overwrite one population's derived parameters (sweeps, gains). */
