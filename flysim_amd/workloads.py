@@ -336,10 +336,26 @@ def write_conf(net: Network, path: str) -> None:
         f.write("".join(out))
 
 
+def seizure_events(names: Sequence[str], seed: int, onset_ms: float = 1.0,
+                   mean_lo: float = 0.04, mean_hi: float = 0.16,
+                   std_ratio: float = 6.0) -> List[str]:
+    """Events of the fly-brain seizure protocol's form
+    (experiment/seizure/network1_900_10nOvf.pro): at ``onset_ms`` every
+    population gets its own ChangeMembraneNoise with GaussMean ~ U(mean_lo,
+    mean_hi) nA and GaussSTD = std_ratio * GaussMean (the protocol's ratio),
+    one event block per population."""
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(mean_lo, mean_hi, size=len(names))
+    return ["EventTime %s\nType=ChangeMembraneNoise\nLabel=#1#\nPopulation:%s\n"
+            "GaussMean=%s\nGaussSTD=%s\nEndEvent\n%%--------------------------------------\n\n"
+            % (_fmt(onset_ms), n, _fmt(m), _fmt(std_ratio * m)) for n, m in zip(names, means)]
+
+
 def write_pro(path: str, t_end_ms: float, seeds: Sequence[int] = (),
               spike_file: Optional[str] = "Spikes.dat",
               rate_file: Optional[str] = None, rate_window_ms: int = 50,
-              rate_print_ms: int = 10, events: Sequence[str] = ()) -> None:
+              rate_print_ms: int = 10, events: Sequence[str] = (),
+              rate_sum: bool = False) -> None:
     """Write ``network.pro`` (code/parser.cpp:911-1407): seeds, events, outputs."""
     out = []
     if seeds:
@@ -354,8 +370,9 @@ def write_pro(path: str, t_end_ms: float, seeds: Sequence[int] = (),
                    % spike_file)
     if rate_file:
         out.append("FileName:%s\nType=FiringRate\nFiringRateWindow=%d\nPrintStep=%d\n"
-                   "population:AllPopulation\nEndOutputFile\n"
-                   % (rate_file, rate_window_ms, rate_print_ms))
+                   "population:AllPopulation\n%sEndOutputFile\n"
+                   % (rate_file, rate_window_ms, rate_print_ms,
+                      "SumAllPopulation:true\n" if rate_sum else ""))
     out.append("EndOutControl\n")
     with open(path, "w") as f:
         f.write("".join(out))

@@ -394,6 +394,30 @@ def noise2_case(store, files):
     return {"spikes": int(len(store["noise2_raster"]))}
 
 
+def seizure_case_1k(store, files, seed=4, g_ext=0.8, t_ms=300.0):
+    """The seizure protocol's form on the 1k E/I network: every single-neuron
+    population gets its own ChangeMembraneNoise at 1 ms (GaussSTD = 6 x
+    GaussMean), Spike and summed FiringRate outputs (window 50 ms, printed
+    every 1 ms) as in experiment/seizure/network1_900_10nOvf.pro.  With
+    g_ext = 0.8 nS the network stays nearly silent for ~130 ms, then ignites
+    into ~300 Hz hyperactivity."""
+    d = tempfile.mkdtemp(prefix="golden_")
+    net = W.c1_network(seed=seed, g_ext=g_ext, t_end_ms=t_ms)
+    W.write_conf(net, os.path.join(d, "network.conf"))
+    ev = W.seizure_events([p.name for p in net.pops], seed)
+    W.write_pro(os.path.join(d, "network.pro"), t_ms, seeds=[seed], spike_file="Spikes.dat",
+                rate_file="FRates.dat", rate_window_ms=50, rate_print_ms=1, events=ev,
+                rate_sum=True)
+    run_ref(d)
+    r = raster(os.path.join(d, "Spikes.dat"))
+    store["seizure1k_raster"] = r
+    files["seizure1k_FRates.dat"] = open(os.path.join(d, "FRates.dat")).read()
+    files["seizure1k_network.pro"] = open(os.path.join(d, "network.pro")).read()
+    shutil.rmtree(d)
+    return {"seed": seed, "g_ext_nS": g_ext, "t_ms": t_ms, "spikes": int(len(r)),
+            "events": len(ev)}
+
+
 def noise_case(store, files):
     d = tempfile.mkdtemp(prefix="golden_")
     open(os.path.join(d, "network.conf"), "w").write(MODULE_CONF)
@@ -677,12 +701,13 @@ def update_noise():
     meta = json.load(open(os.path.join(HERE, "reference_meta.json")))
     meta["noise"] = noise_case(store, files)
     meta["noise2"] = noise2_case(store, files)
+    meta["seizure1k"] = seizure_case_1k(store, files)
     np.savez_compressed(os.path.join(HERE, "reference_runs.npz"), **store)
     with open(os.path.join(HERE, "reference_files.json"), "w") as f:
         json.dump(files, f, indent=0)
     with open(os.path.join(HERE, "reference_meta.json"), "w") as f:
         json.dump(meta, f, indent=1)
-    print(meta["noise"], meta["noise2"])
+    print(meta["noise"], meta["noise2"], meta["seizure1k"])
 
 
 def update_plasticity():
@@ -717,6 +742,7 @@ def main():
     meta["module"] = module_case(store, files)
     meta["noise"] = noise_case(store, files)
     meta["noise2"] = noise2_case(store, files)
+    meta["seizure1k"] = seizure_case_1k(store, files)
     meta["example"] = example_case(store)
     meta["plasticity"] = plasticity_case(store, files)
     meta["outputs"] = outputs_case(store, files)

@@ -135,3 +135,40 @@ def test_cli_injection_fp64_replay_exact_raster(golden, case, steps, tmp_path):
     ref = golden["runs"][case + "_raster"].astype(np.int64)
     assert len(ref) > 300
     np.testing.assert_array_equal(r, ref)
+
+
+def test_cli_seizure_protocol_form_1k(golden, tmp_path):
+    """The fly-brain seizure protocol's form (one ChangeMembraneNoise per
+    single-neuron population at 1 ms, GaussSTD = 6 x GaussMean; Spike and
+    summed FiringRate outputs, window 50 ms printed every 1 ms) on the 1k E/I
+    network, through the drop-in CLI in FP64 with replayed draws.  The
+    network ignites at ~130 ms into ~300 Hz hyperactivity.  The raster equals
+    the reference's for the first 200 ms and the FiringRate lines of those
+    200 ms are identical; about 75 ms into the hyperactive phase FP64 and the
+    reference's float arithmetic part ways (the FP64 oracle does the same), so
+    over 300 ms the spike count is held to 1 %."""
+    files = golden["files"]
+    net = W.c1_network(seed=4, g_ext=0.8, t_end_ms=300.0)
+    W.write_conf(net, str(tmp_path / "network.conf"))
+    (tmp_path / "network.pro").write_text(files["seizure1k_network.pro"])
+    print("case seizure form: 1000 populations, 1000 noise events, drop-in CLI, fp64 replay, 3000 steps",
+          flush=True)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        argv = [b"flysim_amd", b"-fp64", b"-ext", b"replay", b"-t", b"1"]
+        arr = (C.c_char_p * len(argv))(*argv)
+        A.check(A.lib().fs_cli_main(len(argv), arr))
+    finally:
+        os.chdir(cwd)
+    s = W.read_spikes(str(tmp_path / "Spikes.dat"))
+    r = np.stack([np.round(s[:, 0] / 1e-4).astype(np.int64), s[:, 1].astype(np.int64)], 1)
+    ref = golden["runs"]["seizure1k_raster"].astype(np.int64)
+    np.testing.assert_array_equal(r[r[:, 0] < 2000], ref[ref[:, 0] < 2000])
+    assert (ref[:, 0] < 1200).sum() < 400 and (ref[:, 0] >= 2000).sum() > 25000  # quiet, then ignited
+    print("spikes", len(r), "reference", len(ref))
+    assert abs(len(r) / len(ref) - 1) < 0.01
+    got = (tmp_path / "FRates.dat").read_text().split("\n")
+    want = files["seizure1k_FRates.dat"].split("\n")
+    assert len(got) == len(want)
+    assert got[:200] == want[:200]

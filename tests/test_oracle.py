@@ -180,6 +180,25 @@ def test_fp32_oracle_injection_switched_on_and_off(golden):
     np.testing.assert_array_equal(r, ref)
 
 
+def test_fp32_oracle_seizure_protocol_form(golden):
+    """One ChangeMembraneNoise per single-neuron population at 1 ms on the 1k
+    E/I network: quiet, then ignition into ~300 Hz; the FP32 oracle gives the
+    reference's 300 ms raster (48,543 spikes) exactly."""
+    f = golden["files"]
+    with tempfile.TemporaryDirectory() as d:
+        net = W.c1_network(seed=4, g_ext=0.8, t_end_ms=300.0)
+        W.write_conf(net, os.path.join(d, "network.conf"))
+        open(os.path.join(d, "network.pro"), "w").write(f["seizure1k_network.pro"])
+        c = Conf(os.path.join(d, "network.conf"), os.path.join(d, "network.pro"))
+        assert c.n_events == 1001
+        o = Oracle(c.ref_layout(), c.n_neuron, precision=0)
+        o.reset(c.seed(0))
+        r = o.raster(3000)
+    ref = golden["runs"]["seizure1k_raster"].astype(np.int64)
+    assert len(ref) > 40000
+    np.testing.assert_array_equal(r, ref)
+
+
 def test_fp64_oracle_with_replayed_trains_matches_reference_200ms(golden):
     """FP64 restatement fed the reference's draw outcomes keeps the raster."""
     with tempfile.TemporaryDirectory() as d:
