@@ -41,6 +41,9 @@ def parse():
                    help="steps of the CPU-baseline sample (-1: sized to ~15 s)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-sweep", action="store_true", help="skip the sweep points/s figures")
+    p.add_argument("--injection", action="store_true",
+                   help="the seizure protocol's form: Gaussian current injection on every "
+                        "population from step 0 (GaussMean ~ U(0.04, 0.16) nA, GaussSTD = 6 x mean)")
     p.add_argument("--schedule", default="", help="json dict of fs_schedule overrides")
     p.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                    help="only run 10 device-to-device copies of MIB MiB (the known-size "
@@ -170,6 +173,11 @@ def main():
     if args.schedule:
         eng.set_schedule(**json.loads(args.schedule))
     eng.reset(2024)
+    if args.injection:
+        means = np.random.default_rng(4).uniform(0.04, 0.16, size=len(arr.pops))
+        for p, m in enumerate(means):
+            eng.set_injection(p, float(m), float(6.0 * m))
+        desc += "; Gaussian injection on every population (seizure protocol form)"
     if args.warmup:
         eng.run(args.warmup, record=False)
     eng.enable_timing(True)
@@ -225,13 +233,13 @@ def main():
                      "graph_step_ms": kt["graph_step_ms"],
                      "timed_launches": kt["n_timed"]},
     }
-    traffic = measured_traffic(args.workload, dom, res["dtype"])
+    traffic = None if args.injection else measured_traffic(args.workload, dom, res["dtype"])
     if traffic:
         res["roofline"]["traffic"] = traffic
     eng.close()
     if not args.no_sweep and args.workload == "fly20k":
         res["sweep"] = sweep_figures()
-    if not args.no_cpu:
+    if not args.no_cpu and not args.injection:  # the CPU legs run the workload without injection
         res["cpu_baseline"] = cpu_baseline(net, arr, ctr["syn_events"] / sim_s, args.cpu_steps)
         res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
     print(json.dumps(res))

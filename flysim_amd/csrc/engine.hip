@@ -99,6 +99,7 @@ struct Engine final : EngineBase {
   bool pops_contiguous = true;
   uint32_t zig_calls = 0, zig_a = 0;
   int64_t zig_t0 = 0;
+  bool zig_dirty = false;  // injection switched since the last refresh
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
   bool fused() const { return fused_ok && sched.fused && !window(); }
   // windowed path: steps per launch, at most 1 + the smallest synaptic delay
@@ -191,6 +192,20 @@ struct Engine final : EngineBase {
       z.fn[i] = (float)std::exp(-0.5 * dn * dn);
       z.wn[i] = (float)(dn / m1);
     }
+    // jump tables of jcong' = 69069 jcong + 1234567 (code/StiExt.h:222-229):
+    // entry [j][b] advances b * 256^j calls
+    uint32_t a = 69069u, c = 1234567u;  // one call of 256^j
+    for (int j = 0; j < 4; ++j) {
+      uint32_t pa = 1u, pc = 0u;
+      for (int b = 0; b < 256; ++b) {
+        z.ja[j][b] = pa;
+        z.jc[j][b] = pc;
+        pc = a * pc + c;
+        pa = a * pa;
+      }
+      a = pa;  // 256 calls of 256^j
+      c = pc;
+    }
     return z;
   }
 
@@ -201,6 +216,7 @@ struct Engine final : EngineBase {
   // the current step are banked, and each population gets the offsets that
   // make neuron n's call at step t equal t * zig_a + n + zig_b (mod 2^32).
   void zig_refresh() {
+    zig_dirty = false;
     if (net.ext_mode != FS_EXT_REPLAY) return;
     zig_calls += (uint32_t)(step - zig_t0) * zig_a;
     zig_t0 = step;
@@ -1009,11 +1025,14 @@ struct Engine final : EngineBase {
     if (g < 0 || g >= P * B) throw std::invalid_argument("population index out of range");
     const int32_t rep = g / P, i = g - rep * P;
     const bool inj_switch = ((hpop[i].act ^ p.act) & FS_ACT_INJ) != 0;
+    if (inj_switch && (p.act & FS_ACT_INJ) && net.ext_mode == FS_EXT_REPLAY && (B != 1 || !pops_contiguous))
+      throw std::invalid_argument(
+          "replay-mode Gaussian injection needs one instance whose populations are contiguous neuron ranges");
     hpop[i] = p;
     dpop[i] = to_dev(p, dpop[i]);
     HIPCHECK(hipMemcpyAsync(b_pops.as<fsk::DevPop<T>>() + i, &dpop[i], sizeof(fsk::DevPop<T>),
                             hipMemcpyHostToDevice, stream));
-    if (inj_switch) zig_refresh();
+    if (inj_switch) zig_dirty = true;  // refreshed once before the next step (events come in bulk)
     pbr_from_pop(i, rep, p);
     for (int r = 0; r < 5; ++r) note_pb(r, pbr(r, i, rep));
     tables_dirty = true;
@@ -1072,6 +1091,7 @@ struct Engine final : EngineBase {
       upload_replica_tables();
       tables_dirty = false;
     }
+    if (zig_dirty) zig_refresh();
     int64_t written = 0;
     if (off) off[0] = 0;
     t_upd = t_del = t_graph = 0;

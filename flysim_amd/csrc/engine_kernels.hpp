@@ -191,20 +191,20 @@ __device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * (1.0
 struct ZigTables {
   uint32_t kn[128];
   float fn[128], wn[128];
+  // the LCG advanced b * 256^j calls: x -> ja[j][b] x + jc[j][b]
+  uint32_t ja[4][256], jc[4][256];
 };
 
-__device__ __forceinline__ uint32_t lcg_jump(uint32_t seed, uint32_t m) {
-  uint32_t aa = 1u, cc = 0u, a = 69069u, c = 1234567u;  // x -> aa x + cc; step x -> a x + c
-  while (m) {
-    if (m & 1u) {
-      aa *= a;
-      cc = cc * a + c;
-    }
-    c = (a + 1u) * c;
-    a *= a;
-    m >>= 1;
+// the LCG state after m calls from seed: one table step per byte of m (the
+// powers of one affine map commute)
+__device__ __forceinline__ uint32_t lcg_jump(uint32_t seed, uint32_t m, const ZigTables *z) {
+  uint32_t s = seed;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t b = (m >> (8 * j)) & 255u;
+    s = z->ja[j][b] * s + z->jc[j][b];
   }
-  return aa * seed + cc;
+  return s;
 }
 
 // SHR3 step: returns the input plus the shifted state (shr3_seeded / r4_uni,
@@ -227,16 +227,21 @@ __device__ __forceinline__ float zig_uni(uint32_t &s) {
 // logf, rounded from the double logarithm
 __device__ __forceinline__ float zig_logf(float u) { return (float)log((double)u); }
 
+// |hz| < kn[iz], the reference's fabs(hz) < kn[iz] in exact integers
+__device__ __forceinline__ bool zig_inside(int32_t hz, uint32_t kn) {
+  const uint32_t a = hz < 0 ? 0u - (uint32_t)hz : (uint32_t)hz;
+  return a < kn;
+}
+
 // r4_nor (code/StiExt.h:262-319), Marsaglia and Tsang's 128-strip ziggurat
 // with r = 3.442620, in the reference's float / double mix and without
 // contraction.  The rejection loops end after 256 rounds (the expected count
-// is about 1.01), where the reference would keep drawing.
-__device__ __noinline__ float zig_normal(uint32_t jsr, const ZigTables *z) {
+// is about 1.01), where the reference would keep drawing.  Out of line: the
+// wedges and the tail (about 1 % of the draws); zig_normal takes the rest.
+__device__ __noinline__ float zig_normal_slow(int32_t hz, uint32_t iz, uint32_t jsr,
+                                              const ZigTables *z) {
 #pragma clang fp contract(off)
   const float r = 3.442620f;
-  int32_t hz = (int32_t)shr3_next(jsr);
-  uint32_t iz = (uint32_t)hz & 127u;
-  if (fabs((double)hz) < (double)z->kn[iz]) return (float)hz * z->wn[iz];
   for (int round = 0; round < 256; ++round) {
     if (iz == 0) {
       float x = 0.0f;
@@ -252,16 +257,23 @@ __device__ __noinline__ float zig_normal(uint32_t jsr, const ZigTables *z) {
     if ((double)f < exp(-0.5 * (double)x * (double)x)) return x;
     hz = (int32_t)shr3_next(jsr);
     iz = (uint32_t)hz & 127u;
-    if (fabs((double)hz) < (double)z->kn[iz]) return (float)hz * z->wn[iz];
+    if (zig_inside(hz, z->kn[iz])) return (float)hz * z->wn[iz];
   }
   return (float)hz * z->wn[iz];
+}
+
+__device__ __forceinline__ float zig_normal(uint32_t jsr, const ZigTables *z) {
+  const int32_t hz = (int32_t)shr3_next(jsr);
+  const uint32_t iz = (uint32_t)hz & 127u;
+  if (zig_inside(hz, z->kn[iz])) return (float)hz * z->wn[iz];
+  return zig_normal_slow(hz, iz, jsr, z);
 }
 
 // Igauss for call k of the stream seeded with seed: mean + sd * r4_nor
 __device__ __forceinline__ float zig_igauss(uint32_t seed, uint32_t k, float mean, float sd,
                                             const ZigTables *z) {
 #pragma clang fp contract(off)
-  const float v = zig_normal(lcg_jump(seed, k + 1u), z);
+  const float v = zig_normal(lcg_jump(seed, k + 1u, z), z);
   return mean + sd * v;
 }
 
