@@ -124,7 +124,7 @@ EXPORTS = [
     "fs_record_currents",
     "fs_kernel_times", "fs_run_counters", "fs_destroy", "fs_last_error", "fs_conf_load",
     "fs_conf_free", "fs_conf_desc", "fs_conf_sizes", "fs_conf_seed", "fs_conf_ext_trains",
-    "fs_conf_ref_layout", "fs_conf_write_outputs", "fs_cli_main", "fs_create_replicated",
+    "fs_conf_ref_layout", "fs_conf_write_outputs", "fs_conf_write_logs", "fs_cli_main", "fs_create_replicated",
     "fs_set_replica", "fs_create_ei_batch", "fs_set_ext_rates", "fs_mlp_create", "fs_mlp_destroy", "fs_mlp_n_params", "fs_mlp_get_params",
     "fs_mlp_set_params", "fs_mlp_loss_grad", "fs_mlp_predict", "fs_mlp_set_data", "fs_mlp_train",
     "fs_device_alloc", "fs_device_free", "fs_device_copy",
@@ -182,6 +182,7 @@ def lib() -> C.CDLL:
             "fs_conf_ref_layout": [C.c_void_p, P(RefLayout)],
             "fs_conf_write_outputs": [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
                                       C.c_char_p],
+            "fs_conf_write_logs": [C.c_void_p, C.c_char_p],
             "fs_cli_main": [C.c_int, P(C.c_char_p)],
             "fs_mlp_create": [C.c_void_p, C.c_int32, C.c_uint64, P(C.c_void_p)],
             "fs_mlp_destroy": [C.c_void_p],

@@ -118,6 +118,8 @@ int cli_main(int argc, const char *const *argv) {
   const Event &last = c->cfg.events.back();
   if (last.nolimit) return set_error(FS_ERR_ARG, "EventTime NoLimit is not supported");
   const uint64_t endtime = (uint64_t)std::floor(last.t_s * 1e3f / dt);  // code/NetSimGnl2.cpp:270
+  rc = fs_conf_write_logs(c, "");  // ConfsInfo.log and network.log (code/main.cpp:247, NetSimGnl2.cpp:154)
+  if (rc != FS_OK) return rc;
   std::cout << "\ncapable mode of sim07_21 leaky integrate and fire model\n"
             << "time step:" << dt << ", total neurons=" << c->net.n_neuron
             << ", total synapses=" << c->net.total_synapses << "\n";

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "logs.hpp"
 #include "builder.hpp"
 #include "conf_state.hpp"
 #include "events.hpp"
@@ -204,6 +205,17 @@ int fs_conf_write_outputs(fs_conf *c, int32_t repeat, int64_t n_steps, const int
       if (ids[q] >= c->net.n_neuron) return fs::set_error(FS_ERR_ARG, "spike id out of range");
     w.step((uint64_t)k, ids + offsets[k], offsets[k + 1] - offsets[k], none);
   }
+  return FS_OK;
+  FS_TRY_END
+}
+
+int fs_conf_write_logs(fs_conf *c, const char *dir) {
+  FS_TRY_BEGIN
+  if (!c) return fs::set_error(FS_ERR_ARG, "null handle");
+  std::string d = dir ? dir : "";
+  if (!d.empty() && d.back() != '/') d += '/';
+  fs::write_confs_info(c->cfg, d + "ConfsInfo.log");
+  fs::write_network_log(c->cfg, c->net, c->opt, d + "network.log");
   return FS_OK;
   FS_TRY_END
 }

@@ -179,6 +179,10 @@ class Conf:
         A.check(A.lib().fs_conf_write_outputs(self._h, repeat, len(off) - 1, off.ctypes.data,
                                               idv.ctypes.data, directory.encode()))
 
+    def write_logs(self, directory: str) -> None:
+        """Write ConfsInfo.log and network.log into ``directory``."""
+        A.check(A.lib().fs_conf_write_logs(self._h, directory.encode()))
+
     def ref_layout(self) -> Dict[str, np.ndarray]:
         L = A.RefLayout()
         A.check(A.lib().fs_conf_ref_layout(self._h, C.byref(L)))

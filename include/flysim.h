@@ -294,6 +294,13 @@ int fs_conf_seed(fs_conf *c, int32_t repeat, uint32_t *seed);
  * and the ActCtr changes of DoEvt). */
 int fs_conf_ext_trains(fs_conf *c, int32_t repeat, int64_t n_steps, uint32_t *bits);
 
+/* ConfsInfo.log and network.log into dir ("" = current directory), as the
+ * reference writes them before it simulates: Parser::ConfsInfo
+ * (code/parser.cpp:1488-1625, called from code/main.cpp:247) and
+ * NetSimGnl2::logDataWrite (code/NetSimGnl2_info.cpp:93-541, called from
+ * RunInit, code/NetSimGnl2.cpp:154). */
+int fs_conf_write_logs(fs_conf *c, const char *dir);
+
 /* Writes the Spike / FiringRate output files of the protocol (FileOut,
  * code/NetSimGnl2_info.cpp:624-699) for a given spike raster, in the
  * fs_run offsets/ids layout, into directory dir (NULL: current directory). */

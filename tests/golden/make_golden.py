@@ -5,7 +5,7 @@ Runs only where the reference sources exist (this build container).  For
 each case it writes the network in the reference's text format, runs the
 reference with `-t 1 -rp 1` and keeps only small outputs: spike rasters as
 (step, neuron) pairs, synapse orders from network.log, the bytes of small
-output files, and digests.  Usage (repo root):  python tests/golden/make_golden.py [plasticity|noise]
+output files, and digests.  Usage (repo root):  python tests/golden/make_golden.py [plasticity|noise|logs [dir]]
 """
 from __future__ import annotations
 
@@ -418,6 +418,46 @@ def seizure_case_1k(store, files, seed=4, g_ext=0.8, t_ms=300.0):
             "events": len(ev)}
 
 
+LOG_CASES = [("module", "module_network.conf", "module_network.pro", ["-t", "1"]),
+             ("outputs", "outputs_network.conf", "outputs_network.pro", ["-t", "1"]),
+             ("stp", "stp_network.conf", "module_network.pro", ["-t", "1", "-STP"]),
+             ("udf", "udf_network.conf", "module_network.pro", ["-t", "1", "-udfsed", "4242"]),
+             ("ltp", "ltp_network.conf", "module_network.pro", ["-t", "1", "-LTP"]),
+             ("gap", "gap_network.conf", "module_network.pro", ["-t", "1"]),
+             ("noise2", "module_network.conf", "noise2_network.pro", ["-t", "1", "-rp", "2"])]
+
+
+def logs_case(files, keep_dir=None):
+    """ConfsInfo.log and network.log of the reference built from source for
+    stored confs and protocols: digest, size and line count of each."""
+    out = {}
+    for name, conf, pro, args in LOG_CASES:
+        d = tempfile.mkdtemp(prefix="golden_")
+        open(os.path.join(d, "network.conf"), "w").write(files[conf])
+        open(os.path.join(d, "network.pro"), "w").write(files[pro])
+        run_ref(d, args, wait_for="start processing")
+        out[name] = {"args": args, "conf": conf, "pro": pro}
+        for fn in ["ConfsInfo.log", "network.log"]:
+            b = open(os.path.join(d, fn), "rb").read()
+            out[name][fn] = {"sha256": hashlib.sha256(b).hexdigest(), "bytes": len(b),
+                             "lines": b.count(b"\n")}
+            if keep_dir:
+                os.makedirs(keep_dir, exist_ok=True)
+                open(os.path.join(keep_dir, name + "_" + fn), "wb").write(b)
+        shutil.rmtree(d)
+    return out
+
+
+def update_logs(keep_dir=None):
+    build_ref()
+    meta = json.load(open(os.path.join(HERE, "reference_meta.json")))
+    files = json.load(open(os.path.join(HERE, "reference_files.json")))
+    meta["logs"] = logs_case(files, keep_dir)
+    with open(os.path.join(HERE, "reference_meta.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta["logs"], indent=1)[:1500])
+
+
 def noise_case(store, files):
     d = tempfile.mkdtemp(prefix="golden_")
     open(os.path.join(d, "network.conf"), "w").write(MODULE_CONF)
@@ -749,6 +789,7 @@ def main():
     meta["gap"] = gap_case(store, files)
     meta["c1_stp"] = c1_case("c1_stp", 1, 1.25, 1000.0, store, args=["-STP"])
     meta["ltp"] = ltp_case(store, files)
+    meta["logs"] = logs_case(files)
     sz = seizure_case()
     if sz:
         meta["seizure_pro"] = sz
@@ -765,5 +806,7 @@ if __name__ == "__main__":
         update_plasticity()
     elif sys.argv[1:] == ["noise"]:
         update_noise()
+    elif sys.argv[1:2] == ["logs"]:
+        update_logs(sys.argv[2] if len(sys.argv) > 2 else None)
     else:
         main()

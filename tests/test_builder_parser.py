@@ -203,3 +203,32 @@ def test_refractory_and_delay_conversions(tmp_path):
     L = c.ref_layout()
     assert L["rfped"].tolist() == [20, 19, 10, 18]
     assert L["spidy"].tolist() == [10, 3, 20, 0]
+
+
+def test_logs_byte_identical_to_reference(golden, tmp_path):
+    """ConfsInfo.log and network.log (per-neuron derived parameters, synapse
+    arrays in solver order, events, output headers) equal the files the
+    reference built from source writes for seven conf / protocol / switch
+    combinations (digests in the fixture)."""
+    files = golden["files"]
+    cases = golden["meta"]["logs"]
+    assert len(cases) == 7
+    for name, case in cases.items():
+        args = case["args"]
+        kw = {}
+        if "-STP" in args:
+            kw["stp"] = True
+        if "-LTP" in args:
+            kw["ltp"] = True
+        if "-udfsed" in args:
+            kw["udf_seed"] = int(args[args.index("-udfsed") + 1])
+        if "-rp" in args:
+            kw["repeats"] = int(args[args.index("-rp") + 1])
+        d = tmp_path / name
+        d.mkdir()
+        c = Conf(*_write(str(d), files[case["conf"]], files[case["pro"]]), **kw)
+        c.write_logs(str(d))
+        for fn in ["ConfsInfo.log", "network.log"]:
+            b = (d / fn).read_bytes()
+            assert len(b) == case[fn]["bytes"], (name, fn)
+            assert hashlib.sha256(b).hexdigest() == case[fn]["sha256"], (name, fn)

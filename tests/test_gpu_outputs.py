@@ -56,6 +56,11 @@ def test_cli_per_step_outputs_match_reference(golden, tmp_path):
             assert gv.shape == rv.shape, fn
             scale = np.max(np.abs(rv)) if rv.size else 0.0
             np.testing.assert_allclose(gv, rv, rtol=1e-4, atol=1e-5 * scale, err_msg=fn)
+    # the drop-in CLI also writes the reference's ConfsInfo.log and network.log
+    import hashlib
+    for fn in ["ConfsInfo.log", "network.log"]:
+        digest = hashlib.sha256((tmp_path / fn).read_bytes()).hexdigest()
+        assert digest == golden["meta"]["logs"]["outputs"][fn]["sha256"], fn
     # the gates file carries values (neuron 3 spikes within the run)
     gates = _sampled(files["outputs_Gate.dat"], 1)
     assert any(float(x) != 0.0 for line in gates for x in line.split()[1:])
