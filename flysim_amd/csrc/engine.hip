@@ -780,7 +780,15 @@ struct Engine final : EngineBase {
     // the windowed path when synapses have delays (several steps per
     // launch), with graphs of whole windows
     sched.window = win_max >= 2 ? 64 : 0;
-    if (sched.window) sched.steps_per_graph = 10 * std::min(sched.window, win_max);
+    if (sched.window) {
+      // 10..100 windows per graph: each batch ends in a host round trip
+      // (overflow check, spike readback), so long batches amortise it; the
+      // spike record (worst case every neuron every step) stays within 2^26
+      // entries
+      const int64_t w = std::min(sched.window, win_max);
+      const int64_t per = std::max<int64_t>(10, std::min<int64_t>(100, ((int64_t)1 << 26) / ((int64_t)N * w)));
+      sched.steps_per_graph = (int32_t)(per * w);
+    }
     apply_schedule();
     reset(0);
   }
