@@ -414,7 +414,10 @@ __device__ __forceinline__ void load_const(const Net<T> &net, PopC<T> &P, int32_
 // gn: NMDA conductance of the previous Spike phase; inc: ring_take's output.
 // Writes the NMDA gate history slot hslot (step t); everything else stays
 // in S.
-template <typename T, bool REPLAY>
+// WIN: the windowed path's instance, which never runs with LTP or gap
+// junctions (the host gives such networks no window), so their branches are
+// compiled out and leave the registers to the window's state.
+template <typename T, bool REPLAY, bool WIN = false>
 __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const NConst<T> &C,
                                             int32_t n, int64_t t, int32_t hslot, T gn,
                                             const T inc[4], NState<T> &S, bool &spk, T &a_push) {
@@ -433,7 +436,7 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
     g[rid] = gv;
   }
   if (net.has_nmda_syn) g[2] = gn;
-  if (net.ltp_mode) {  // single synapses, summed by k_learn with the weights before its update
+  if (!WIN && net.ltp_mode) {  // single synapses, summed by k_learn with the weights before its update
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
       T *gp = net.gl + (size_t)r * N + n;
@@ -490,7 +493,7 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
   }
   // Isyn: the gap current of the previous Spike phase, plus the Gaussian
   // injection drawn now (BgInit adds to Isyn, code/EqsGnl2.cpp:335)
-  T isyn = net.igap ? net.igap[n] : T(0);
+  T isyn = (!WIN && net.igap) ? net.igap[n] : T(0);
   if (act & ACT_INJ) {
     if (REPLAY) {
       // the reference's own ziggurat stream (single-thread order: neurons in
@@ -545,7 +548,7 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
     if (v > P.vth) {
       cs |= F_FIRE;
       firing();
-      if (net.ltp_mode) cs |= F_LPFIRE;
+      if (!WIN && net.ltp_mode) cs |= F_LPFIRE;
     }
   }
   spk = (cs & F_SPIKE) != 0;
@@ -591,7 +594,7 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
     const T d = S.sD;
     S.sD = spk ? d * (T(1) - P.stp_pv) : P.stp_dfD * d + P.stp_efD;
   }
-  if (net.ltp_mode) {
+  if (!WIN && net.ltp_mode) {
     // firLP: the LTP spike follows the onset after SpiLPDy steps
     // (code/EqsGnl2.cpp:254-272); ltpEul: LP trace (:274-280)
     uint32_t c = net.cntlp[n];
@@ -1184,7 +1187,7 @@ k_window(Net<T> net, int32_t k0, int32_t W) {
     bool spk = false;
     T a_push = T(0);
     if (act)
-      neuron_core<T, REPLAY>(net, *(PopC<T> *)Pp, C, n, t, slot_of(tm1, k, net.DN), gn, inc, S, spk,
+      neuron_core<T, REPLAY, true>(net, *(PopC<T> *)Pp, C, n, t, slot_of(tm1, k, net.DN), gn, inc, S, spk,
                              a_push);
     // the step's spikes of this block: wave ballots, one LDS count
     if (threadIdx.x == 0) s_cnt = 0u;
