@@ -13,8 +13,13 @@ OutputWriters::OutputWriters(const Config &cfg, const BuiltNet &net, int repeat,
   one_ms_ = (unsigned)(1u / cfg.dt);  // OneMsStep=(unsigned int)1/dt (code/NetSimGnl2.cpp:272)
   if (one_ms_ == 0) one_ms_ = 1;
   for (const OutCtrl &oc : cfg.outs) {
-    if (oc.type_id != OUT_SPIKE && oc.type_id != OUT_FRATE && oc.type_id != OUT_MEMPOT &&
-        oc.type_id != OUT_DVOVF && oc.type_id != OUT_ISYNSEP && oc.type_id != OUT_GATVAR) {
+    // SaveAllStates and SynapticWeight files are opened like every output
+    // file but stay empty: FileOut has nothing for them
+    // (code/NetSim.cpp:47-62, code/NetSimGnl2_info.cpp:760-761)
+    const bool empty_file = oc.type_id == OUT_SAVE || oc.type_id == OUT_SYNWGT;
+    if (!empty_file && oc.type_id != OUT_SPIKE && oc.type_id != OUT_FRATE &&
+        oc.type_id != OUT_MEMPOT && oc.type_id != OUT_DVOVF && oc.type_id != OUT_ISYNSEP &&
+        oc.type_id != OUT_GATVAR) {
       unsupported_.push_back(oc.type);
       continue;
     }

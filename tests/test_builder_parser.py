@@ -232,3 +232,25 @@ def test_logs_byte_identical_to_reference(golden, tmp_path):
             b = (d / fn).read_bytes()
             assert len(b) == case[fn]["bytes"], (name, fn)
             assert hashlib.sha256(b).hexdigest() == case[fn]["sha256"], (name, fn)
+
+
+def test_state_and_weight_outputs_are_empty_files(golden, tmp_path):
+    """SaveAllStates and SynapticWeight output files are opened and left
+    empty, as the reference's FileOut writes nothing for them; the Spike file
+    beside them is still written."""
+    f = golden["files"]
+    pro = f["module_network.pro"].replace(
+        "EndOutControl",
+        "FileName:States.dat\nType=SaveAllStates\npopulation:AllPopulation\nEndOutputFile\n"
+        "FileName:Weights.dat\nType=SynapticWeight\npopulation:AllPopulation\nEndOutputFile\n"
+        "EndOutControl")
+    c = Conf(*_write(str(tmp_path), f["module_network.conf"], pro))
+    r = golden["runs"]["module_raster"].astype(np.int64)
+    counts = np.bincount(r[:, 0], minlength=3000)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    out = tmp_path / "out"
+    out.mkdir()
+    c.write_outputs(off, r[:, 1].astype(np.uint32), str(out))
+    assert (out / "States.dat").read_bytes() == b""
+    assert (out / "Weights.dat").read_bytes() == b""
+    assert (out / "Spikes.dat").read_text() == f["module_Spikes.dat"]
