@@ -8,7 +8,7 @@ ziggurat there.  Bars: the drop-in command line in FP64 with replayed draws
 reproduces the reference rasters of two injection protocols exactly (the
 second switches injection on and off for populations in and out of neuron
 order); 4000 injected neurons keep the FP64 oracle's potentials to 1e-6 mV
-over 300 ms (12 M draws).
+over 450 ms (18 M draws).
 
 The counter-based mode draws with Philox + Box-Muller instead, so its bar is
 statistical: after 300 ms of noise-only drive the membrane potentials of 4000
@@ -54,7 +54,7 @@ Population: Drive
 GaussMean=0.05
 GaussSTD=0.4
 EndEvent
-EventTime 300.0
+EventTime 450.0
 Type=EndTrial
 EndEvent
 OutControl
@@ -94,7 +94,9 @@ def test_injection_membrane_distribution_matches_oracle():
 
 
 def test_injection_replay_follows_the_reference_stream():
-    steps = 3000
+    # 4000 x 4500 = 18 M draws: call indices past 2^24 use every byte of the
+    # LCG jump tables
+    steps = 4500
     with tempfile.TemporaryDirectory() as d:
         open(os.path.join(d, "network.conf"), "w").write(CONF)
         open(os.path.join(d, "network.pro"), "w").write(PRO)
