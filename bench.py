@@ -8,8 +8,10 @@ of the whole network (LIF update + spike delivery + Poisson input).
 
 Prints one JSON line: synaptic events/s (whole job), sim-s/wall-s, the
 roofline of the dominant kernel (HIP events on the engine's stream over the
-timed region) and the CPU baseline (the CPU restatement timed on this host's
-cores over a bounded sample of the same network).
+timed region) and the CPU baseline (the reference built from source, timed on
+this host's cores over a bounded sample of the same network).  --steps and
+--warmup are floors: the timed run covers at least the simulated time the
+workload's configuration states (2 s for the fly-like net).
 """
 from __future__ import annotations
 
@@ -38,7 +40,7 @@ def parse():
     p.add_argument("--workload", default="fly20k", choices=["fly20k", "ei1k", "sparse1m"])
     p.add_argument("--fp64", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=-1,
-                   help="steps of the CPU-baseline sample (-1: sized to ~15 s)")
+                   help="steps of the CPU-baseline sample (-1: 1,000 steps, 100 ms simulated)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-sweep", action="store_true", help="skip the sweep points/s figures")
     p.add_argument("--injection", action="store_true",
@@ -74,10 +76,32 @@ def make_network(name):
                                   "delays 1-5 ms, Poisson 2 kHz, dt 0.1 ms")
 
 
+def host_cpu():
+    """Logical CPUs this process may use and the host's CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": usable, "cpu_model": model}
+
+
 def cpu_baseline(net, arr, events_per_sim_s, n_steps):
     """The reference built from source (oracle/_ref/flysim_ref) timed on this
-    host's cores on the module form of the same network; the CPU restatement
-    when the reference binary is absent."""
+    host's cores on the same network written in its single-neuron-population
+    form (every synapse its own single synapse with its own efficacy, the
+    form of its fly-brain experiment); the CPU restatement when the reference
+    binary is absent.  Step-loop time from the reference's own 'real time='
+    line (code/NetSimGnl2.cpp:351-354), best of 1 thread and this process's
+    CPU share (at most 16)."""
     import re
     import subprocess
     import tempfile
@@ -85,34 +109,42 @@ def cpu_baseline(net, arr, events_per_sim_s, n_steps):
     if not os.path.exists(ref):
         return cpu_baseline_port(net, arr, events_per_sim_s, n_steps)
     from flysim_amd import workloads as W
-    cores = min(16, os.cpu_count() or 1)
-    sim_ms = 200.0 if n_steps < 0 else n_steps * net.dt
+    host = host_cpu()
+    cores = max(1, min(16, host["nproc"]))
+    sim_ms = 100.0 if n_steps < 0 else n_steps * net.dt
+    single = W.single_neuron_form(net)
+    runs = {}
     with tempfile.TemporaryDirectory() as d:
-        info = W.write_module_conf(net, os.path.join(d, "network.conf"))
+        W.write_conf(single, os.path.join(d, "network.conf"))
         W.write_pro(os.path.join(d, "network.pro"), sim_ms, seeds=[2], spike_file=None)
-        best = None
         for threads in sorted({1, cores}):
+            t0 = time.perf_counter()
             out = subprocess.run([ref, "-conf", "network.conf", "-pro", "network.pro", "-t",
                                   str(threads)], cwd=d, capture_output=True, text=True,
                                  timeout=900).stdout.replace("\r", "\n")
+            total = time.perf_counter() - t0
             m = re.findall(r"real time=([0-9.eE+-]+)s 100%", out)
             if not m:
                 raise RuntimeError("reference run produced no timing line")
-            wall = float(m[-1])
-            if best is None or wall < best[1]:
-                best = (threads, wall)
-    threads, wall = best
+            runs[threads] = (float(m[-1]), total)
+    threads = min(runs, key=lambda t: runs[t][0])
+    wall = runs[threads][0]
     steps = int(np.floor(np.float32(sim_ms) * np.float32(1e-3) * np.float32(1e3) / np.float32(net.dt)))
     sim_per_wall = steps * net.dt * 1e-3 / wall
     return {"value": sim_per_wall * events_per_sim_s, "unit": "synaptic events/s",
             "cores": threads, "kind": "reference", "sim_s_per_wall_s": sim_per_wall,
-            "sample": "reference flysim built from source, -t %d (best of 1 and %d threads), "
-                      "%d steps (%.0f ms simulated) of the module form of the workload "
-                      "(%d population links, %d synapses, SpikeDly %.2f ms); step-loop time "
-                      "from its own 'real time=' line; events/s = GPU-measured events per "
-                      "simulated second x CPU sim-s/wall-s"
-                      % (threads, cores, steps, sim_ms, info["links"], info["approx_synapses"],
-                         info["spike_delay_ms"])}
+            "host_nproc": host["nproc"], "cpu_model": host["cpu_model"],
+            "step_loop_s": {str(t): runs[t][0] for t in runs},
+            "process_s": {str(t): round(runs[t][1], 2) for t in runs},
+            "sample": "reference flysim built from source, -t %d (best of %s threads), %d steps "
+                      "(%.0f ms simulated) of the same network (%d neurons, %d receptor-synapses, "
+                      "same efficacies and drives) as single-neuron populations; per-synapse "
+                      "delays become SpikeDly %.2f ms (the format has no per-synapse delay; its "
+                      "per-step cost does not depend on delays); step-loop time from its own "
+                      "'real time=' line; events/s = GPU-measured events per simulated second x "
+                      "CPU sim-s/wall-s"
+                      % (threads, "/".join(str(t) for t in sorted(runs)), steps, sim_ms, net.n,
+                         net.n_syn, single.notes["spike_delay_ms"])}
 
 
 def cpu_baseline_port(net, arr, events_per_sim_s, n_steps):
@@ -155,6 +187,75 @@ def cpu_baseline_port(net, arr, events_per_sim_s, n_steps):
                       "second x CPU sim-s/wall-s" % (n_steps, n_steps * net.dt * 1e-3, cores)}
 
 
+# Timed steps and warm-up steps never go below the simulated time the
+# workload's configuration states (2 s for the fly-like net, 1 s for the
+# others) and a tenth of it: shorter runs start from rest and fire nothing.
+FLOORS = {"fly20k": (20000, 2000), "ei1k": (10000, 1000), "sparse1m": (10000, 1000)}
+
+
+def roofline_of(kt, ctr, sched, fused, n, steps, fp64):
+    """The dominant kernel, its algorithmic bytes per launch and its average
+    launch duration.  The LIF update moves 84 B per neuron-step (160 B in
+    FP64); delivery 8 B per spike (row pointer) + 17 B per pushed synapse
+    (column, weight, delay, atomic read-modify-write); the NMDA gather 12 B
+    per term + 8 B per row."""
+    upd_bytes = (UPDATE_BYTES_FP64 if fp64 else UPDATE_BYTES_FP32) * n
+    del_bytes = (8 * ctr["spikes"] + 17 * ctr["lin_pushes"] + 12 * ctr["nmda_terms"]
+                 + 8 * n * steps) / max(steps, 1)
+    upd_ms, del_ms = kt["update_ms"], kt["deliver_ms"]
+    win = sched.get("window", 0)
+    if win:
+        # W steps per launch; graphs hold whole windows, so a launch lasts
+        # W graph steps
+        dom, b = "k_window", (upd_bytes + del_bytes) * win
+        ms = kt["graph_step_ms"] * win if kt["graph_step_ms"] > 0 else upd_ms
+    elif fused:
+        dom, b = "k_step", upd_bytes + del_bytes
+        ms = kt["graph_step_ms"] if kt["graph_step_ms"] > 0 else upd_ms
+    elif del_ms >= upd_ms:
+        dom, b, ms = "k_deliver", del_bytes, del_ms
+    else:
+        dom, b, ms = "k_update", upd_bytes, upd_ms
+    achieved = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_launch": b, "avg_launch_ms": ms, "update_ms": upd_ms,
+            "deliver_ms": del_ms, "graph_step_ms": kt["graph_step_ms"],
+            "timed_launches": kt["n_timed"]}
+
+
+def assemble(args, steps, warmup, wall, ctr, kt, sched, fused, net_n, n_syn, pairs, dt,
+             desc):
+    """The bench line (without the CPU and sweep legs): whole-job synaptic
+    events per second over the timed steps, with inputs resident in HBM."""
+    sim_s = steps * dt * 1e-3
+    wall = max(wall, 1e-12)
+    return {
+        "metric": METRIC, "value": ctr["syn_events"] / wall, "unit": "synaptic events/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "steps_requested": args.steps, "warmup_requested": args.warmup,
+        "ms_per_step": wall / max(steps, 1) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.fp64 else "f32", "data": "synthetic",
+        "config": {"workload": desc, "neurons": net_n, "receptor_synapses": n_syn,
+                   "pairs": pairs, "sim_seconds": sim_s, "schedule": sched},
+        "sim_s_per_wall_s": sim_s / wall,
+        "spikes_per_s": ctr["spikes"] / wall,
+        "mean_rate_hz": ctr["spikes"] / max(net_n, 1) / sim_s if sim_s > 0 else 0.0,
+        "roofline": roofline_of(kt, ctr, sched, fused, net_n, steps, args.fp64),
+    }
+
+
+def add_cpu_leg(res, cpu):
+    """Attach the CPU baseline; the GPU/CPU ratio only when both are positive."""
+    res["cpu_baseline"] = cpu
+    v = cpu.get("value") or 0.0
+    res["speedup_vs_cpu"] = res["value"] / v if v > 0 and res["value"] > 0 else None
+    s = cpu.get("sim_s_per_wall_s") or 0.0
+    res["sim_speedup_vs_cpu"] = res["sim_s_per_wall_s"] / s if s > 0 else None
+    return res
+
+
 def main():
     args = parse()
     if args.copy_calib:
@@ -166,6 +267,8 @@ def main():
         sys.exit(2)
     from flysim_amd import _capi as A
     from flysim_amd.engine import Engine, arrays_from_network
+    steps = max(args.steps, FLOORS[args.workload][0])
+    warmup = max(args.warmup, FLOORS[args.workload][1])
     net, desc = make_network(args.workload)
     arr = arrays_from_network(net)
     prec = A.FS_FP64 if args.fp64 else A.FS_FP32
@@ -178,70 +281,24 @@ def main():
         for p, m in enumerate(means):
             eng.set_injection(p, float(m), float(6.0 * m))
         desc += "; Gaussian injection on every population (seizure protocol form)"
-    if args.warmup:
-        eng.run(args.warmup, record=False)
+    eng.run(warmup, record=False)
     eng.enable_timing(True)
     t0 = time.perf_counter()
-    eng.run(args.steps, record=False)
+    eng.run(steps, record=False)
     wall = time.perf_counter() - t0
     ctr = eng.counters()
-    kt = eng.kernel_times()
-    sim_s = args.steps * net.dt * 1e-3
-    # synaptic events: every receptor-synapse a spike reaches (linear pushes
-    # plus the NMDA synapses of the spiking neurons)
-    ev_value = ctr["syn_events"] / wall
-    # roofline of the dominant kernel (algorithmic bytes per launch): the LIF
-    # update moves 84 B per neuron-step (160 B in FP64); delivery moves 8 B per
-    # spike (row pointer) + 17 B per pushed synapse (col, weight, delay, atomic
-    # read-modify-write) and the NMDA gather 12 B per term + 8 B per row
-    n_launch = args.steps
-    upd_bytes = (UPDATE_BYTES_FP64 if args.fp64 else UPDATE_BYTES_FP32) * net.n
-    del_bytes = (8 * ctr["spikes"] + 17 * ctr["lin_pushes"] + 12 * ctr["nmda_terms"]
-                 + 8 * net.n * n_launch) / n_launch
-    upd_ms, del_ms = kt["update_ms"], kt["deliver_ms"]
-    win = eng.schedule.get("window", 0)
-    if win:
-        # W steps per launch; graphs hold whole windows (steps_per_graph a
-        # multiple of W), so a launch lasts W graph steps
-        dom, b = "k_window", (upd_bytes + del_bytes) * win
-        ms = kt["graph_step_ms"] * win if kt["graph_step_ms"] > 0 else upd_ms
-    elif eng.fused:
-        # one kernel per step: its launch time is the graph time per step
-        # (the eager first step of a batch also carries its dispatch latency)
-        dom, b = "k_step", upd_bytes + del_bytes
-        ms = kt["graph_step_ms"] if kt["graph_step_ms"] > 0 else upd_ms
-    elif del_ms >= upd_ms:
-        dom, b, ms = "k_deliver", del_bytes, del_ms
-    else:
-        dom, b, ms = "k_update", upd_bytes, upd_ms
-    achieved = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    res = {
-        "metric": METRIC, "value": ev_value, "unit": "synaptic events/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.fp64 else "f32", "data": "synthetic",
-        "config": {"workload": desc, "neurons": net.n, "receptor_synapses": net.n_syn,
-                   "pairs": int(net.notes.get("pairs", net.n_syn)),
-                   "sim_seconds": sim_s, "schedule": eng.schedule},
-        "sim_s_per_wall_s": sim_s / wall,
-        "spikes_per_s": ctr["spikes"] / wall,
-        "mean_rate_hz": ctr["spikes"] / net.n / sim_s,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": b, "avg_launch_ms": ms,
-                     "update_ms": upd_ms, "deliver_ms": del_ms,
-                     "graph_step_ms": kt["graph_step_ms"],
-                     "timed_launches": kt["n_timed"]},
-    }
-    traffic = None if args.injection else measured_traffic(args.workload, dom, res["dtype"])
+    res = assemble(args, steps, warmup, wall, ctr, eng.kernel_times(), eng.schedule, eng.fused,
+                   net.n, net.n_syn, int(net.notes.get("pairs", net.n_syn)), net.dt, desc)
+    traffic = None if args.injection else measured_traffic(args.workload, res["roofline"]["kernel"],
+                                                          res["dtype"])
     if traffic:
         res["roofline"]["traffic"] = traffic
     eng.close()
     if not args.no_sweep and args.workload == "fly20k":
         res["sweep"] = sweep_figures()
     if not args.no_cpu and not args.injection:  # the CPU legs run the workload without injection
-        res["cpu_baseline"] = cpu_baseline(net, arr, ctr["syn_events"] / sim_s, args.cpu_steps)
-        res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+        sim_s = steps * net.dt * 1e-3
+        add_cpu_leg(res, cpu_baseline(net, arr, ctr["syn_events"] / sim_s, args.cpu_steps))
     print(json.dumps(res))
 
 

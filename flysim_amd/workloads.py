@@ -336,6 +336,30 @@ def write_conf(net: Network, path: str) -> None:
         f.write("".join(out))
 
 
+def single_neuron_form(net: Network, spike_delay_ms: Optional[float] = None) -> Network:
+    """The same neurons, synapses, efficacies and drives as `net`, with every
+    neuron its own population ("N<id>", parameters of its population), the
+    form the reference's fly-brain experiment uses (one population per
+    neuron, experiment/seizure/network1_900_10nOvf.pro) and the one in which
+    every link is a single synapse with its own efficacy
+    (code/NetSimGnl2_BulPp.cpp:571-695).  Per-synapse delays have no
+    counterpart there: they become one SpikeDly for every neuron (their mean
+    when `spike_delay_ms` is None); the reference's cost per step does not
+    depend on delays (it updates every synapse every step)."""
+    off = net.pop_offsets()
+    if spike_delay_ms is None:
+        spike_delay_ms = float(np.mean(net.delay)) * net.dt if net.n_syn else 0.0
+    pops = []
+    for i, p in enumerate(net.pops):
+        for k in range(p.n):
+            pops.append(dataclasses.replace(p, name="N%d" % (off[i] + k), n=1,
+                                            spike_delay_ms=float(spike_delay_ms)))
+    return Network(name=net.name + "-single", pops=pops, pre=net.pre, post=net.post, w=net.w,
+                   receptor=net.receptor, delay=np.zeros(net.n_syn, np.uint8), dt=net.dt,
+                   t_end_ms=net.t_end_ms, seed=net.seed,
+                   notes=dict(net.notes, spike_delay_ms=float(spike_delay_ms)))
+
+
 def seizure_events(names: Sequence[str], seed: int, onset_ms: float = 1.0,
                    mean_lo: float = 0.04, mean_hi: float = 0.16,
                    std_ratio: float = 6.0) -> List[str]:
