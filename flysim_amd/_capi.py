@@ -126,7 +126,8 @@ EXPORTS = [
     "fs_conf_free", "fs_conf_desc", "fs_conf_sizes", "fs_conf_seed", "fs_conf_ext_trains",
     "fs_conf_ref_layout", "fs_conf_write_outputs", "fs_conf_write_logs", "fs_cli_main", "fs_create_replicated",
     "fs_set_replica", "fs_create_ei_batch", "fs_set_ext_rates", "fs_mlp_create", "fs_mlp_destroy", "fs_mlp_n_params", "fs_mlp_get_params",
-    "fs_mlp_set_params", "fs_mlp_loss_grad", "fs_mlp_predict", "fs_mlp_set_data", "fs_mlp_train",
+    "fs_mlp_set_params", "fs_mlp_loss_grad", "fs_mlp_predict", "fs_mlp_predict_noalloc",
+    "fs_mlp_set_data", "fs_mlp_train",
     "fs_device_alloc", "fs_device_free", "fs_device_copy",
 ]
 
@@ -193,6 +194,7 @@ def lib() -> C.CDLL:
                                  C.c_void_p],
             "fs_mlp_predict": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32,
                                P(C.c_double)],
+            "fs_mlp_predict_noalloc": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
             "fs_mlp_set_data": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64],
             "fs_mlp_train": [C.c_void_p, C.c_int32, C.c_int32, P(Adam), C.c_uint64, C.c_int32,
                              C.c_void_p],

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <stdexcept>
@@ -64,6 +65,33 @@ struct Buf {
 
 using fsm::bf16;
 
+// Allocation-free forward (fs_mlp_predict_noalloc): the packed input rows and
+// two ping-pong hidden activations of one chunk of rows live in this fixed
+// device array.  Each GPU the library runs on holds its own copy until the
+// process ends; host code uses only the per-device address resolved below,
+// never the array's name.
+constexpr size_t NOALLOC_BYTES = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+// fs_mlp_create admits widths up to 8192, so one 64-row chunk always fits
+static_assert(64 * 2 * (8192 + 2 * 8192) <= NOALLOC_BYTES, "scratch below one chunk of the widest net");
+
+// Device address of `scratch` on the current device, resolved once per device
+// (at fs_mlp_create, with that device current).
+void *noalloc_scratch_current_device() {
+  static std::mutex mu;
+  static std::vector<void *> ptrs;
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((size_t)dev >= ptrs.size()) ptrs.resize((size_t)dev + 1, nullptr);
+  if (!ptrs[dev]) {
+    void *p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptrs[dev] = p;
+  }
+  return ptrs[dev];
+}
+
 inline int32_t pad64(int32_t x) { return (x + 63) / 64 * 64; }
 
 struct Mlp {
@@ -84,6 +112,7 @@ struct Mlp {
   int32_t graph_rows = -1;
   fs_adam graph_opt{};
   hipEvent_t ev[2] = {nullptr, nullptr};
+  void *noalloc = nullptr;        // this device's address of `scratch`
 
   Mlp(const int32_t *sizes, int32_t n_layers, uint64_t seed) {
     if (n_layers < 1 || n_layers > 16) throw std::invalid_argument("1..16 layers");
@@ -106,6 +135,7 @@ struct Mlp {
     }
     HIPCHECK(hipStreamCreate(&stream));
     for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+    noalloc = noalloc_scratch_current_device();
     p32.alloc(n_params * 4);
     grad.alloc(n_params * 4);
     m1.alloc(n_params * 4);
@@ -296,6 +326,53 @@ struct Mlp {
     return ms;
   }
 
+  // Rows of one chunk of the allocation-free forward: the packed input
+  // (in_pad bf16) and two hidden buffers (widest hidden layer, bf16) per row
+  // in NOALLOC_BYTES, a multiple of the 64-row GEMM tile; 0 if 64 rows do
+  // not fit.
+  int64_t noalloc_rows() const {
+    int64_t maxw = 0;
+    for (int i = 1; i < L; ++i) maxw = std::max<int64_t>(maxw, sp[i]);
+    const int64_t per_row = 2 * ((int64_t)sp[0] + 2 * maxw);
+    return (int64_t)NOALLOC_BYTES / per_row / 64 * 64;
+  }
+
+  // x (f32 [n][in]) and y (f32 [n][out]) in device memory; no allocation:
+  // every intermediate lives in this device's `scratch`.  The GEMMs read only
+  // the unpadded inner dimension (K = width), so nothing depends on what an
+  // earlier call left in the padding columns.
+  void predict_noalloc(const float *x, int64_t n, float *y) {
+    const int64_t R = noalloc_rows();
+    if (R < 64)
+      throw std::length_error("layer widths need more than the 8 MiB allocation-free scratch "
+                              "for one 64-row chunk");
+    int64_t maxw = 0;
+    for (int i = 1; i < L; ++i) maxw = std::max<int64_t>(maxw, sp[i]);
+    bf16 *x16s = static_cast<bf16 *>(noalloc);
+    bf16 *hbuf[2] = {x16s + R * sp[0], x16s + R * sp[0] + R * maxw};
+    for (int64_t r0 = 0; r0 < n; r0 += R) {
+      const int32_t rows = (int32_t)std::min<int64_t>(R, n - r0);
+      const int blocks = std::max(1, std::min(8192, (int)(((int64_t)rows * sp[0] + 255) / 256)));
+      hipLaunchKernelGGL(fsm::k_pack_rows, dim3(blocks), dim3(256), 0, stream, x + r0 * s[0],
+                         (const int64_t *)nullptr, (const int64_t *)nullptr, (int64_t)rows, s[0], sp[0],
+                         x16s);
+      HIPCHECK(hipGetLastError());
+      const bf16 *in = x16s;
+      for (int i = 0; i < L; ++i) {
+        if (i < L - 1) {
+          bf16 *out = hbuf[i & 1];
+          gemm(in, sp[i], 1, W16(i), 1, sp[i], rows, sp[i + 1], s[i], fsm::EPI_BIAS_GELU, s[i + 1],
+               P32(offb[i]), nullptr, 0, out, sp[i + 1], nullptr, 0);
+          in = out;
+        } else {
+          gemm(in, sp[i], 1, W16(i), 1, sp[i], rows, sp[i + 1], s[i], fsm::EPI_BIAS, s[i + 1],
+               P32(offb[i]), y + r0 * s[L], s[L], nullptr, 0, nullptr, 0);
+        }
+      }
+    }
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
   void set_data(const float *x, const float *y, int64_t n) {
     if (n < 1) throw std::invalid_argument("empty training set");
     xall.alloc((size_t)n * s[0] * 4);
@@ -375,6 +452,8 @@ struct fs_mlp {
     return FS_OK;                                                       \
   } catch (const HipError &ex) {                                        \
     return fs::set_error(FS_ERR_HIP, ex.what());                        \
+  } catch (const std::length_error &ex) {                               \
+    return fs::set_error(FS_ERR_CAPACITY, ex.what());                   \
   } catch (const std::bad_alloc &) {                                    \
     return fs::set_error(FS_ERR_CAPACITY, "host allocation failed");    \
   } catch (const std::exception &ex) {                                  \
@@ -424,6 +503,11 @@ int fs_mlp_predict(fs_mlp *h, const float *x, int64_t n, float *y, int32_t devic
     const double t = n ? h->m->predict(x, n, y, device_ptrs != 0) : 0.0;
     if (ms) *ms = t;
   })
+}
+
+int fs_mlp_predict_noalloc(fs_mlp *h, const float *x, int64_t n, float *y) {
+  if (!h || !x || !y || n < 0) return fs::set_error(FS_ERR_ARG, "bad argument");
+  MLP_CALL(if (n) h->m->predict_noalloc(x, n, y))
 }
 
 int fs_mlp_set_data(fs_mlp *h, const float *x, const float *y, int64_t n) {

@@ -43,6 +43,19 @@ int fs_mlp_loss_grad(fs_mlp *m, const float *x, const float *y, int32_t rows, fl
  * packing + forward kernels measured with HIP events. */
 int fs_mlp_predict(fs_mlp *m, const float *x, int64_t n, float *y, int32_t device_ptrs, double *ms);
 
+/* Allocation-free forward.  No counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  x (f32 [n][in]) and y
+ * (f32 [n][out]) are device pointers; the packed input and the hidden
+ * activations of each chunk of rows live in one fixed 8 MiB device array per
+ * GPU (resolved at fs_mlp_create), so the call allocates nothing.  Any n is
+ * processed in chunks of that array's size (a multiple of 64 rows; the
+ * widths fs_mlp_create admits always fit one chunk).  Output equals
+ * fs_mlp_predict's bit for bit.  Every call on one GPU shares the array and
+ * leaves its contents behind: calls that overlap on one GPU (other threads or
+ * other handles) race.  Returns after the GPU has finished. */
+int fs_mlp_predict_noalloc(fs_mlp *m, const float *x, int64_t n, float *y);
+
 /* Training set copied to the device (f32 [n][in], f32 [n][out]). */
 int fs_mlp_set_data(fs_mlp *m, const float *x, const float *y, int64_t n);
 
