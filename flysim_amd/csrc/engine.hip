@@ -91,7 +91,9 @@ struct Engine final : EngineBase {
       b_ens, b_stp, b_irep, b_rp_g, b_col_g, b_w_g, b_igap, b_lp, b_cntlp, b_lpre, b_lpost, b_lrec,
       b_lw, b_lw0, b_gl, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
       b_total, b_rec_k, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase, b_suboff, b_subcap,
-      b_pbr, b_gain, b_skey, b_tmod, b_zig;
+      b_pbr, b_gain, b_skey, b_tmod, b_zig, b_wblk;
+  std::vector<uint32_t> hnpop;   // population of each local neuron
+  std::vector<uint64_t> hwblk;   // windowed path: first | count << 32 of each block
   // replay mode's Gaussian injection: population ranges and the position in
   // the reference's ziggurat stream (zig_calls calls before step zig_t0,
   // zig_a calls per step from there)
@@ -103,8 +105,17 @@ struct Engine final : EngineBase {
   bool fused_ok = false;  // every synapse has delay >= 1: one launch per step is valid
   bool fused() const { return fused_ok && sched.fused && !window(); }
   // windowed path: steps per launch, at most 1 + the smallest synaptic delay
+  // and at most what the block's LDS holds (window_lds_bytes <= 64 KiB)
   int32_t win_max = 1;
-  int32_t window() const { return sched.window > 0 ? std::min(sched.window, win_max) : 0; }
+  int32_t window() const {
+    if (sched.window <= 0) return 0;
+    const int npb = std::max(1, sched.update_block / sched.gather_lanes);
+    const int nlu = __builtin_popcount((uint32_t)net.lin_mask);
+    int32_t w = std::min(sched.window, win_max);
+    while (w > 1 && fsk::window_lds_bytes<T>(npb, w, nlu) > (size_t)65536) --w;
+    if (w >= 1 && fsk::window_lds_bytes<T>(npb, w, nlu) > (size_t)65536) return 0;
+    return w;
+  }
   bool fused_active() const override { return fused(); }
   int32_t window_active() const override { return window(); }
   hipStream_t stream = nullptr;
@@ -635,7 +646,10 @@ struct Engine final : EngineBase {
     // the windowed path writes up to win_max steps ahead of the slots other
     // blocks still read (the fused step needs DN >= max NMDA delay + 2)
     net.D = maxd_p + 1 + win_max;
-    net.DN = maxd_n + 1 + win_max;
+    // the gate history: a power of two of slots per neuron (slot arithmetic by mask)
+    int32_t dn = 1;
+    while (dn < maxd_n + 1 + win_max) dn <<= 1;
+    net.DN = dn;
     fused_ok = fusable;
     net.lin_mask = lin_mask;
     net.has_nmda_syn = syn_nmda > 0;
@@ -665,6 +679,7 @@ struct Engine final : EngineBase {
     const size_t n = (size_t)N, n0 = (size_t)N0;
     b_npop.alloc(n0 * 4);
     HIPCHECK(hipMemcpy(b_npop.p, neuron_pop, n0 * 4, hipMemcpyHostToDevice));
+    hnpop.assign(neuron_pop, neuron_pop + n0);
     b_pops.alloc(sizeof(fsk::DevPop<T>) * P);
     upload_pops();
     pfirst.assign(P, -1);
@@ -810,8 +825,13 @@ struct Engine final : EngineBase {
     // region holds its blocks' neurons for K steps.
     const int64_t npb = (fused() || window()) ? ub / L : ub;  // neurons per block
     std::vector<uint64_t> cov(fsk::NSUB, 0);
-    const int64_t nblk = ((int64_t)N + npb - 1) / npb;
-    for (int64_t b = 0; b < nblk; ++b) cov[b % fsk::NSUB] += std::min<int64_t>(npb, N - b * npb);
+    if (window()) {
+      build_window_blocks((int32_t)npb);
+      for (size_t b = 0; b < hwblk.size(); ++b) cov[b % fsk::NSUB] += hwblk[b] >> 32;
+    } else {
+      const int64_t nblk = ((int64_t)N + npb - 1) / npb;
+      for (int64_t b = 0; b < nblk; ++b) cov[b % fsk::NSUB] += std::min<int64_t>(npb, N - b * npb);
+    }
     uint64_t cap = (uint64_t)N * (uint64_t)K;
     const uint64_t cap_max = ((uint64_t)1 << 31) - 1;
     hsub_off.assign(fsk::NSUB, 0u);
@@ -844,6 +864,35 @@ struct Engine final : EngineBase {
   }
 
   int update_grid() const { return (N + sched.update_block - 1) / sched.update_block; }
+
+  // Blocks of the windowed path: runs of neurons of one population (global
+  // order), cut into pieces of at most npb, so every block's update lanes
+  // share one parameter block and update in one pass; where that would take
+  // more than 1.25 times the blocks of plain npb-neuron pieces (many small
+  // populations), plain pieces (the update then runs once per population
+  // present in a wave).
+  void build_window_blocks(int32_t npb) {
+    std::vector<uint64_t> runs;
+    int64_t g = 0;
+    while (g < N) {
+      const uint32_t p = hnpop[g / B];
+      int64_t e = g + 1;
+      while (e < N && hnpop[e / B] == p) ++e;
+      for (int64_t a = g; a < e; a += npb)
+        runs.push_back((uint64_t)a | ((uint64_t)std::min<int64_t>(npb, e - a) << 32));
+      g = e;
+    }
+    const int64_t plain = ((int64_t)N + npb - 1) / npb;
+    if ((int64_t)runs.size() * 4 > plain * 5) {
+      runs.clear();
+      for (int64_t a = 0; a < N; a += npb)
+        runs.push_back((uint64_t)a | ((uint64_t)std::min<int64_t>(npb, N - a) << 32));
+    }
+    hwblk.swap(runs);
+    b_wblk.alloc(hwblk.size() * 8);
+    HIPCHECK(hipMemcpy(b_wblk.p, hwblk.data(), hwblk.size() * 8, hipMemcpyHostToDevice));
+    net.wblk = b_wblk.as<uint64_t>();
+  }
   // fused step: a group of gather_lanes lanes per neuron
   int step_grid() const {
     const int64_t threads = (int64_t)N * sched.gather_lanes;
@@ -924,11 +973,18 @@ struct Engine final : EngineBase {
   // windowed path: steps k0 .. k0+w-1 in one launch
   void launch_window(int32_t k0, int32_t w, bool timed) {
     if (timed) HIPCHECK(hipEventRecord(ev[0], stream));
-    const dim3 g(step_grid()), b(sched.update_block);
-    if (net.ext_mode == FS_EXT_REPLAY)
-      hipLaunchKernelGGL((fsk::k_window<T, true>), g, b, 0, stream, net, k0, w);
-    else
-      hipLaunchKernelGGL((fsk::k_window<T, false>), g, b, 0, stream, net, k0, w);
+    const dim3 g((unsigned)hwblk.size()), b(sched.update_block);
+    const size_t lds = fsk::window_lds_bytes<T>(sched.update_block / sched.gather_lanes, w,
+                                                __builtin_popcount((uint32_t)net.lin_mask));
+    // replicas in whole waves with one lane per row: the row tables are wave-uniform
+    const bool uni = (B % 64) == 0 && sched.gather_lanes == 1;
+    if (net.ext_mode == FS_EXT_REPLAY) {
+      if (uni) hipLaunchKernelGGL((fsk::k_window<T, true, true>), g, b, lds, stream, net, k0, w);
+      else hipLaunchKernelGGL((fsk::k_window<T, true, false>), g, b, lds, stream, net, k0, w);
+    } else {
+      if (uni) hipLaunchKernelGGL((fsk::k_window<T, false, true>), g, b, lds, stream, net, k0, w);
+      else hipLaunchKernelGGL((fsk::k_window<T, false, false>), g, b, lds, stream, net, k0, w);
+    }
     HIPCHECK(hipGetLastError());
     if (timed) {
       HIPCHECK(hipEventRecord(ev[1], stream));
@@ -1102,6 +1158,12 @@ struct Engine final : EngineBase {
     if (zig_dirty) zig_refresh();
     int64_t written = 0;
     if (off) off[0] = 0;
+    // the windowed kernel writes the spike record only when it is read back
+    const int32_t rec_on = (off || ids) ? 1 : 0;
+    if (net.rec_on != rec_on) {
+      net.rec_on = rec_on;
+      graph_dirty = true;  // a kernel argument
+    }
     t_upd = t_del = t_graph = 0;
     n_timed = 0;
     n_graph_steps = 0;
@@ -1287,8 +1349,12 @@ struct Engine final : EngineBase {
       case 8: d2h(b_spre.p); break;
       case 9: d2h(b_xn.p); break;
       case 10: {
-        const int64_t s = ((step - 1) % net.DN + net.DN) % net.DN;
-        d2h(b_shist.as<T>() + (size_t)s * n);
+        const int32_t s = (int32_t)(((step - 1) % net.DN + net.DN) % net.DN);
+        hipLaunchKernelGGL((fsk::k_read_hist<T>), dim3((N + 255) / 256), dim3(256), 0, stream, net, s,
+                           b_tmp.as<T>());
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(stream));
+        d2h(b_tmp.p);
         break;
       }
       case 11: case 12: case 13: case 14: d2h(b_elin.as<T>() + (size_t)(f - 11) * n); break;

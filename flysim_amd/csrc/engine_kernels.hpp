@@ -110,7 +110,7 @@ struct Net {
   T *ring;              // [4][D][N]
   T *spre;              // [N] linear presynaptic gate
   T *xn;                // [N] NMDA presynaptic x
-  T *shist;             // [DN][N] NMDA presynaptic s history
+  T *shist;             // [N0][DN][B] NMDA presynaptic s history (slot-contiguous per neuron)
   T *gn;                // [N] gathered NMDA conductance of the last step
   T *elin;              // [4][N] external linear gates
   T *enx, *ens;         // [N] external NMDA gate
@@ -164,6 +164,9 @@ struct Net {
   unsigned long long *counters; // [3][NCTR]: spikes, linear pushes, NMDA out-synapses of
                                 // spikers, each spread over NCTR slots by block index
   int32_t gather_lanes;
+  int32_t rec_on;           // 1: the batch's spikes go to the record (readback asked for)
+  const uint64_t *wblk;     // windowed path: block b covers neurons [first, first + count),
+                            // first | count << 32, at most blockDim / gather_lanes of them
   int32_t push_waves;       // waves of k_deliver dedicated to the push
   int32_t push_blocks;      // fused step: leading blocks that push; the rest gather + update
 };
@@ -285,9 +288,28 @@ __device__ __forceinline__ X ldc(const X *p) {
   return *(const __attribute__((address_space(4))) X *)p;
 }
 
+// Barrier for data exchanged through LDS only: its fences order LDS accesses
+// (s_waitcnt lgkmcnt) and leave the block's global loads, stores and atomics
+// in flight, where __syncthreads would wait for every one of them.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // (t0 % M + k) mod M for a step offset k >= -M of the batch starting at t0
 __device__ __forceinline__ int32_t slot_of(int32_t t0mod, int32_t k, int32_t M) {
   return (int32_t)((uint32_t)(t0mod + k + M) % (uint32_t)M);
+}
+
+// Position of the NMDA gate of local neuron nl, replica rep, history slot
+// `slot` (DN, a power of two, slots per neuron): the history is slot-contiguous
+// per neuron, [N0][DN][B], so the consecutive slots one window reads for a
+// presynaptic neuron share one or two cache lines (and B copies of one neuron
+// are adjacent).
+template <typename T>
+__device__ __forceinline__ size_t hidx(const Net<T> &net, uint32_t nl, uint32_t rep, uint32_t slot) {
+  return ((size_t)nl * (uint32_t)net.DN + slot) * (uint32_t)net.B + rep;
 }
 
 // ------------------------------------------------------------------ update
@@ -310,10 +332,12 @@ template <typename T>
 using PopC = const __attribute__((address_space(4))) DevPop<T>;
 
 // the state of neuron n as of the end of step k - 1 (hprev: gate history
-// slot of step k - 1)
+// slot of step k - 1).  The presynaptic gates are loaded whether or not the
+// neuron's population has outgoing synapses of their kind (unused values go
+// back unchanged), so no load waits on the population parameters.
 template <typename T>
-__device__ __forceinline__ void load_state(const Net<T> &net, PopC<T> &P, int32_t n, int32_t hprev,
-                                           NState<T> &S) {
+__device__ __forceinline__ void load_state(const Net<T> &net, int32_t nl, int32_t rep, int32_t n,
+                                           int32_t hprev, NState<T> &S) {
   const size_t N = (size_t)net.N;
   S.v = net.V[n];
   S.cs = net.cs[n];
@@ -322,11 +346,9 @@ __device__ __forceinline__ void load_state(const Net<T> &net, PopC<T> &P, int32_
     if (net.lin_mask & (1 << li)) S.G[li] = net.G[li * N + n];
     if (net.ext_mask & (1 << lin_rid(li))) S.elin[li] = net.elin[li * N + n];
   }
-  if (P.has_lin_out) S.spre = net.spre[n];
-  if (P.has_nmda_out) {
-    S.xn = net.xn[n];
-    S.sn = net.shist[(size_t)hprev * N + n];
-  }
+  S.spre = net.spre[n];
+  S.xn = net.xn[n];
+  S.sn = net.shist[hidx(net, (uint32_t)nl, (uint32_t)rep, (uint32_t)hprev)];
   if (net.ext_mask & (1 << 2)) {
     S.enx = net.enx[n];
     S.ens = net.ens[n];
@@ -342,8 +364,7 @@ __device__ __forceinline__ void load_state(const Net<T> &net, PopC<T> &P, int32_
 }
 
 template <typename T>
-__device__ __forceinline__ void store_state(const Net<T> &net, PopC<T> &P, int32_t n,
-                                            const NState<T> &S) {
+__device__ __forceinline__ void store_state(const Net<T> &net, int32_t n, const NState<T> &S) {
   const size_t N = (size_t)net.N;
   net.V[n] = S.v;
   net.cs[n] = S.cs;
@@ -352,8 +373,8 @@ __device__ __forceinline__ void store_state(const Net<T> &net, PopC<T> &P, int32
     if (net.lin_mask & (1 << li)) net.G[li * N + n] = S.G[li];
     if (net.ext_mask & (1 << lin_rid(li))) net.elin[li * N + n] = S.elin[li];
   }
-  if (P.has_lin_out) net.spre[n] = S.spre;
-  if (P.has_nmda_out) net.xn[n] = S.xn;  // s goes to the history every step
+  net.spre[n] = S.spre;
+  net.xn[n] = S.xn;  // s goes to the history every step
   if (net.ext_mask & (1 << 2)) {
     net.enx[n] = S.enx;
     net.ens[n] = S.ens;
@@ -394,6 +415,7 @@ struct NConst {
   T eg;
   uint64_t key;
   uint32_t ctr;
+  uint32_t nl, rep;  // local neuron and replica
 };
 
 template <typename T>
@@ -406,21 +428,79 @@ __device__ __forceinline__ void load_const(const Net<T> &net, PopC<T> &P, int32_
   C.eg = T(net.gainr[5 * net.B + rep]);
   C.key = net.seed ^ net.skey[rep] ^ (((uint64_t)P.key_hi << 32) | P.key_lo);
   C.ctr = (uint32_t)(nl - P.first);
+  C.nl = (uint32_t)nl;
+  C.rep = (uint32_t)rep;
+}
+
+// External event outcomes and the Gaussian injection draw of neuron n at step
+// t, the stochastic part of BgInit (code/EqsGnl2.cpp:284-337): bit rid of
+// dbits is set when receptor rid's external gate receives an event this step
+// (order A, G, NMDA, ACh, GluCl of the receptor ids; receptors whose activity
+// bit is cleared, pb < 0, draw nothing); injv is the injection draw -- replay
+// mode: the reference's Igauss value (nA) from its ziggurat stream;
+// counter-based mode: a standard normal (Box-Muller on the second Philox
+// block) that the update scales by the population's mean and deviation.
+template <typename T, bool REPLAY>
+__device__ __forceinline__ void neuron_draws(const Net<T> &net, PopC<T> &P, const NConst<T> &C,
+                                             int32_t n, int64_t t, uint32_t &dbits, float &injv) {
+  dbits = 0u;
+  injv = 0.0f;
+  const uint32_t act = P.act;
+  if (REPLAY) {
+    const int64_t s = t - net.ext_first;
+    if (s >= 0 && s < net.ext_steps) {
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+        if ((net.ext_mask & (1 << r)) && C.pb[r] >= 0.0f)
+          dbits |= ((net.ext_bits[((size_t)s * 5 + r) * net.words + (n >> 5)] >> (n & 31)) & 1u) << r;
+    }
+    if (act & ACT_INJ) {
+      // the reference's own ziggurat stream (single-thread order: neurons in
+      // index order, only those with injection on draw)
+      const uint32_t k = (uint32_t)t * P.zig_a + (uint32_t)n + P.zig_b;
+      injv = zig_igauss((uint32_t)net.seed, k, (float)P.inj_mean, (float)P.inj_std, net.zig);
+    }
+    return;
+  }
+  if (!net.ext_mask && !(act & ACT_INJ)) return;
+  // Philox counter (neuron within its instance, step), key C.key
+  const uint64_t key = C.key;
+  uint32_t rnd[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  uint32_t c[4] = {C.ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0u};
+  philox10(c, (uint32_t)key, (uint32_t)(key >> 32));
+  rnd[0] = c[0]; rnd[1] = c[1]; rnd[2] = c[2]; rnd[3] = c[3];
+  if ((net.ext_mask & (1 << 4)) || (act & ACT_INJ)) {
+    uint32_t e[4] = {C.ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 1u};
+    philox10(e, (uint32_t)key, (uint32_t)(key >> 32));
+    rnd[4] = e[0]; rnd[5] = e[1]; rnd[6] = e[2]; rnd[7] = e[3];
+  }
+#pragma unroll
+  for (int r = 0; r < 5; ++r)
+    if ((net.ext_mask & (1 << r)) && C.pb[r] >= 0.0f) dbits |= (C.pb[r] > u24(rnd[r]) ? 1u : 0u) << r;
+  if (act & ACT_INJ) {
+    const float u1 = ((float)(rnd[6] >> 8) + 1.0f) * (1.0f / 16777216.0f);
+    const float u2 = u24(rnd[7]);
+    injv = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+  }
 }
 
 // One neuron's step t (the reference's Activation, then its presynaptic
 // gates and plasticity) on state S: (1) linear conductances of the previous
 // Spike phase, (2) external gates, (3) membrane, (4) gates, (4b) STP/STD.
-// gn: NMDA conductance of the previous Spike phase; inc: ring_take's output.
-// Writes the NMDA gate history slot hslot (step t); everything else stays
-// in S.
+// gn: NMDA conductance of the previous Spike phase; inc: ring_take's output;
+// dbits / injv: neuron_draws' output for (n, t).  Writes the NMDA gate
+// history slot hslot (step t); everything else stays in S.
 // WIN: the windowed path's instance, which never runs with LTP or gap
 // junctions (the host gives such networks no window), so their branches are
-// compiled out and leave the registers to the window's state.
-template <typename T, bool REPLAY, bool WIN = false>
-__device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const NConst<T> &C,
+// compiled out and leave the registers to the window's state; it leaves the
+// step's NMDA gate in S.sn for its caller to stage instead of storing it to
+// the history.  PT: the parameter block's type (global constant memory or an
+// LDS copy).
+template <typename T, bool REPLAY, bool WIN = false, typename PT = PopC<T>>
+__device__ __forceinline__ void neuron_core(const Net<T> &net, PT &P, const NConst<T> &C,
                                             int32_t n, int64_t t, int32_t hslot, T gn,
-                                            const T inc[4], NState<T> &S, bool &spk, T &a_push) {
+                                            const T inc[4], uint32_t dbits, float injv, NState<T> &S,
+                                            bool &spk, T &a_push) {
   const int32_t N = net.N;
   uint32_t cs = S.cs;
   uint32_t cnt = cs & CNT_MASK;
@@ -444,47 +524,23 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
       *gp = T(0);
     }
   }
-  // (2) external Poisson gates, order A, G, ACh, GluCl, NMDA (BgInit);
-  // Philox counter (neuron within its instance, step), key C.key
+  // (2) external Poisson gates, order A, G, ACh, GluCl, NMDA (BgInit)
   const uint32_t act = P.act;
-  const uint64_t key = C.key;
-  const uint32_t ctr = C.ctr;
-  uint32_t rnd[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-  if (!REPLAY && (net.ext_mask || (act & ACT_INJ))) {
-    uint32_t c[4] = {ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0u};
-    philox10(c, (uint32_t)key, (uint32_t)(key >> 32));
-    rnd[0] = c[0]; rnd[1] = c[1]; rnd[2] = c[2]; rnd[3] = c[3];
-    if ((net.ext_mask & (1 << 4)) || (act & ACT_INJ)) {
-      uint32_t e[4] = {ctr, (uint32_t)t, (uint32_t)((uint64_t)t >> 32), 1u};
-      philox10(e, (uint32_t)key, (uint32_t)(key >> 32));
-      rnd[4] = e[0]; rnd[5] = e[1]; rnd[6] = e[2]; rnd[7] = e[3];
-    }
-  }
-  auto draw = [&](int rid, float pb) -> bool {
-    if (REPLAY) {
-      const int64_t s = t - net.ext_first;
-      if (s < 0 || s >= net.ext_steps) return false;
-      return (net.ext_bits[((size_t)s * 5 + rid) * net.words + (n >> 5)] >> (n & 31)) & 1u;
-    }
-    return pb > u24(rnd[rid]);
-  };
   const T eg = C.eg;
 #pragma unroll
   for (int li = 0; li < 4; ++li) {
     const int rid = lin_rid(li);
     if (!(net.ext_mask & (1 << rid))) continue;
-    const float pb = C.pb[rid];
-    if (pb < 0.0f) continue;
+    if (C.pb[rid] < 0.0f) continue;
     T e = S.elin[li];
-    e = draw(rid, pb) ? e + T(1) : e * P.dfst[rid];
+    e = ((dbits >> rid) & 1u) ? e + T(1) : e * P.dfst[rid];
     S.elin[li] = e;
     g[rid] = g[rid] + P.gext[rid] * eg * e;
   }
   if (net.ext_mask & (1 << 2)) {
-    const float pb = C.pb[2];
-    if (pb >= 0.0f) {
+    if (C.pb[2] >= 0.0f) {
       T x = S.enx, s = S.ens;
-      x = draw(2, pb) ? x + T(1) : x * P.dfxt_n;
+      x = ((dbits >> 2) & 1u) ? x + T(1) : x * P.dfxt_n;
       s = P.dfst[2] * s + P.efas_n * x * (T(1) - s);
       S.enx = x;
       S.ens = s;
@@ -495,18 +551,8 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
   // injection drawn now (BgInit adds to Isyn, code/EqsGnl2.cpp:335)
   T isyn = (!WIN && net.igap) ? net.igap[n] : T(0);
   if (act & ACT_INJ) {
-    if (REPLAY) {
-      // the reference's own ziggurat stream (single-thread order: neurons in
-      // index order, only those with injection on draw)
-      const uint32_t k = (uint32_t)t * P.zig_a + (uint32_t)n + P.zig_b;
-      isyn = isyn + T(zig_igauss((uint32_t)net.seed, k, (float)P.inj_mean, (float)P.inj_std, net.zig));
-    } else {
-      // Box-Muller on the second Philox block (the counter-based stream)
-      const float u1 = ((float)(rnd[6] >> 8) + 1.0f) * (1.0f / 16777216.0f);
-      const float u2 = u24(rnd[7]);
-      const float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
-      isyn = isyn + (P.inj_mean + P.inj_std * T(z));
-    }
+    if (REPLAY) isyn = isyn + T(injv);
+    else isyn = isyn + (P.inj_mean + P.inj_std * T(injv));
   }
   // (3) membrane (VmEul / firing)
   auto firing = [&]() {
@@ -573,7 +619,7 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PopC<T> &P, const
     S.xn = x;
     const T s1 = P.pre_dfst * s0 + P.pre_efas * x * (T(1) - s0);
     S.sn = s1;
-    net.shist[(size_t)hslot * N + n] = s1;
+    if (!WIN) net.shist[hidx(net, C.nl, C.rep, (uint32_t)hslot)] = s1;
   }
   // (4b) EqsGnl2::STP / STD (code/EqsGnl2.cpp:1048-1090), after the gates
   if (net.stp_mode == STP_FULL) {
@@ -645,14 +691,18 @@ __device__ __forceinline__ void neuron_step_mem(const Net<T> &net, int32_t n, in
   pid = p0 * (uint32_t)net.B + (uint32_t)rep;
   PopC<T> &P = *(PopC<T> *)(net.pops + p0);
   NState<T> S;
-  load_state(net, P, n, slot_of(net.tmod[1], k - 1, net.DN), S);
+  load_state(net, nl, rep, n, slot_of(net.tmod[1], k - 1, net.DN), S);
   NConst<T> C;
   load_const(net, P, nl, rep, p0, C);
   T inc[4];
   ring_take(net, n, slot_of(net.tmod[0], k - 1, net.D), inc);
   const T gn = net.has_nmda_syn ? (gn_given ? gn_in : net.gn[n]) : T(0);
-  neuron_core<T, REPLAY>(net, P, C, n, t, slot_of(net.tmod[1], k, net.DN), gn, inc, S, spk, a_push);
-  store_state(net, P, n, S);
+  uint32_t dbits;
+  float injv;
+  neuron_draws<T, REPLAY>(net, P, C, n, t, dbits, injv);
+  neuron_core<T, REPLAY>(net, P, C, n, t, slot_of(net.tmod[1], k, net.DN), gn, inc, dbits, injv, S, spk,
+                         a_push);
+  store_state(net, n, S);
 }
 
 // One neuron's step t (the reference's Activation) followed by the wave's
@@ -853,10 +903,10 @@ __device__ __forceinline__ void block_add_events(const Net<T> &net, unsigned lon
                                                  unsigned long long nout) {
   __shared__ unsigned long long s_ev[2];
   if (threadIdx.x == 0) { s_ev[0] = 0; s_ev[1] = 0; }
-  __syncthreads();
+  lds_barrier();
   if (events) atomicAdd(&s_ev[0], events);
   if (nout) atomicAdd(&s_ev[1], nout);
-  __syncthreads();
+  lds_barrier();
   if (threadIdx.x == 0) {
     const int slot = (int)(blockIdx.x & (NCTR - 1));
     if (s_ev[0]) atomicAdd(&net.counters[NCTR + slot], s_ev[0]);
@@ -864,90 +914,44 @@ __device__ __forceinline__ void block_add_events(const Net<T> &net, unsigned lon
   }
 }
 
-// Sum of the n entries of one lane's share of a row in a fixed order:
-// quadruples of consecutive entries into four partial sums (entry 4j+q into
-// sum q), the last n % 4 entries into the first, then (s0 + s1) + (s2 + s3).
-// head(k, m, w) gives entry k < 4 (k a constant after unrolling), tail(i, m,
-// w) entry i from memory; val(m, w) is the entry's term.  The entries of
-// quadruple j + 1 are fetched while the gate loads of quadruple j are in
-// flight, and the remainder's entries are fetched together (a missing one is
-// replaced by a present one whose term is then dropped).
-template <typename T, typename Head, typename Tail, typename Val>
-__device__ __forceinline__ T row_terms(int32_t n, Head head, Tail tail, Val val) {
-  T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
-  const int32_t q = n >> 2;
-  uint32_t m0 = 0u, m1 = 0u, m2 = 0u, m3 = 0u;
-  float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f, w3 = 0.0f;
-  if (q > 0) {
-    head(0, m0, w0);
-    head(1, m1, w1);
-    head(2, m2, w2);
-    head(3, m3, w3);
-  }
-  for (int32_t j = 0; j < q; ++j) {
-    const T x0 = val(m0, w0), x1 = val(m1, w1), x2 = val(m2, w2), x3 = val(m3, w3);
-    if (j + 1 < q) {
-      const int32_t i = 4 * (j + 1);
-      tail(i, m0, w0);
-      tail(i + 1, m1, w1);
-      tail(i + 2, m2, w2);
-      tail(i + 3, m3, w3);
-    }
-    a0 += x0;
-    a1 += x1;
-    a2 += x2;
-    a3 += x3;
-  }
-  const int32_t r = n - 4 * q;
-  if (r > 0) {
-    if (q == 0) {
-      head(0, m0, w0);
-      head(r > 1 ? 1 : 0, m1, w1);
-      head(r > 2 ? 2 : 0, m2, w2);
-    } else {
-      const int32_t i = 4 * q;
-      tail(i, m0, w0);
-      tail(r > 1 ? i + 1 : i, m1, w1);
-      tail(r > 2 ? i + 2 : i, m2, w2);
-    }
-    const T x0 = val(m0, w0), x1 = val(m1, w1), x2 = val(m2, w2);
-    a0 += x0;
-    a0 += r > 1 ? x1 : T(0);
-    a0 += r > 2 ? x2 : T(0);
-  }
-  return (a0 + a1) + (a2 + a3);
-}
-
-// NMDA conductance of one row: sum_j w * s_j(ts - d) over CSR-by-post row p,
-// with L lanes of a group striding the row (lane sl takes entries
-// r0 + sl + i*L, summed by row_terms) and a shuffle reduction (result valid
-// in every lane of the group).  All lanes of the group must call it.
-// UNI: all lanes of the wave gather copies of one local row (L == 1,
+// NMDA conductance of one row for one step: sum_j w * s_j(ts - d) over
+// CSR-by-post row p.  L lanes of a group stride the row (lane sl takes entries
+// r0 + sl + i*L); each lane sums its entries in entry order with fused
+// multiply-adds, the sum is scaled by the replica's NMDA gain, and a shuffle
+// reduction over the group follows (result valid in every lane of the group).
+// gather_window evaluates the same expression for several steps at once, so
+// every path gives the same bits.  Entries are fetched four at a time ahead
+// of their sums.  All lanes of the group must call it.  h0: history slot of
+// ts.  UNI: all lanes of the wave gather copies of one local row (L == 1,
 // replicas in whole waves), so the row's columns and weights are scalar loads.
 template <typename T, bool UNI>
 __device__ __forceinline__ T gather_row(const Net<T> &net, int64_t p, int32_t h0, int L, int sl) {
   T acc = T(0);
   if (p < net.N) {
-    const uint32_t B = (uint32_t)net.B;
+    const uint32_t B = (uint32_t)net.B, dmask = (uint32_t)net.DN - 1u;
     uint32_t pl = (uint32_t)p / B;
     if constexpr (UNI) pl = __builtin_amdgcn_readfirstlane(pl);
     const uint32_t rep = (uint32_t)p - pl * B;
     const uint64_t r0 = ldc(net.rp_n + pl), r1 = ldc(net.rp_n + pl + 1);
-    const int32_t DN = net.DN;  // h0: history slot of ts; the slot of ts - d wraps once
+    const size_t rs = (size_t)net.DN * B;  // history of one presynaptic neuron
     const T *sh = net.shist + rep;
-    const uint64_t e0 = r0 + (uint64_t)sl;
-    const int32_t n = e0 < r1 ? (int32_t)((r1 - e0 + (uint64_t)L - 1) / (uint64_t)L) : 0;
-    auto tail = [&](int32_t i, uint32_t &m, float &w) {
-      const uint64_t e = e0 + (uint64_t)i * (uint64_t)L;
-      m = ldc(net.meta_n + e);
-      w = ldc(net.w_n + e);
+    auto gate = [&](uint32_t m) -> T {
+      return sh[(size_t)(m & 0x00FFFFFFu) * rs + (size_t)(((uint32_t)h0 - (m >> 24)) & dmask) * B];
     };
-    auto val = [&](uint32_t m, float w) -> T {
-      int32_t hs = h0 - (int32_t)(m >> 24);
-      hs += hs < 0 ? DN : 0;
-      return T(w) * sh[(size_t)hs * net.N + (size_t)(m & 0x00FFFFFFu) * B];
-    };
-    acc = row_terms<T>(n, tail, tail, val);
+    const uint64_t step = (uint64_t)L;
+    uint64_t e = r0 + (uint64_t)sl;
+    for (; e + 3 * step < r1; e += 4 * step) {
+      const uint32_t m0 = ldc(net.meta_n + e), m1 = ldc(net.meta_n + e + step),
+                     m2 = ldc(net.meta_n + e + 2 * step), m3 = ldc(net.meta_n + e + 3 * step);
+      const float w0 = ldc(net.w_n + e), w1 = ldc(net.w_n + e + step), w2 = ldc(net.w_n + e + 2 * step),
+                  w3 = ldc(net.w_n + e + 3 * step);
+      const T g0 = gate(m0), g1 = gate(m1), g2 = gate(m2), g3 = gate(m3);
+      acc = fma(T(w0), g0, acc);
+      acc = fma(T(w1), g1, acc);
+      acc = fma(T(w2), g2, acc);
+      acc = fma(T(w3), g3, acc);
+    }
+    for (; e < r1; e += step) acc = fma(T(ldc(net.w_n + e)), gate(ldc(net.meta_n + e)), acc);
     acc *= T(net.gainr[2 * B + rep]);
   }
   for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
@@ -1019,240 +1023,372 @@ __global__ void __launch_bounds__(1024) k_step(Net<T> net, int32_t k) {
                                 pbase[sub] + pcnt[sub], sub);
 }
 
-// NMDA row of one lane of a gather group, held across the steps of a window:
-// the first RC entries of the lane's share of the row (entries r0 + sl + i*L)
-// in registers, the rest re-read.  The sum is row_terms', as in gather_row,
-// so both paths give the same bits.
-constexpr int RC = 4;
-template <typename T>
-struct RowCache {
-  uint64_t r0, r1;
-  uint32_t m[RC];
-  float w[RC];
-  int32_t cnt;  // entries of this lane in the row
-  T gain;       // the replica's NMDA gain
-};
-
-template <typename T>
-__device__ __forceinline__ void row_cache_load(const Net<T> &net, int64_t p, int L, int sl,
-                                               RowCache<T> &c) {
-  c.cnt = 0;
-  c.r0 = c.r1 = 0;
-  c.gain = T(0);
+// NMDA conductances of post row p for the W steps k0 .. k0+W-1 of a window,
+// into s_gn[s * npb + i] (i: the row's neuron within the block).  The L lanes
+// of the row are TW = min(L, 16) time lanes times L / TW entry sub-lanes:
+// sub-lane q takes the row's entries q, q + L/TW, ... and time lane tl the
+// steps tl, tl + TW, ... (four per pass).  A term's gates of consecutive
+// steps are consecutive history slots of one presynaptic neuron, so the time
+// lanes of a row read one or two cache lines per term together, and its
+// column and weight are one broadcast load.  Each step's terms are summed in
+// entry order per sub-lane with fused multiply-adds, scaled by the replica's
+// NMDA gain and reduced over the sub-lanes by a shuffle tree (gather_row's
+// expression with L / TW lanes).  Four entries are in flight per lane
+// hb: history slot of step k0 - 1.  All lanes of the row must call it.
+template <typename T, bool UNI>
+__device__ __forceinline__ void gather_window(const Net<T> &net, int64_t p, int32_t hb, int32_t W, int L,
+                                              int sl, T *s_gn, int npb, int i) {
+  constexpr int NA = 4;  // steps per lane per pass
+  constexpr int U = 4;   // entries per round
+  const uint32_t B = (uint32_t)net.B, dmask = (uint32_t)net.DN - 1u;
+  const int TW = L < 16 ? L : 16;
+  const int G = L / TW;
+  const int q = sl / TW, tl = sl - q * TW;
+  const bool live = p < net.N;
+  uint64_t r0 = 0, r1 = 0;
+  uint32_t rep = 0;
+  T gain = T(0);
+  if (live) {
+    uint32_t pl = (uint32_t)p / B;
+    if constexpr (UNI) pl = __builtin_amdgcn_readfirstlane(pl);
+    rep = (uint32_t)p - pl * B;
+    r0 = ldc(net.rp_n + pl);
+    r1 = ldc(net.rp_n + pl + 1);
+    gain = T(net.gainr[2 * B + rep]);
+  }
+  const size_t rs = (size_t)net.DN * B;
+  const T *sh = net.shist + rep;
+  const uint64_t st = (uint64_t)G;
+  for (int32_t c0 = 0; c0 < W; c0 += TW * NA) {
+    // this lane's steps: c0 + tl + a * TW, a < na
+    int na = 0;
 #pragma unroll
-  for (int i = 0; i < RC; ++i) { c.m[i] = 0u; c.w[i] = 0.0f; }
-  if (p >= net.N) return;
-  const uint32_t pl = (uint32_t)p / (uint32_t)net.B;
-  c.gain = T(net.gainr[2 * net.B + ((uint32_t)p - pl * (uint32_t)net.B)]);
-  c.r0 = ldc(net.rp_n + pl);
-  c.r1 = ldc(net.rp_n + pl + 1);
-  const uint64_t e0 = c.r0 + (uint64_t)sl;
-  c.cnt = e0 < c.r1 ? (int32_t)((c.r1 - e0 + (uint64_t)L - 1) / (uint64_t)L) : 0;
+    for (int a = 0; a < NA; ++a) na += (c0 + tl + a * TW < W) ? 1 : 0;
+    const uint32_t sb = (uint32_t)(hb + c0 + tl);  // slot of step k0 + c0 + tl - 1 (before the delay)
+    T acc[NA];
 #pragma unroll
-  for (int i = 0; i < RC; ++i) {
-    const uint64_t e = e0 + (uint64_t)i * (uint64_t)L;
-    if (e < c.r1) {
-      c.m[i] = ldc(net.meta_n + e);
-      c.w[i] = ldc(net.w_n + e);
+    for (int a = 0; a < NA; ++a) acc[a] = T(0);
+    uint64_t e = r0 + (uint64_t)q;
+    uint32_t m[U];
+    float w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t eu = e + (uint64_t)u * st;
+      m[u] = eu < r1 ? ldc(net.meta_n + eu) : 0u;
+      w[u] = eu < r1 ? ldc(net.w_n + eu) : 0.0f;
+    }
+    while (e < r1) {
+      // the gates of this round's entries, then the next round's entries
+      T x[U][NA];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool ok = e + (uint64_t)u * st < r1;
+        const T *g = sh + (size_t)(m[u] & 0x00FFFFFFu) * rs;
+        const uint32_t bs = sb - (m[u] >> 24);
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+          x[u][a] = (ok && a < na) ? g[(size_t)((bs + (uint32_t)(a * TW)) & dmask) * B] : T(0);
+      }
+      const uint64_t en = e + (uint64_t)U * st;
+      uint32_t mn[U];
+      float wn[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t eu = en + (uint64_t)u * st;
+        mn[u] = eu < r1 ? ldc(net.meta_n + eu) : 0u;
+        wn[u] = eu < r1 ? ldc(net.w_n + eu) : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e + (uint64_t)u * st < r1) {
+          const T tw = T(w[u]);
+#pragma unroll
+          for (int a = 0; a < NA; ++a) acc[a] = fma(tw, x[u][a], acc[a]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        m[u] = mn[u];
+        w[u] = wn[u];
+      }
+      e = en;
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      T v = acc[a] * gain;
+      for (int off = L >> 1; off >= TW; off >>= 1) v += __shfl_xor(v, off, L);
+      if (q == 0 && a < na && live) s_gn[(c0 + tl + a * TW) * npb + i] = v;
     }
   }
 }
 
+// Bytes of dynamic LDS k_window takes for npb neurons per block, W steps and
+// nlu linear receptors in use (the carve-up at the top of k_window).
 template <typename T>
-__device__ __forceinline__ T gather_cached(const Net<T> &net, const RowCache<T> &c, int64_t p,
-                                           int32_t h0, int L, int sl) {
-  T acc = T(0);
-  if (p < net.N) {
-    const uint32_t B = (uint32_t)net.B;
-    const uint32_t rep = (uint32_t)p - ((uint32_t)p / B) * B;
-    const int32_t DN = net.DN;
-    const T *sh = net.shist + rep;
-    auto head = [&](int k, uint32_t &m, float &w) {  // k: a constant after unrolling
-      m = k == 0 ? c.m[0] : k == 1 ? c.m[1] : k == 2 ? c.m[2] : c.m[3];
-      w = k == 0 ? c.w[0] : k == 1 ? c.w[1] : k == 2 ? c.w[2] : c.w[3];
-    };
-    auto tail = [&](int32_t i, uint32_t &m, float &w) {
-      const uint64_t e = c.r0 + (uint64_t)sl + (uint64_t)i * (uint64_t)L;
-      m = ldc(net.meta_n + e);
-      w = ldc(net.w_n + e);
-    };
-    auto val = [&](uint32_t m, float w) -> T {
-      int32_t hs = h0 - (int32_t)(m >> 24);
-      hs += hs < 0 ? DN : 0;
-      return T(w) * sh[(size_t)hs * net.N + (size_t)(m & 0x00FFFFFFu) * B];
-    };
-    acc = row_terms<T>(c.cnt, head, tail, val);
-    acc *= c.gain;
-  }
-  for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
-  return acc;
+__host__ __device__ constexpr size_t window_lds_bytes(int npb, int W, int nlu) {
+  return (size_t)8 * npb + sizeof(T) * ((size_t)W * npb * (2 + nlu) + (size_t)((npb + 63) / 64) * 64 * W) +
+         (size_t)4 * ((size_t)npb + 2 * (size_t)W * npb + (size_t)((npb + 63) / 64) * 64 * W);
 }
 
 // Temporal blocking over the smallest synaptic delay: one launch runs the W
 // consecutive steps k0 .. k0+W-1 of the batch, W <= 1 + the smallest delay of
-// any synapse.  A spike of step u lands in its targets' ring slot u + d, read
-// by their update of step u + d + 1 >= k0 + W, and its NMDA gate of step u
-// enters gathers of steps >= u + d + 1 as well: nothing a block reads during
-// the launch is written by another block during it, so each block runs its
-// W steps with block barriers only and pushes its own spikes right after each
-// step.  The ring (D >= W + max delay + 1 slots) and the gate history
-// (DN >= W + max NMDA delay + 1) are deep enough that no write of the launch
-// lands in a slot some block has still to read.  Across the window each
-// neuron's state, its NMDA row (first RC entries per lane) and its push-row
-// bounds stay in registers; the step's spikes are staged in LDS and written
-// to the record (with their step, rec_k) when the staging area fills and at
-// the end, one atomic on total[b % NSUB] per flush.
-// (three waves per SIMD: 168 registers, so a 20k-neuron network's 625
-// blocks of 256 threads are all resident at once)
-template <typename T, bool REPLAY>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3)))
-k_window(Net<T> net, int32_t k0, int32_t W) {
-  constexpr int WREC = 1024;
-  __shared__ uint32_t s_ids[512], s_deg[512];  // blockDim <= 512
-  __shared__ uint64_t s_r0[512];
-  __shared__ T s_sc[4][512];  // push amplitude x the replica's gain, by linear receptor
-  __shared__ uint32_t w_ids[WREC], w_k[WREC];
-  __shared__ uint32_t s_cnt, s_base, s_wn;
+// any synapse.  A spike of step u >= k0 lands in ring slot u + d >= k0 + W - 1,
+// read only by updates of steps >= k0 + W, and its NMDA gate enters gathers of
+// steps >= u + d + 1 >= k0 + W: every input a window reads was written before
+// the launch.  So a block runs three phases, exchanging data through LDS:
+//   A  prefetch, all threads: the NMDA sums of all W steps (gather_window, L
+//      lanes per row), the W ring slots of each linear receptor, and the
+//      external event outcomes and injection draws of all W steps
+//      (neuron_draws);
+//   B  update, one lane per neuron: W steps of neuron_core from registers and
+//      LDS; the population parameters come through the scalar cache (each
+//      wave runs the update once per distinct population among its lanes,
+//      usually one), so the loop issues no vector memory access and never
+//      waits on one; each wave stages its spikes (neuron, step, push
+//      amplitude) and the step's NMDA gates in LDS;
+//   C  push, all waves: one wave per staged spike walks its CSR-by-pre row
+//      into the ring slots of the spike's step plus each synapse's delay; the
+//      ring slots read in A are cleared, the NMDA gates go to the history, and
+//      the spike record (when readback is on) and counters are written.
+// The barriers between the phases wait for LDS only (lds_barrier): global
+// stores and atomics stay in flight until the kernel ends.  The ring
+// (D >= W + max delay + 1 slots) and the gate history (DN >= W + max NMDA
+// delay + 1) are deep enough that no write of the launch lands in a slot some
+// block still has to read.  UNI: replicas in whole waves with L == 1 (the
+// row's tables are scalar loads).
+template <typename T, bool REPLAY, bool UNI>
+__global__ void __launch_bounds__(512) k_window(Net<T> net, int32_t k0, int32_t W) {
+  extern __shared__ __align__(16) unsigned char wsm[];
   const int L = net.gather_lanes;
-  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nn = gt / L;
-  const int sl = (int)(gt % L);
-  // update lanes: the block's first blockDim/L lanes, one neuron each (the
-  // gather groups hand their sums over through LDS), so the waves past them
-  // skip the update
-  const int npb = (int)blockDim.x / L;
-  const int64_t nu = (int64_t)blockIdx.x * npb + threadIdx.x;
-  const bool act = (int)threadIdx.x < npb && nu < net.N;
-  const int32_t n = act ? (int32_t)nu : net.N;
-  __shared__ T s_gn[512];
-  const int sub = (int)(blockIdx.x % NSUB);
-  const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-  const int wpb = (int)(blockDim.x >> 6);
+  const int npb = (int)blockDim.x / L;  // neurons a block can hold
+  const uint64_t blk = net.wblk[blockIdx.x];
+  const int32_t nb = (int32_t)(uint32_t)blk;         // first neuron of the block
+  const int nbc = (int)(uint32_t)(blk >> 32);        // its neurons (<= npb)
   const int32_t N = net.N, B = net.B;
-  const uint32_t off = net.sub_off[sub], cap = net.sub_cap[sub];
-  // registers across the window
+  const int tid = (int)threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wpb = (int)(blockDim.x >> 6);
+  const int nlu = __builtin_popcount((uint32_t)net.lin_mask);
+  const int nst = ((npb + 63) / 64) * 64 * W;  // spike staging entries (64 W per update wave)
+  // LDS carve-up (window_lds_bytes)
+  uint64_t *s_prow = (uint64_t *)wsm;                         // [npb] push row start
+  T *s_gn = (T *)(s_prow + npb);                              // [W][npb] NMDA sums
+  T *s_inc = s_gn + (size_t)W * npb;                          // [nlu][W][npb] ring slots
+  T *s_hs = s_inc + (size_t)nlu * W * npb;                    // [W][npb] NMDA gates of each step
+  T *s_amp = s_hs + (size_t)W * npb;                          // [nst] staged push amplitudes
+  uint32_t *s_pdeg = (uint32_t *)(s_amp + nst);               // [npb] push row length
+  uint32_t *s_draw = s_pdeg + npb;                            // [W][npb] event outcomes
+  float *s_inj = (float *)(s_draw + (size_t)W * npb);         // [W][npb] injection draws
+  uint32_t *s_spk = (uint32_t *)(s_inj + (size_t)W * npb);    // [nst] step << 24 | neuron
+  __shared__ uint32_t s_wcnt[8], s_base;
+  if (tid < 8) s_wcnt[tid] = 0u;
   const int64_t tb = *net.t0;
   const int32_t tm0 = net.tmod[0], tm1 = net.tmod[1];
+
+  // update lanes: npb consecutive threads, one neuron each, starting at a
+  // wave that rotates with the block index, so the update waves of blocks
+  // sharing a compute unit do not all land on one SIMD
+  const int ubase = (int)(blockIdx.x % (unsigned)wpb) * 64;
+  const int ul = (tid - ubase + (int)blockDim.x) % (int)blockDim.x;  // update-lane index
+  const int uw = ul >> 6;                                            // update wave ordinal
+  const bool act = ul < nbc;
+  const int32_t n = act ? nb + ul : N;
   int32_t nl = 0, rep = 0;
   uint32_t p0 = 0;
   NState<T> S;
   NConst<T> C;
-  uint64_t pr0 = 0;
-  uint32_t pdeg = 0;
-  T lg[4] = {T(0), T(0), T(0), T(0)};  // the replica's linear gains
-  const DevPop<T> *Pp = net.pops;
   if (act) {
     locate<false>(net.npop, B, n, nl, rep, p0);
-    Pp = net.pops + p0;
-    load_state(net, *(PopC<T> *)Pp, n, slot_of(tm1, k0 - 1, net.DN), S);
-    load_const(net, *(PopC<T> *)Pp, nl, rep, p0, C);
-    pr0 = ldc(net.rp_p + nl);
-    pdeg = (uint32_t)(ldc(net.rp_p + nl + 1) - pr0);
-#pragma unroll
-    for (int li = 0; li < 4; ++li) lg[li] = T(net.gainr[lin_rid(li) * B + rep]);
+    load_const(net, *(PopC<T> *)(net.pops + p0), nl, rep, p0, C);
+    load_state(net, nl, rep, n, slot_of(tm1, k0 - 1, net.DN), S);
+    const uint64_t r0 = ldc(net.rp_p + nl);
+    s_prow[ul] = r0;
+    s_pdeg[ul] = (uint32_t)(ldc(net.rp_p + nl + 1) - r0);
   }
-  RowCache<T> rc;
-  if (net.has_nmda_syn) row_cache_load(net, nn, L, sl, rc);
-  if (threadIdx.x == 0) s_wn = 0u;
-  unsigned long long events = 0, nout = 0, nspk = 0;
-  auto flush = [&](uint32_t wn) {  // block-uniform call
-    if (threadIdx.x == 0) s_base = atomicAdd(net.total + sub, wn);
-    __syncthreads();
-    const uint32_t base = s_base;
-    for (uint32_t i = threadIdx.x; i < wn; i += blockDim.x) {
-      const uint32_t r = base + i;
+
+  // ---- A: prefetch the window's inputs.  Thread -> (neuron ir, steps s0,
+  // s0 + L, ...) for the ring slots and the draws (blockDim = npb * L): their
+  // loads are issued first and consumed after the NMDA gather, whose latency
+  // they share.
+  const int ir = tid % npb, s0 = tid / npb;
+  const int32_t nr = nb + ir;
+  const bool rl = ir < nbc;  // this thread's neuron of the ring / draw work is in the block
+  constexpr int RQ = 3;      // ring slots per receptor held in registers across the gather
+  T rv[4][RQ];
+  NConst<T> Cr;
+  uint32_t pr = 0;
+  if (rl) {
+#pragma unroll
+    for (int li = 0; li < 4; ++li) {
+#pragma unroll
+      for (int j = 0; j < RQ; ++j) {
+        const int32_t s = s0 + j * L;
+        rv[li][j] = ((net.lin_mask & (1 << li)) && s < W)
+                        ? net.ring[((size_t)li * net.D + slot_of(tm0, k0 + s - 1, net.D)) * (size_t)N + nr]
+                        : T(0);
+      }
+    }
+    int32_t nlr, repr;
+    locate<false>(net.npop, B, nr, nlr, repr, pr);
+    load_const(net, *(PopC<T> *)(net.pops + pr), nlr, repr, pr, Cr);
+  }
+  if (net.has_nmda_syn)
+    gather_window<T, UNI>(net, tid / L < nbc ? (int64_t)nb + tid / L : (int64_t)N, slot_of(tm1, k0 - 1, net.DN),
+                          W, L, tid % L, s_gn, npb, tid / L);
+  if (rl) {
+    int c = 0;
+#pragma unroll
+    for (int li = 0; li < 4; ++li) {
+      if (!(net.lin_mask & (1 << li))) continue;
+      int j = 0;
+      for (int32_t s = s0; s < W; s += L, ++j) {
+        const T v = j < RQ ? (j == 0 ? rv[li][0] : j == 1 ? rv[li][1] : rv[li][2])
+                           : net.ring[((size_t)li * net.D + slot_of(tm0, k0 + s - 1, net.D)) * (size_t)N + nr];
+        s_inc[((size_t)c * W + s) * npb + ir] = v;
+      }
+      ++c;
+    }
+    PopC<T> &Pr = *(PopC<T> *)(net.pops + pr);
+    for (int32_t s = s0; s < W; s += L) {
+      uint32_t db;
+      float iv;
+      neuron_draws<T, REPLAY>(net, Pr, Cr, nr, tb + k0 + s, db, iv);
+      s_draw[s * npb + ir] = db;
+      s_inj[s * npb + ir] = iv;
+    }
+  }
+  lds_barrier();
+
+  // ---- B: the W updates of the block's neurons, spikes staged per wave;
+  // one pass per distinct population among the wave's lanes (usually one),
+  // with that population's parameters copied into registers by scalar loads
+  uint32_t hn = 0u;  // the lane's population has NMDA outputs
+  if (uw * 64 < npb) {
+    uint32_t wcnt = 0u;  // spikes this wave staged (wave-uniform)
+    uint32_t mine = 0u;  // spikes of this lane's neuron
+    const uint32_t wbase = (uint32_t)uw * 64u * (uint32_t)W;
+    uint64_t todo = __ballot(act);
+    while (todo) {
+      const int ld = __ffsll((unsigned long long)todo) - 1;
+      const uint32_t pc = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(p0, ld));
+      const bool here = act && p0 == pc;
+      todo &= ~__ballot(here);
+      const DevPop<T> P = net.pops[pc];
+      if (here) hn = P.has_nmda_out;
+      for (int32_t s = 0; s < W; ++s) {
+        const int32_t k = k0 + s;
+        bool spk = false;
+        T a_push = T(0);
+        if (here) {
+          T inc[4] = {T(0), T(0), T(0), T(0)};
+          int c = 0;
+#pragma unroll
+          for (int li = 0; li < 4; ++li) {
+            if (!(net.lin_mask & (1 << li))) continue;
+            inc[li] = s_inc[((size_t)c * W + s) * npb + ul];
+            ++c;
+          }
+          const T gn = net.has_nmda_syn ? s_gn[s * npb + ul] : T(0);
+          neuron_core<T, REPLAY, true, const DevPop<T>>(net, P, C, n, tb + k, 0, gn, inc, s_draw[s * npb + ul],
+                                                        s_inj[s * npb + ul], S, spk, a_push);
+          s_hs[s * npb + ul] = S.sn;
+        }
+        const uint64_t mask = __ballot(spk);
+        if (mask) {
+          if (spk) {
+            const uint32_t pos = wbase + wcnt + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            s_spk[pos] = ((uint32_t)s << 24) | (uint32_t)n;
+            s_amp[pos] = a_push;
+            ++mine;
+          }
+          wcnt += (uint32_t)__popcll(mask);
+        }
+      }
+    }
+    if (act) store_state(net, n, S);
+    // per-population spike counts, one atomic when the wave's spikers share a population
+    if (wcnt) {
+      const uint32_t pid = p0 * (uint32_t)B + (uint32_t)rep;
+      const uint64_t who = __ballot(mine != 0u);
+      const int leader = __ffsll((unsigned long long)who) - 1;
+      const uint32_t q0 = __shfl(pid, leader);
+      if (__ballot(mine != 0u && pid != q0) == 0ull) {
+        if (lane == leader) atomicAdd(&net.pop_cnt[q0], (unsigned long long)wcnt);
+      } else if (mine) {
+        atomicAdd(&net.pop_cnt[pid], (unsigned long long)mine);
+      }
+    }
+    if (lane == 0) s_wcnt[uw] = wcnt;
+  }
+  lds_barrier();
+
+  // ---- C: push the staged spikes, one wave per spike
+  const int nwu = (npb + 63) / 64;
+  uint32_t cnt = 0u;
+  for (int w = 0; w < nwu; ++w) cnt += s_wcnt[w];
+  auto entry = [&](uint32_t q) -> uint32_t {  // position of spike q of the concatenated lists
+    int w = 0;
+    while (q >= s_wcnt[w]) q -= s_wcnt[w++];
+    return (uint32_t)w * 64u * (uint32_t)W + q;
+  };
+  unsigned long long events = 0, nout = 0;
+  for (uint32_t q = (uint32_t)wave; q < cnt; q += (uint32_t)wpb) {
+    const uint32_t ent = entry(q);
+    const uint32_t pk = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_spk[ent]);
+    const uint32_t j = pk & 0x00FFFFFFu, s = pk >> 24;
+    const T amp = s_amp[ent];
+    const uint32_t jl = j / (uint32_t)B, jr = j - jl * (uint32_t)B;
+    const uint64_t r0 = s_prow[j - (uint32_t)nb];
+    const uint64_t r1 = r0 + s_pdeg[j - (uint32_t)nb];
+    const T ga = amp * T(net.gainr[0 * B + jr]), gg = amp * T(net.gainr[1 * B + jr]),
+            gh = amp * T(net.gainr[3 * B + jr]), gl = amp * T(net.gainr[4 * B + jr]);
+    if (lane == 0) {
+      events += r1 - r0;
+      nout += ldc(net.nmda_out + jl);
+    }
+    const int32_t sp = slot_of(tm0, k0 + (int32_t)s, net.D);  // ring slot of the emission step
+    for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
+      const uint32_t m = ldc(net.meta_p + e);
+      const float w = ldc(net.w_p + e);
+      const uint32_t col = (m & 0x00FFFFFFu) * (uint32_t)B + jr;
+      const uint32_t li = m >> 30;
+      const T sc = li == 0 ? ga : li == 1 ? gg : li == 2 ? gh : gl;
+      int32_t slot = sp + (int32_t)((m >> 24) & 63u);
+      slot -= slot >= net.D ? net.D : 0;
+      atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(w) * sc);
+    }
+  }
+  // the ring slots read in A, cleared for reuse (no push of this launch lands there)
+  if (rl) {
+#pragma unroll
+    for (int li = 0; li < 4; ++li) {
+      if (!(net.lin_mask & (1 << li))) continue;
+      for (int32_t s = s0; s < W; s += L)
+        net.ring[((size_t)li * net.D + slot_of(tm0, k0 + s - 1, net.D)) * (size_t)N + nr] = T(0);
+    }
+  }
+  // the window's NMDA gates into the history (consecutive slots of the neuron)
+  if (act && hn)
+    for (int32_t s = 0; s < W; ++s)
+      net.shist[hidx(net, (uint32_t)nl, (uint32_t)rep, (uint32_t)slot_of(tm1, k0 + s, net.DN))] =
+          s_hs[s * npb + ul];
+  if (net.rec_on) {
+    // the block's range of its sub-list: (neuron, step within the batch)
+    const int sub = (int)(blockIdx.x % NSUB);
+    if (tid == 0) s_base = cnt ? atomicAdd(net.total + sub, cnt) : 0u;
+    lds_barrier();
+    const uint32_t off = net.sub_off[sub], cap = net.sub_cap[sub];
+    for (uint32_t q = (uint32_t)tid; q < cnt; q += blockDim.x) {
+      const uint32_t pk = s_spk[entry(q)];
+      const uint32_t r = s_base + q;
       if (r < cap) {
-        net.rec_ids[off + r] = w_ids[i];
-        net.rec_k[off + r] = w_k[i];
+        net.rec_ids[off + r] = pk & 0x00FFFFFFu;
+        net.rec_k[off + r] = (uint32_t)k0 + (pk >> 24);
       } else {
         atomicOr(net.overflow, 1u);
       }
     }
-    __syncthreads();
-  };
-  for (int32_t s = 0; s < W; ++s) {
-    const int32_t k = k0 + s;
-    const int64_t t = tb + k;
-    T inc[4] = {T(0), T(0), T(0), T(0)};
-    if (act) ring_take(net, n, slot_of(tm0, k - 1, net.D), inc);
-    T gn = T(0);
-    if (net.has_nmda_syn) {
-      gn = gather_cached(net, rc, nn, slot_of(tm1, k - 1, net.DN), L, sl);
-      if (L > 1) {
-        if (sl == 0) s_gn[threadIdx.x / L] = gn;
-        __syncthreads();
-        gn = act ? s_gn[threadIdx.x] : T(0);
-      }
-    }
-    bool spk = false;
-    T a_push = T(0);
-    if (act)
-      neuron_core<T, REPLAY, true>(net, *(PopC<T> *)Pp, C, n, t, slot_of(tm1, k, net.DN), gn, inc, S, spk,
-                             a_push);
-    // the step's spikes of this block: wave ballots, one LDS count
-    if (threadIdx.x == 0) s_cnt = 0u;
-    __syncthreads();
-    const uint64_t mask = __ballot(spk);
-    const int leader = mask ? __ffsll((unsigned long long)mask) - 1 : 0;
-    const uint32_t cntw = (uint32_t)__popcll(mask);
-    uint32_t woff = 0u;
-    if (mask && lane == leader) woff = atomicAdd(&s_cnt, cntw);
-    woff = __shfl(woff, leader);
-    __syncthreads();
-    const uint32_t cnt = s_cnt;
-    uint32_t wn = s_wn;
-    if (cnt && wn + cnt > (uint32_t)WREC) {
-      flush(wn);
-      wn = 0u;
-    }
-    if (spk) {
-      const uint32_t pos = woff + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      s_ids[pos] = (uint32_t)n;
-#pragma unroll
-      for (int li = 0; li < 4; ++li) s_sc[li][pos] = a_push * lg[li];
-      s_r0[pos] = pr0;
-      s_deg[pos] = pdeg;
-      w_ids[wn + pos] = (uint32_t)n;
-      w_k[wn + pos] = (uint32_t)k;
-      const uint32_t pid = p0 * (uint32_t)B + (uint32_t)rep;
-      const uint32_t q0 = __shfl(pid, leader);
-      const bool same = __ballot(spk && pid != q0) == 0ull;
-      if (same) {
-        if (lane == leader) atomicAdd(&net.pop_cnt[q0], (unsigned long long)cntw);
-      } else {
-        atomicAdd(&net.pop_cnt[pid], 1ull);
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      s_wn = wn + cnt;
-      nspk += cnt;
-    }
-    // push of the block's spikes of step k: one wave per spike, lanes over its row
-    const int32_t s0 = slot_of(tm0, k, net.D);
-    for (uint32_t i = (uint32_t)wave; i < cnt; i += (uint32_t)wpb) {
-      const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ids[i]);
-      const uint32_t jl = j / (uint32_t)B, jr = j - jl * (uint32_t)B;
-      const uint64_t r0 = s_r0[i], r1 = r0 + s_deg[i];
-      const T ga = s_sc[0][i], gg = s_sc[1][i], gh = s_sc[2][i], gl = s_sc[3][i];
-      if (lane == 0) { events += r1 - r0; nout += ldc(net.nmda_out + jl); }
-      for (uint64_t e = r0 + (uint64_t)lane; e < r1; e += 64) {
-        const uint32_t m = ldc(net.meta_p + e);
-        const float w = ldc(net.w_p + e);
-        const uint32_t col = (m & 0x00FFFFFFu) * (uint32_t)B + jr;
-        const uint32_t li = m >> 30;
-        const T sc = li == 0 ? ga : li == 1 ? gg : li == 2 ? gh : gl;
-        int32_t slot = s0 + (int32_t)((m >> 24) & 63u);
-        slot -= slot >= net.D ? net.D : 0;
-        atomic_add_t(net.ring + ((size_t)li * net.D + slot) * N + col, T(w) * sc);
-      }
-    }
   }
-  __syncthreads();
-  const uint32_t wn = s_wn;
-  if (wn) flush(wn);
-  if (act) store_state(net, *(PopC<T> *)Pp, n, S);
-  if (threadIdx.x == 0 && nspk) atomicAdd(&net.counters[blockIdx.x & (NCTR - 1)], nspk);
+  if (tid == 0 && cnt) atomicAdd(&net.counters[blockIdx.x & (NCTR - 1)], (unsigned long long)cnt);
   block_add_events(net, events, nout);
 }
 
@@ -1367,7 +1503,7 @@ __global__ void __launch_bounds__(256) k_learn(Net<T> net, int32_t k) {
   const int r = net.l_rec[i];
   const size_t N = (size_t)net.N;
   T w = net.l_w[i];
-  const T gate = r == 2 ? net.shist[(size_t)hslot * N + a] : net.spre[a];
+  const T gate = r == 2 ? net.shist[hidx(net, a, 0u, (uint32_t)hslot)] : net.spre[a];
   atomic_add_t(net.gl + (size_t)r * N + b, w * gate);
   const DevPop<T> &Pa = net.pops[net.npop[a]];
   if (net.cs[a] & F_SPIKE) {
@@ -1414,6 +1550,15 @@ __global__ void k_read_g(Net<T> net, int64_t t, int li, T *out) {
   const int32_t slot = (int32_t)(((t % net.D) + net.D) % net.D);
   out[n] = P.dfst[lin_rid(li)] * net.G[(size_t)li * net.N + n] +
            net.ring[((size_t)li * net.D + slot) * net.N + n];
+}
+
+// NMDA presynaptic gate of every neuron at history slot `slot` (state readback)
+template <typename T>
+__global__ void k_read_hist(Net<T> net, int32_t slot, T *out) {
+  const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= net.N) return;
+  const uint32_t nl = (uint32_t)n / (uint32_t)net.B;
+  out[n] = net.shist[hidx(net, nl, (uint32_t)n - nl * (uint32_t)net.B, (uint32_t)slot)];
 }
 
 }  // namespace fsk
