@@ -5,7 +5,7 @@ Runs only where the reference sources exist (this build container).  For
 each case it writes the network in the reference's text format, runs the
 reference with `-t 1 -rp 1` and keeps only small outputs: spike rasters as
 (step, neuron) pairs, synapse orders from network.log, the bytes of small
-output files, and digests.  Usage (repo root):  python tests/golden/make_golden.py [plasticity|noise|logs [dir]]
+output files, and digests.  Usage (repo root):  python tests/golden/make_golden.py [plasticity|noise|achgcl|logs [dir]]
 """
 from __future__ import annotations
 
@@ -275,6 +275,97 @@ population:Inh
 EndOutputFile
 EndOutControl
 """
+
+
+# ACh and GluCl (code/EqsGnl2.cpp:307-325, 404-502, 937-1002): external drive
+# on both receptors, group links (two or more targets) between ExcA and ExcB
+# and one-target links through Solo1 / Solo2 (the single-synapse path).
+# Every ACh group link has a GluCl twin with the same targets (Connectivity
+# 1, so no shuffle): the reference builds its GluCl group rows' target map
+# from the ACh rows (code/NetSimGnl2_BulPp.cpp:549-550), which is only
+# defined, and then equal to the GluCl rows' own targets, when the two row
+# structures coincide.  The linear receptors share one time constant (4 ms):
+# every population projects onto several of them, and the engine keeps one
+# presynaptic linear gate per neuron.
+def _lif(name, n, receptors, targets, extra=""):
+    return ("NeuralPopulation: %s\nN=%d\nC=0.5\nTaum=20\nThreshold=-50\nRestPot=-70\nResetPot=-55\n"
+            "RefractoryPeriod=2\nSpikeDly=0.5\n%s%s%sEndNeuralPopulation\n\n"
+            % (name, n, extra, receptors, targets))
+
+
+def _rec(kind, tau, rev, fext=None, eff=None):
+    s = "Receptor:%s\nTau=%s\nRevPot=%s\n" % (kind, tau, rev)
+    if fext is not None:
+        s += "FreqExt=%s\nMeanExtEff=%s\n" % (fext, eff)
+    return s + "EndReceptor\n"
+
+
+def _tgt(pop, rec, eff, conn=None):
+    s = "TargetPopulation: %s\nTargetReceptor=%s\nMeanEff=%s\n" % (pop, rec, eff)
+    if conn is not None:
+        s += "Connectivity=%s\n" % conn
+    return s + "EndTargetPopulation\n"
+
+
+ACHGCL_CONF = (
+    _lif("ExcA", 30, _rec("AMPA", 4, 0, 2400, 1.25) + _rec("Ach", 4, 0, 900, 0.7) + _rec("GluCl", 4, -70, 500, 0.4)
+         + _rec("GABA", 5, -70),
+         _tgt("ExcB", "Ach", 0.35, 1) + _tgt("ExcB", "GluCl", 0.2, 1) + _tgt("ExcA", "AMPA", 0.6, 0.3))
+    + _lif("ExcB", 20, _rec("AMPA", 4, 0, 2200, 1.2) + _rec("Ach", 4, 0) + _rec("GluCl", 4, -70),
+           _tgt("ExcA", "Ach", 0.3, 1) + _tgt("ExcA", "GluCl", 0.25, 1) + _tgt("Solo1", "Ach", 1.1))
+    + _lif("Solo1", 1, _rec("AMPA", 4, 0, 2600, 1.4) + _rec("Ach", 4, 0, 700, 0.9) + _rec("GluCl", 4, -70),
+           _tgt("Solo2", "GluCl", 1.5) + _tgt("ExcB", "Ach", 0.9, 0.05))
+    + _lif("Solo2", 1, _rec("AMPA", 4, 0, 2500, 1.35) + _rec("Ach", 4, 0) + _rec("GluCl", 4, -70, 800, 0.5),
+           _tgt("Solo1", "Ach", 1.2) + _tgt("ExcA", "GluCl", 0.8, 0.03)))
+
+ACHGCL_PRO = """DefineMacro
+RandomSeeds: 13 EndRandomSeeds
+EndDefineMacro
+
+EventTime 120.0
+Type=ChangeExtFreq
+Label=#1#
+Population: ExcA
+Receptor: Ach
+FreqExt=1500.0
+EndEvent
+
+EventTime 200.0
+Type=ChangeExtFreq
+Label=#2#
+Population: Solo2
+Receptor: GluCl
+FreqExt=0
+EndEvent
+
+EventTime 300.0
+Type=EndTrial
+Label=End_of_the_trial
+EndEvent
+
+OutControl
+FileName:Spikes.dat
+Type=Spike
+population:AllPopulation
+EndOutputFile
+EndOutControl
+"""
+
+
+def achgcl_case(store, files):
+    d = tempfile.mkdtemp(prefix="golden_")
+    open(os.path.join(d, "network.conf"), "w").write(ACHGCL_CONF)
+    open(os.path.join(d, "network.pro"), "w").write(ACHGCL_PRO)
+    out = run_ref(d, ["-t", "1"])
+    store["achgcl_raster"] = raster(os.path.join(d, "Spikes.dat"))
+    for k, v in syn_sections(os.path.join(d, "network.log")).items():
+        store["achgcl_" + k] = v
+    files["achgcl_network.conf"] = ACHGCL_CONF
+    files["achgcl_network.pro"] = ACHGCL_PRO
+    meta = {"total_synapses": total_synapses(out), "spikes": int(len(store["achgcl_raster"])),
+            "sections": sorted(k for k in store if k.startswith("achgcl_") and k != "achgcl_raster")}
+    shutil.rmtree(d)
+    return meta
 
 
 def module_case(store, files):
@@ -772,6 +863,21 @@ def update_plasticity():
     print(json.dumps(meta["ltp"], indent=1))
 
 
+def update_achgcl():
+    """Regenerate only the ACh / GluCl fixture, keeping the others as stored."""
+    build_ref()
+    store = dict(np.load(os.path.join(HERE, "reference_runs.npz")))
+    files = json.load(open(os.path.join(HERE, "reference_files.json")))
+    meta = json.load(open(os.path.join(HERE, "reference_meta.json")))
+    meta["achgcl"] = achgcl_case(store, files)
+    np.savez_compressed(os.path.join(HERE, "reference_runs.npz"), **store)
+    with open(os.path.join(HERE, "reference_files.json"), "w") as f:
+        json.dump(files, f, indent=0)
+    with open(os.path.join(HERE, "reference_meta.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta["achgcl"], indent=1))
+
+
 def main():
     build_ref()
     store, files, meta = {}, {}, {}
@@ -789,6 +895,7 @@ def main():
     meta["gap"] = gap_case(store, files)
     meta["c1_stp"] = c1_case("c1_stp", 1, 1.25, 1000.0, store, args=["-STP"])
     meta["ltp"] = ltp_case(store, files)
+    meta["achgcl"] = achgcl_case(store, files)
     meta["logs"] = logs_case(files)
     sz = seizure_case()
     if sz:
@@ -806,6 +913,8 @@ if __name__ == "__main__":
         update_plasticity()
     elif sys.argv[1:] == ["noise"]:
         update_noise()
+    elif sys.argv[1:] == ["achgcl"]:
+        update_achgcl()
     elif sys.argv[1:2] == ["logs"]:
         update_logs(sys.argv[2] if len(sys.argv) > 2 else None)
     else:

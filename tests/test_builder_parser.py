@@ -44,7 +44,7 @@ def _order(L, group, rid):
 
 SECTIONS = {"SynAMPA1": (0, 0), "SynGABA1": (0, 1), "SynNMDA1": (0, 2), "SynACH1": (0, 3),
             "SynGCL1": (0, 4), "PSynAMPA": (1, 0), "PSynGABA": (1, 1), "PSynNMDA": (1, 2),
-            "PSynACH": (1, 3)}
+            "PSynACH": (1, 3), "PSynGCL": (1, 4)}
 
 
 def _check_sections(L, runs, prefix):
@@ -88,6 +88,31 @@ def test_module_group_gates_shuffle_and_seeds(golden):
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_CODE), reason="reference example files absent")
+def test_ach_gcl_synapse_orders_and_oracle_raster(golden):
+    """ACh and GluCl links, group (two or more targets) and single, with
+    external drive on both receptors and ChangeExtFreq events on them: the
+    builder's synapse arrays equal the reference's network.log sections, and
+    the CPU restatement (FP32 and FP64) reproduces the reference's 300 ms
+    raster.  The ACh group map holds two more entries than the GluCl one
+    (Connectivity 0.05 of 20 neurons is two targets in float), so the
+    reference's GluCl rows, whose targets it reads through the ACh row map
+    (code/NetSimGnl2_BulPp.cpp:549-550), are exercised with shifted indices
+    that stay inside each row."""
+    from oracle.oracle import Oracle
+    f = golden["files"]
+    with tempfile.TemporaryDirectory() as d:
+        c = Conf(*_write(d, f["achgcl_network.conf"], f["achgcl_network.pro"]))
+        L = c.ref_layout()
+        n = _check_sections(L, golden["runs"], "achgcl_")
+        assert n == c.total_synapses == golden["meta"]["achgcl"]["total_synapses"]
+        assert len(golden["runs"]["achgcl_PSynACH"]) == len(golden["runs"]["achgcl_PSynGCL"]) + 2
+        ref = golden["runs"]["achgcl_raster"].astype(np.int64)
+        for prec in (0, 1):
+            o = Oracle(L, c.n_neuron, precision=prec)
+            o.reset(c.seed(0))
+            np.testing.assert_array_equal(o.raster(3000), ref)
+
+
 def test_reference_example_network():
     with tempfile.TemporaryDirectory() as d:
         import shutil
