@@ -46,7 +46,9 @@ def parse():
     p.add_argument("--injection", action="store_true",
                    help="the seizure protocol's form: Gaussian current injection on every "
                         "population from step 0 (GaussMean ~ U(0.04, 0.16) nA, GaussSTD = 6 x mean)")
-    p.add_argument("--schedule", default="", help="json dict of fs_schedule overrides")
+    p.add_argument("--schedule", default="",
+                   help="json dict of fs_schedule overrides (default: autotuned on this device "
+                        "before the timed region)")
     p.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                    help="only run 10 device-to-device copies of MIB MiB (the known-size "
                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -273,8 +275,13 @@ def main():
     arr = arrays_from_network(net)
     prec = A.FS_FP64 if args.fp64 else A.FS_FP32
     eng = Engine(arr, precision=prec, ext_mode=A.FS_EXT_PHILOX, record=False)
+    tuned = None
     if args.schedule:
         eng.set_schedule(**json.loads(args.schedule))
+    else:
+        from flysim_amd.tune import tune
+        eng.reset(2024)
+        tuned = tune(eng, cache=None)   # timed live on this device, outside the timed region
     eng.reset(2024)
     if args.injection:
         means = np.random.default_rng(4).uniform(0.04, 0.16, size=len(arr.pops))
@@ -293,6 +300,11 @@ def main():
                                                           res["dtype"])
     if traffic:
         res["roofline"]["traffic"] = traffic
+    res["config"]["schedule_source"] = "override" if args.schedule else "autotuned"
+    if tuned:
+        res["config"]["tuning"] = {"candidates": len(tuned["trials"]), "seconds": round(tuned["tune_s"], 2),
+                                   "trial_steps": tuned["trial_steps"],
+                                   "time_vs_default_schedule": round(tuned["vs_start"], 4)}
     eng.close()
     if not args.no_sweep and args.workload == "fly20k":
         res["sweep"] = sweep_figures()

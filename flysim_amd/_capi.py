@@ -128,7 +128,7 @@ EXPORTS = [
     "fs_set_replica", "fs_create_ei_batch", "fs_set_ext_rates", "fs_mlp_create", "fs_mlp_destroy", "fs_mlp_n_params", "fs_mlp_get_params",
     "fs_mlp_set_params", "fs_mlp_loss_grad", "fs_mlp_predict", "fs_mlp_predict_noalloc",
     "fs_mlp_set_data", "fs_mlp_train",
-    "fs_device_alloc", "fs_device_free", "fs_device_copy",
+    "fs_device_alloc", "fs_device_free", "fs_device_copy", "fs_network_stats", "fs_device_info",
 ]
 
 _lib = None
@@ -173,6 +173,8 @@ def lib() -> C.CDLL:
             "fs_kernel_times": [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double),
                                 P(C.c_int64)],
             "fs_run_counters": [C.c_void_p, C.c_void_p],
+            "fs_network_stats": [C.c_void_p, C.c_void_p],
+            "fs_device_info": [C.c_char_p, C.c_int32, P(C.c_int32)],
             "fs_destroy": [C.c_void_p],
             "fs_conf_load": [C.c_char_p, C.c_char_p, P(CliOpts), P(C.c_void_p)],
             "fs_conf_free": [C.c_void_p],

@@ -123,14 +123,21 @@ def subset(spec: EiBatchSpec, idx) -> EiBatchSpec:
                        spec.g_lo, spec.g_hi, spec.nmda_ratio, spec.dt)
 
 
-def run_batch(spec: EiBatchSpec, steps: int, seed: int = 1, schedule=None, log=print):
-    """Generate and run all instances in one handle; per-instance E/I rates."""
+def run_batch(spec: EiBatchSpec, steps: int, seed: int = 1, schedule="auto", log=print):
+    """Generate and run all instances in one handle; per-instance E/I rates.
+    schedule: fs_schedule overrides, or "auto" to autotune the launch
+    schedule first (cached by network signature and device)."""
     import time
     from . import _capi as A
     t0 = time.perf_counter()
     eng = Engine.ei_batch(spec, precision=A.FS_FP32, record=False)
     t_gen = time.perf_counter() - t0
-    if schedule:
+    tuned = None
+    if schedule == "auto":
+        from .tune import tune
+        eng.reset(seed)
+        tuned = tune(eng)
+    elif schedule:
         eng.set_schedule(**schedule)
     eng.reset(seed)
     eng.enable_timing(True)
@@ -144,7 +151,8 @@ def run_batch(spec: EiBatchSpec, steps: int, seed: int = 1, schedule=None, log=p
     sizes = np.stack([ne, spec.n_neuron - ne], axis=1).astype(np.float64)
     rates = counts / np.maximum(sizes, 1) / (steps * spec.dt * 1e-3)
     out = {"t_gen_s": t_gen, "wall_s": wall, "ms_per_step": wall / steps * 1e3,
-           "counters": ctr, "kernel_times": kt, "rates": rates, "schedule": eng.schedule}
+           "counters": ctr, "kernel_times": kt, "rates": rates, "schedule": eng.schedule,
+           "tune_s": tuned["tune_s"] if tuned else 0.0}
     eng.close()
     return out
 
@@ -159,12 +167,13 @@ def main(argv=None):
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--plain", type=int, default=256,
                     help="instances also run one by one (0: skip)")
-    ap.add_argument("--lanes", type=int, default=16, help="NMDA gather lanes per row")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="NMDA gather lanes per row (0: autotuned schedule)")
     ap.add_argument("--out", default="")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     spec = c4_spec(n_inst=a.instances)
-    sched = {"gather_lanes": a.lanes}
+    sched = {"gather_lanes": a.lanes} if a.lanes else "auto"
     n = spec.n_neuron.astype(np.float64)
     ne = spec.n_exc().astype(np.float64)
     pairs_expect = float(np.sum(n * (n - 1)) * spec.p_conn)
@@ -182,10 +191,17 @@ def main(argv=None):
     # gather bytes per step: 8 B per NMDA term (column+weight) + a 4 B gate read
     res["batched"]["gather_bytes_per_step"] = 12.0 * nmda + 8.0 * n.sum()
     if a.plain:
+        plain_sched = sched
+        if sched == "auto":
+            # one schedule for every plain run: the one tuned on the largest
+            # instance (tuning each of them would dominate their run time)
+            big = int(np.argmax(spec.n_neuron))
+            plain_sched = run_batch(subset(spec, big), 10, schedule="auto")["schedule"]
+            res["plain_schedule"] = plain_sched
         t0 = time.perf_counter()
         walls = []
         for k in range(min(a.plain, n.size)):
-            r = run_batch(subset(spec, k), a.steps, schedule=sched)
+            r = run_batch(subset(spec, k), a.steps, schedule=plain_sched)
             walls.append(r["t_gen_s"] + r["wall_s"])
             if (k + 1) % 16 == 0:
                 log("plain: %d instances, %.1f s" % (k + 1, time.perf_counter() - t0))

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -63,6 +64,7 @@ struct EngineBase {
   virtual void pop_counts(uint64_t *out) = 0;
   virtual void times(double *u, double *p, double *g, int64_t *n) = 0;
   virtual void counters(int64_t *out) = 0;
+  virtual void stats(int64_t *out) const = 0;
   virtual bool fused_active() const = 0;
   virtual int32_t window_active() const = 0;
   virtual void record_currents(bool on) = 0;
@@ -107,6 +109,7 @@ struct Engine final : EngineBase {
   // windowed path: steps per launch, at most 1 + the smallest synaptic delay
   // and at most what the block's LDS holds (window_lds_bytes <= 64 KiB)
   int32_t win_max = 1;
+  int32_t mind = 0, maxdp = 0, maxdn = 0;  // smallest delay; largest linear / NMDA delay (steps)
   int32_t window() const {
     if (sched.window <= 0) return 0;
     const int npb = std::max(1, sched.update_block / sched.gather_lanes);
@@ -422,6 +425,9 @@ struct Engine final : EngineBase {
     init_pops(pg);
     const int64_t n_chem = d.n_syn - gap_ends / 2;
     win_max = n_chem > 0 ? std::min(mind_all + 1, 64) : 64;
+    mind = n_chem > 0 ? mind_all : 0;
+    maxdp = maxd_p;
+    maxdn = maxd_n;
     // gap currents use every neuron's potential of the step before: one
     // step per launch, after the whole update (the two-launch path)
     if (gap_ends || ltp) win_max = 0;
@@ -796,12 +802,12 @@ struct Engine final : EngineBase {
     // launch), with graphs of whole windows
     sched.window = win_max >= 2 ? 64 : 0;
     if (sched.window) {
-      // 10..100 windows per graph: each batch ends in a host round trip
+      // up to 100 windows per graph: each batch ends in a host round trip
       // (overflow check, spike readback), so long batches amortise it; the
       // spike record (worst case every neuron every step) stays within 2^26
-      // entries
+      // entries, down to one window per graph for the largest networks
       const int64_t w = std::min(sched.window, win_max);
-      const int64_t per = std::max<int64_t>(10, std::min<int64_t>(100, ((int64_t)1 << 26) / ((int64_t)N * w)));
+      const int64_t per = std::max<int64_t>(1, std::min<int64_t>(100, ((int64_t)1 << 26) / ((int64_t)N * w)));
       sched.steps_per_graph = (int32_t)(per * w);
     }
     apply_schedule();
@@ -1209,9 +1215,15 @@ struct Engine final : EngineBase {
           n_graph_steps += K - w0;
         }
       }
+      // from here on the batch has run: a failure still advances the step
+      // index, so that it stays the index of the device state
+      auto batch_fail = [&](const char *msg) {
+        step += K;
+        throw std::length_error(msg);
+      };
       uint32_t ovf = 0;
       HIPCHECK(hipMemcpy(&ovf, b_overflow.p, 4, hipMemcpyDeviceToHost));
-      if (ovf) throw std::length_error("device spike record overflow; lower steps_per_graph");
+      if (ovf) batch_fail("device spike record overflow; lower steps_per_graph");
       if ((off || ids) && W) {
         // windowed path: sub-list s holds total[s] entries (id, step) in
         // arrival order; bucket them by step, ascending ids within a step
@@ -1234,12 +1246,12 @@ struct Engine final : EngineBase {
         }
         std::vector<int64_t> kst(K + 1, 0);
         for (uint64_t i = 0; i < tot; ++i) {
-          if (stage_k[i] >= (uint32_t)K) throw std::runtime_error("corrupt spike record");
+          if (stage_k[i] >= (uint32_t)K) batch_fail("corrupt spike record");
           kst[stage_k[i] + 1]++;
         }
         for (int32_t k = 0; k < K; ++k) kst[k + 1] += kst[k];
         if (ids) {
-          if (written + (int64_t)tot > cap) throw std::length_error("spike id buffer too small");
+          if (written + (int64_t)tot > cap) batch_fail("spike id buffer too small");
           std::vector<int64_t> fill(kst.begin(), kst.end() - 1);
           for (uint64_t i = 0; i < tot; ++i) ids[written + fill[stage_k[i]]++] = stage[i];
           for (int32_t k = 0; k < K; ++k) std::sort(ids + written + kst[k], ids + written + kst[k + 1]);
@@ -1255,7 +1267,7 @@ struct Engine final : EngineBase {
         uint64_t tot = 0;
         for (int s = 0; s < S; ++s) tot += cum_h[(size_t)K * S + s];
         if (ids) {
-          if (written + (int64_t)tot > cap) throw std::length_error("spike id buffer too small");
+          if (written + (int64_t)tot > cap) batch_fail("spike id buffer too small");
           stage.resize(tot);
           std::vector<uint64_t> st(S + 1, 0);
           for (int s = 0; s < S; ++s) {
@@ -1387,6 +1399,10 @@ struct Engine final : EngineBase {
   }
   void counters(int64_t *out) override {
     for (int i = 0; i < 4; ++i) out[i] = (int64_t)ctr_last[i];
+  }
+  void stats(int64_t *out) const override {
+    const int64_t v[10] = {N0, P, B, syn_lin, syn_nmda, mind, maxdp, maxdn, precision, net.ext_mode};
+    for (int i = 0; i < 10; ++i) out[i] = v[i];
   }
 };
 
@@ -1571,6 +1587,24 @@ int fs_kernel_times(fs_handle *h, double *u, double *p, double *g, int64_t *n) {
   if (!h) return fs::set_error(FS_ERR_ARG, "null handle");
   h->e->times(u, p, g, n);
   return FS_OK;
+}
+
+int fs_network_stats(fs_handle *h, int64_t out[10]) {
+  if (!h || !out) return fs::set_error(FS_ERR_ARG, "null argument");
+  h->e->stats(out);
+  return FS_OK;
+}
+
+int fs_device_info(char *name, int32_t name_cap, int32_t *compute_units) {
+  if (!name || name_cap < 1) return fs::set_error(FS_ERR_ARG, "bad argument");
+  ENGINE_CALL({
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    hipDeviceProp_t p{};
+    HIPCHECK(hipGetDeviceProperties(&p, dev));
+    std::snprintf(name, (size_t)name_cap, "%s %s", p.name, p.gcnArchName);
+    if (compute_units) *compute_units = p.multiProcessorCount;
+  })
 }
 
 int fs_run_counters(fs_handle *h, int64_t out[4]) {

@@ -302,22 +302,28 @@ class Engine:
         self._bits = np.ascontiguousarray(bits, dtype=np.uint32)
         A.check(A.lib().fs_load_ext_trains(self._h, self._bits.ctypes.data, first_step, n_steps))
 
+    # id buffer of a recorded run: the run goes in chunks whose worst case
+    # (every neuron spiking every step) fits it, so fs_run never overflows
+    RECORD_IDS = 1 << 26
+
     def run(self, n_steps: int, record: bool = True) -> Tuple[np.ndarray, np.ndarray]:
         """Run n steps; returns (offsets[n+1], ids) of the per-step spike lists."""
         if not record:
             A.check(A.lib().fs_run(self._h, n_steps, None, None, 0))
             return np.zeros(0, np.int64), np.zeros(0, np.uint32)
+        chunk = max(1, min(n_steps, self.RECORD_IDS // max(self.n, 1)))
+        ids = np.zeros(max(1024, self.n * min(chunk, n_steps)), dtype=np.uint32)
         off = np.zeros(n_steps + 1, dtype=np.int64)
-        cap = max(1024, self.n * min(n_steps, 2000))
-        while True:
-            ids = np.zeros(cap, dtype=np.uint32)
-            s0 = self.step
-            rc = A.lib().fs_run(self._h, n_steps, off.ctypes.data, ids.ctypes.data, cap)
-            if rc == -3 and self.step == s0 and cap < self.n * n_steps:
-                cap = self.n * n_steps
-                continue
-            A.check(rc)
-            return off, ids[: off[-1]].copy()
+        parts = []
+        done = 0
+        while done < n_steps:
+            k = min(chunk, n_steps - done)
+            o = np.zeros(k + 1, dtype=np.int64)
+            A.check(A.lib().fs_run(self._h, k, o.ctypes.data, ids.ctypes.data, ids.size))
+            off[done + 1: done + k + 1] = o[1:] + off[done]
+            parts.append(ids[: o[-1]].copy())
+            done += k
+        return off, (np.concatenate(parts) if parts else np.zeros(0, np.uint32))
 
     def raster(self, n_steps: int) -> np.ndarray:
         """Run n steps and return an (k, 2) int64 array of (step, nid)."""
@@ -387,6 +393,16 @@ class Engine:
         return {"update_ms": u.value, "deliver_ms": p.value, "graph_step_ms": g.value,
                 "n_timed": n.value}
 
+    def stats(self) -> Dict[str, int]:
+        """Shape of the network (fs_network_stats): neurons and populations
+        of one copy, copies, linear and NMDA receptor-synapses, smallest
+        delay, largest linear and NMDA delays (steps), precision, input mode."""
+        out = np.zeros(10, np.int64)
+        A.check(A.lib().fs_network_stats(self._h, out.ctypes.data))
+        keys = ("neurons", "pops", "replicas", "syn_lin", "syn_nmda", "min_delay", "max_delay_lin",
+                "max_delay_nmda", "precision", "ext_mode")
+        return {k: int(v) for k, v in zip(keys, out)}
+
     def counters(self) -> Dict[str, int]:
         """Counters of the last run(): spikes, linear receptor-synapses
         pushed, NMDA receptor-synapses of the spiking neurons, their sum
@@ -395,3 +411,11 @@ class Engine:
         A.check(A.lib().fs_run_counters(self._h, out.ctypes.data))
         return {"spikes": int(out[0]), "lin_pushes": int(out[1]), "nmda_out": int(out[2]),
                 "syn_events": int(out[1] + out[2]), "nmda_terms": int(out[3])}
+
+
+def device_info() -> Tuple[str, int]:
+    """Name / architecture of the current device and its compute units."""
+    buf = C.create_string_buffer(256)
+    cu = C.c_int32()
+    A.check(A.lib().fs_device_info(buf, 256, C.byref(cu)))
+    return buf.value.decode(errors="replace"), int(cu.value)

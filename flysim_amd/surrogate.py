@@ -144,11 +144,14 @@ def module_pops(net: Network) -> List[List[int]]:
 
 class SweepSimulator:
     """Per-module rates of sweep points on one network, B points per launch
-    (replicated handle); replicas=1 is the plain per-instance simulation."""
+    (replicated handle); replicas=1 is the plain per-instance simulation.
+    schedule: fs_schedule overrides, or "auto" to autotune the launch
+    schedule on this network (flysim_amd/tune.py; the result is cached by
+    network signature and device)."""
 
     def __init__(self, net: Network, replicas: int = 128, warm_steps: int = 1000,
                  meas_steps: int = 2000, precision: int = A.FS_FP32,
-                 schedule: Optional[Dict[str, int]] = None):
+                 schedule=None):
         self.net = net
         self.B = int(replicas)
         self.warm, self.meas = int(warm_steps), int(meas_steps)
@@ -158,7 +161,12 @@ class SweepSimulator:
         self.mod_size = np.array([sum(net.pops[i].n for i in m) for m in self.mods], np.float64)
         self.eng = Engine(arrays_from_network(net), precision=precision, record=False,
                           replicas=self.B)
-        if schedule:
+        self.tuned = None
+        if schedule == "auto":
+            from .tune import tune
+            self.eng.reset(0)
+            self.tuned = tune(self.eng)
+        elif schedule:
             self.eng.set_schedule(**schedule)
         self.sim_seconds = 0.0
         self.wall_seconds = 0.0

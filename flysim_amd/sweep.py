@@ -27,7 +27,7 @@ import glob
 import json
 import sys
 import time
-from typing import Dict, Optional
+from typing import Dict
 
 import numpy as np
 
@@ -44,9 +44,8 @@ def network():
 
 
 def make_sim(replicas: int, warm: int = WARM_STEPS, meas: int = MEAS_STEPS,
-             schedule: Optional[Dict[str, int]] = None) -> SweepSimulator:
-    if schedule is None:
-        schedule = {"gather_lanes": 1} if replicas >= 64 else {}
+             schedule="auto") -> SweepSimulator:
+    """The sweep's simulator; its launch schedule autotuned unless given."""
     return SweepSimulator(network(), replicas=replicas, warm_steps=warm, meas_steps=meas,
                           schedule=schedule)
 

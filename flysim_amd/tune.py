@@ -1,12 +1,17 @@
 """Autotuner of the engine's launch schedule.
 
-Times short runs of candidate schedules (neuron-update block size, NMDA
-gather lanes per row, delivery block size, steps per captured graph, and the
-step path: windowed, fused one-launch or two-launch) on the network at hand
-and keeps the fastest.  Results are cached in a JSON file keyed by a
-signature of the network's shape (neuron and synapse counts by class,
-delays, precision, device) so that sweeps of many same-shaped networks tune
-once.
+Times short runs of candidate schedules on the network at hand and keeps the
+fastest.  The knobs: the step path (windowed: as many steps per launch as the
+smallest synaptic delay allows; fused: one launch per step; two launches per
+step: update, then delivery), the block size, and the NMDA gather lanes per
+row (in the windowed path up to 16 of them take consecutive steps of one
+term, the rest split the row).  The reference's only schedule knob is its
+thread count (-t, code/NetSimGnl2.cpp:160-229).
+
+Results are cached in a JSON file keyed by a signature of the network's
+shape -- neurons, populations, copies, synapse counts by class, delay range,
+precision, input mode -- and of the device, so that sweeps of many
+same-shaped networks tune once.
 """
 from __future__ import annotations
 
@@ -14,91 +19,127 @@ import hashlib
 import itertools
 import json
 import os
+import statistics
 import time
 from typing import Dict, Iterable, List, Optional
 
-import numpy as np
-
-from .engine import Engine
+from .engine import Engine, device_info
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "tune_cache.json")
 
 
 def signature(eng: Engine, extra: str = "") -> str:
-    key = "%d|%d|%d|%d|%s" % (eng.n, eng.n_pop, getattr(eng, "replicas", 1), eng.precision, extra)
+    st = eng.stats()
+    dev, cu = device_info()
+    key = json.dumps({"net": st, "device": dev, "cu": cu, "extra": extra}, sort_keys=True)
     return hashlib.sha1(key.encode()).hexdigest()[:16]
 
 
-def candidates(replicas: int = 1, has_nmda: bool = True, fused_ok: bool = True) -> List[Dict]:
-    """Block sizes x NMDA lanes per row x step path (windowed: as many steps
-    per launch as the smallest delay allows; fused one launch per step; two
-    launches per step).  Replicas in whole waves read one CSR row per wave,
-    so one lane per row is the natural choice there; single networks spread
-    a row over 1-16 lanes."""
-    blocks = [128, 256, 512, 1024]  # the windowed path takes at most 512
-    if not has_nmda:
-        lanes = [8]
-    elif replicas >= 64:
-        lanes = [1, 2]
-    else:
-        lanes = [1, 2, 4, 8, 16]
+def candidates(stats: Dict[str, int]) -> List[Dict]:
+    """Block sizes x gather lanes per row x the step paths the network
+    admits (windowed and fused need a delay of at least one step on every
+    synapse).  Replicas in whole waves read one CSR row per wave, so one or
+    two lanes per row suit them."""
+    delayed = stats["min_delay"] >= 1
+    has_nmda = stats["syn_nmda"] > 0
+    reps = stats["replicas"]
     out = []
-    paths = [(0, 64)] + ([(1, 0)] if fused_ok else []) + [(0, 0)]
-    for b, L in itertools.product(blocks, lanes):
-        for f, w in paths:
-            if w and b > 512:
-                continue
-            out.append(dict(update_block=b, gather_lanes=L, push_block=256, steps_per_graph=100,
-                            use_graph=1, fused=f, window=w))
+    if delayed:
+        lanes = [1, 2] if reps >= 64 else ([2, 4, 8, 16, 32] if has_nmda else [8])
+        for b, L in itertools.product([128, 256, 512], lanes):
+            if L * 2 <= b:
+                out.append(dict(update_block=b, gather_lanes=L, fused=0, window=64))
+        for b, L in itertools.product([256, 512], [1, 2] if reps >= 64 else ([1, 4, 8] if has_nmda else [8])):
+            out.append(dict(update_block=b, gather_lanes=L, fused=1, window=0))
+    lanes = [1, 2] if reps >= 64 else ([1, 4, 8, 16] if has_nmda else [8])
+    for b, L in itertools.product([256, 512, 1024], lanes):
+        out.append(dict(update_block=b, gather_lanes=L, fused=0, window=0))
     return out
 
 
-def time_schedule(eng: Engine, sched: Dict, steps: int, warmup: int) -> float:
+def time_schedule(eng: Engine, sched: Dict, steps: int, warmup: Optional[int] = None) -> float:
+    """Seconds per step of `steps` steps on `sched`, after an untimed run of
+    `warmup` steps (default: as many, so that the timed run replays the
+    launch graph the warm-up captured)."""
     eng.set_schedule(**sched)
-    eng.run(warmup, record=False)
+    eng.run(steps if warmup is None else warmup, record=False)
     t0 = time.perf_counter()
     eng.run(steps, record=False)
     return (time.perf_counter() - t0) / steps
 
 
-def tune(eng: Engine, steps: int = 400, warmup: int = 100, cands: Optional[Iterable[Dict]] = None,
-         cache: Optional[str] = DEFAULT_CACHE, key_extra: str = "", verbose: bool = False) -> Dict:
-    """Pick the fastest schedule for `eng` (state is advanced by the trial
-    runs; reset the engine afterwards).  Returns {"schedule", "s_per_step",
-    "trials"} and applies the schedule."""
-    sig = signature(eng, key_extra)
-    db = {}
+def _load(cache: Optional[str]) -> Dict:
     if cache and os.path.exists(cache):
         try:
-            db = json.load(open(cache))
+            return json.load(open(cache))
         except (OSError, ValueError):
-            db = {}
+            pass
+    return {}
+
+
+def tune(eng: Engine, steps: int = 200, trial_s: float = 0.03, cands: Optional[Iterable[Dict]] = None,
+         cache: Optional[str] = DEFAULT_CACHE, key_extra: str = "", passes: int = 3, keep: int = 5,
+         verbose: bool = False) -> Dict:
+    """Pick the fastest schedule for `eng` and apply it.
+
+    The trials run one after another on the live state, whose activity can
+    drift (a network that ignites runs several times slower per step
+    afterwards), so each candidate is timed right after the engine's
+    starting schedule and ranked by the ratio of the two times: every
+    candidate once, the `keep` best again in `passes - 1` more passes, by
+    the median of their ratios.  A trial runs up to `steps` steps, fewer on
+    networks whose steps are slow (about `trial_s` seconds per timed run).
+    The trial runs advance the engine's state: reset it afterwards.
+    steps_per_graph stays the engine's own unless a candidate sets it.
+    Returns {"schedule", "s_per_step", "vs_start", "signature", "tune_s",
+    "trials" (absent when the cache answered)}."""
+    t_start = time.perf_counter()
+    sig = signature(eng, key_extra)
+    db = _load(cache)
     if sig in db:
         eng.set_schedule(**db[sig]["schedule"])
-        return db[sig]
+        return dict(db[sig], signature=sig, tune_s=time.perf_counter() - t_start)
     if cands is None:
-        cands = candidates(getattr(eng, "replicas", 1))
-    trials = []
-    best = None
+        cands = candidates(eng.stats())
+    base = dict(eng.schedule)
+    probe = time_schedule(eng, base, 10)
+    n = int(min(steps, max(20, trial_s / max(probe, 1e-9))))
+
+    def trial(sched):
+        t0 = time_schedule(eng, base, n)
+        t1 = time_schedule(eng, sched, n)
+        return t1 / t0, t1
+
+    runs = []   # [applied schedule, [ratio, ...], [s_per_step, ...]]
     for sc in cands:
         try:
-            dt = time_schedule(eng, sc, steps, warmup)
+            r, dt = trial(dict(base, **sc))
         except Exception as e:  # an invalid candidate is skipped, not fatal
             if verbose:
                 print("schedule", sc, "failed:", e)
+            eng.set_schedule(**base)
             continue
-        trials.append({"schedule": sc, "s_per_step": dt})
+        runs.append([dict(eng.schedule), [r], [dt]])
         if verbose:
-            print("%-90s %.2f us/step" % (sc, dt * 1e6))
-        if best is None or dt < best["s_per_step"]:
-            best = {"schedule": sc, "s_per_step": dt}
-    if best is None:
+            print("%-100s %.2f us/step (%.3f x start)" % (eng.schedule, dt * 1e6, r), flush=True)
+    if not runs:
         raise RuntimeError("no schedule candidate ran")
-    eng.set_schedule(**best["schedule"])
-    res = {"schedule": best["schedule"], "s_per_step": best["s_per_step"], "trials": trials}
+    top = sorted(runs, key=lambda q: q[1][0])[:max(1, keep)]
+    for p in range(1, passes if len(runs) > 1 else 1):
+        for q in (top[::-1] if p % 2 else top):
+            r, dt = trial(q[0])
+            q[1].append(r)
+            q[2].append(dt)
+    rank = lambda q: float(statistics.median(q[1]))
+    best = min(top, key=rank)
+    eng.set_schedule(**best[0])
+    trials = [{"schedule": q[0], "vs_start": rank(q), "s_per_step": float(statistics.median(q[2])),
+               "runs": len(q[1])} for q in runs]
+    out = {"schedule": best[0], "vs_start": rank(best), "s_per_step": float(statistics.median(best[2])),
+           "trial_steps": n}
     if cache:
-        db[sig] = {"schedule": best["schedule"], "s_per_step": best["s_per_step"]}
+        db[sig] = out
         os.makedirs(os.path.dirname(cache), exist_ok=True)
         with open(cache, "w") as f:
             json.dump(db, f, indent=1)
-    return res
+    return dict(out, signature=sig, trials=trials, tune_s=time.perf_counter() - t_start)

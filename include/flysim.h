@@ -201,7 +201,11 @@ int fs_load_ext_trains(fs_handle *h, const uint32_t *bits, int64_t first_step, i
  * (code/NetSimGnl2.cpp:231-264: Activation + Spike; DoEvt and FileOut stay
  * with the caller).  Spikes (SPIKE_F) of step s are returned in
  * ids[offsets[s-s0] .. offsets[s-s0+1]) in ascending neuron order when
- * offsets/ids are given (offsets has n_steps+1 entries). */
+ * offsets/ids are given (offsets has n_steps+1 entries).  The steps run in
+ * batches of the schedule's steps_per_graph; FS_ERR_CAPACITY (ids_cap
+ * exceeded, device record overflow) comes after the batch that overflowed
+ * has run and its spikes are lost: fs_step_index reports how far the state
+ * went.  ids_cap >= n_neuron * n_steps never overflows. */
 int fs_run(fs_handle *h, int64_t n_steps, int64_t *offsets, uint32_t *ids, int64_t ids_cap);
 
 /* Number of the next step to run. */
@@ -257,6 +261,18 @@ int fs_kernel_times(fs_handle *h, double *update_ms, double *push_ms, double *ga
  * receptor-synapses pushed, out[2] NMDA receptor-synapses of the spiking
  * neurons (out[1]+out[2] = synaptic events), out[3] NMDA terms gathered. */
 int fs_run_counters(fs_handle *h, int64_t out[4]);
+
+/* Shape of the network on the handle, for schedule caches (no counterpart in
+ * the reference): out[0] neurons of one copy, [1] populations of one copy,
+ * [2] copies (replicas), [3] linear receptor-synapses, [4] NMDA
+ * receptor-synapses (of one copy), [5] smallest synaptic delay, [6] largest
+ * linear delay, [7] largest NMDA delay (steps), [8] precision, [9] input
+ * mode. */
+int fs_network_stats(fs_handle *h, int64_t out[10]);
+
+/* Name and architecture of the current HIP device, and its compute units
+ * (no counterpart in the reference; schedule caches key on it). */
+int fs_device_info(char *name, int32_t name_cap, int32_t *compute_units);
 
 int fs_destroy(fs_handle *h);
 const char *fs_last_error(void);

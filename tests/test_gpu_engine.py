@@ -308,3 +308,39 @@ def test_cli_module_network_fp64_replay_matches_reference_files(golden, tmp_path
     for fn in ["Spikes.dat", "FRates.dat", "FRsum.dat", "SpAlign.dat"]:
         got = (tmp_path / fn).read_text()
         assert got == files["module_" + fn], fn
+
+
+def test_capacity_error_after_a_batch_keeps_the_step_index(c1):
+    """An id buffer too small for the spikes of a batch: fs_run returns
+    FS_ERR_CAPACITY after that batch has run, and the step index counts it
+    (the state went that far), so a caller cannot mistake it for an untouched
+    state; the next run continues from there exactly as an uninterrupted run
+    does."""
+    conf, bits = c1
+
+    def make():
+        e = Engine(conf, precision=A.FS_FP64, ext_mode=A.FS_EXT_REPLAY)
+        e.set_schedule(steps_per_graph=500)
+        e.reset(conf.seed(0))
+        e.load_ext_trains(bits, 0, STEPS_FULL)
+        return e
+
+    ref = make()
+    want = ref.raster(1100)
+    ref.close()
+    cum = np.cumsum(np.bincount(want[:, 0], minlength=1100))
+    cap = 4
+    fail_at = next(e for e in (500, 1000) if cum[e - 1] > cap)  # end of the batch that overflows
+    print("case c1 FP64 replay, %d-entry id buffer: batch ending at step %d overflows" % (cap, fail_at),
+          flush=True)
+    eng = make()
+    off = np.zeros(1001, np.int64)
+    ids = np.zeros(cap, np.uint32)
+    rc = A.lib().fs_run(eng._h, 1000, off.ctypes.data, ids.ctypes.data, ids.size)
+    assert rc == -3  # FS_ERR_CAPACITY
+    assert b"too small" in A.lib().fs_last_error()
+    assert eng.step == fail_at
+    r = eng.raster(100)
+    assert eng.step == fail_at + 100
+    np.testing.assert_array_equal(r, want[(want[:, 0] >= fail_at) & (want[:, 0] < fail_at + 100)])
+    eng.close()

@@ -24,9 +24,9 @@ def test_tune_picks_applies_and_caches(tmp_path):
     cands = [dict(update_block=b, gather_lanes=L, push_block=256, steps_per_graph=50, use_graph=1,
                   fused=f) for b, L, f in ((256, 8, 1), (128, 4, 1), (512, 2, 0))]
     cache = str(tmp_path / "tune.json")
-    res = tune.tune(eng, steps=100, warmup=20, cands=cands, cache=cache)
-    assert len(res["trials"]) == 3
-    best = min(res["trials"], key=lambda t: t["s_per_step"])["schedule"]
+    res = tune.tune(eng, steps=100, cands=cands, cache=cache)
+    assert len(res["trials"]) == 3 and all(t["runs"] == 3 for t in res["trials"])
+    best = min(res["trials"], key=lambda t: t["vs_start"])["schedule"]
     assert res["schedule"] == best
     got = eng.schedule
     for k in ("update_block", "gather_lanes", "steps_per_graph"):
@@ -35,7 +35,7 @@ def test_tune_picks_applies_and_caches(tmp_path):
     assert len(db) == 1
     # a second engine of the same shape takes the cached schedule without timing
     e2 = Engine(arr, precision=A.FS_FP64)
-    r2 = tune.tune(e2, steps=100, warmup=20, cands=cands, cache=cache)
+    r2 = tune.tune(e2, steps=100, cands=cands, cache=cache)
     assert "trials" not in r2 and r2["schedule"] == best
     # results do not depend on the schedule
     e2.reset(9)
