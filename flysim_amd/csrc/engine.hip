@@ -93,9 +93,10 @@ struct Engine final : EngineBase {
       b_ens, b_stp, b_irep, b_rp_g, b_col_g, b_w_g, b_igap, b_lp, b_cntlp, b_lpre, b_lpost, b_lrec,
       b_lw, b_lw0, b_gl, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
       b_total, b_rec_k, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase, b_suboff, b_subcap,
-      b_pbr, b_gain, b_skey, b_tmod, b_zig, b_wblk;
+      b_pbr, b_gain, b_skey, b_tmod, b_zig, b_wblk, b_wgat;
   std::vector<uint32_t> hnpop;   // population of each local neuron
   std::vector<uint64_t> hwblk;   // windowed path: first | count << 32 of each block
+  std::vector<uint64_t> hrp_n;   // gather CSR row starts of one copy (block balancing)
   // replay mode's Gaussian injection: population ranges and the position in
   // the reference's ziggurat stream (zig_calls calls before step zig_t0,
   // zig_a calls per step from there)
@@ -115,8 +116,8 @@ struct Engine final : EngineBase {
     const int npb = std::max(1, sched.update_block / sched.gather_lanes);
     const int nlu = __builtin_popcount((uint32_t)net.lin_mask);
     int32_t w = std::min(sched.window, win_max);
-    while (w > 1 && fsk::window_lds_bytes<T>(npb, w, nlu) > (size_t)65536) --w;
-    if (w >= 1 && fsk::window_lds_bytes<T>(npb, w, nlu) > (size_t)65536) return 0;
+    while (w > 1 && fsk::window_lds_bytes<T>(npb, w, nlu, B == 1) > (size_t)65536) --w;
+    if (w >= 1 && fsk::window_lds_bytes<T>(npb, w, nlu, B == 1) > (size_t)65536) return 0;
     return w;
   }
   bool fused_active() const override { return fused(); }
@@ -446,6 +447,7 @@ struct Engine final : EngineBase {
     }
     b_rp_n.alloc((n0 + 1) * 8);
     HIPCHECK(hipMemcpy(b_rp_n.p, cnt_n.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
+    hrp_n.assign(cnt_n.begin(), cnt_n.begin() + (n0 + 1));
     b_meta_n.alloc(syn_nmda * 4);
     b_w_n.alloc(syn_nmda * 4);
     if (syn_nmda) {
@@ -585,6 +587,7 @@ struct Engine final : EngineBase {
       for (size_t i = 0; i < n0; ++i) h[i + 1] += h[i];
       HIPCHECK(hipMemcpy(rp.p, h.data(), (n0 + 1) * 8, hipMemcpyHostToDevice));
       (w == 0 ? syn_lin : syn_nmda) = (int64_t)h[n0];
+      if (w == 1) hrp_n.swap(h);
     }
     b_meta_p.alloc((size_t)syn_lin * 4);
     b_w_p.alloc((size_t)syn_lin * 4);
@@ -876,20 +879,37 @@ struct Engine final : EngineBase {
   // share one parameter block and update in one pass; where that would take
   // more than 1.25 times the blocks of plain npb-neuron pieces (many small
   // populations), plain pieces (the update then runs once per population
-  // present in a wave).
+  // present in a wave).  With one network copy a block's NMDA gather time
+  // follows its entry count, so a piece also ends before it would hold more
+  // than the whole chunks of gather_block that the mean plain piece needs
+  // (a single row may).  wgat: each block's range of gather entries.
   void build_window_blocks(int32_t npb) {
     std::vector<uint64_t> runs;
-    int64_t g = 0;
+    const int64_t plain = ((int64_t)N + npb - 1) / npb;
+    const bool bal = B == 1 && hrp_n.size() == (size_t)N0 + 1 && hrp_n[N0] > 0;
+    uint64_t cap = 0;
+    if (bal) {
+      const double mean = (double)hrp_n[N0] / (double)plain;
+      const double ch = (double)fsk::window_chunk<T>(std::max(1, window()));
+      cap = (uint64_t)(ch * std::max(1.0, std::ceil(mean / ch)));
+    }
+    int64_t g = 0, pieces = 0;
     while (g < N) {
       const uint32_t p = hnpop[g / B];
       int64_t e = g + 1;
       while (e < N && hnpop[e / B] == p) ++e;
-      for (int64_t a = g; a < e; a += npb)
-        runs.push_back((uint64_t)a | ((uint64_t)std::min<int64_t>(npb, e - a) << 32));
+      pieces += (e - g + npb - 1) / npb;
+      for (int64_t a = g; a < e;) {
+        int64_t c = std::min<int64_t>(npb, e - a);
+        if (bal)
+          for (int64_t k = 1; k < c; ++k)
+            if (hrp_n[a + k + 1] - hrp_n[a] > cap) { c = k; break; }
+        runs.push_back((uint64_t)a | ((uint64_t)c << 32));
+        a += c;
+      }
       g = e;
     }
-    const int64_t plain = ((int64_t)N + npb - 1) / npb;
-    if ((int64_t)runs.size() * 4 > plain * 5) {
+    if (pieces * 4 > plain * 5) {
       runs.clear();
       for (int64_t a = 0; a < N; a += npb)
         runs.push_back((uint64_t)a | ((uint64_t)std::min<int64_t>(npb, N - a) << 32));
@@ -898,6 +918,18 @@ struct Engine final : EngineBase {
     b_wblk.alloc(hwblk.size() * 8);
     HIPCHECK(hipMemcpy(b_wblk.p, hwblk.data(), hwblk.size() * 8, hipMemcpyHostToDevice));
     net.wblk = b_wblk.as<uint64_t>();
+    net.wgat = nullptr;
+    if (B == 1 && hrp_n.size() == (size_t)N0 + 1) {
+      std::vector<uint64_t> ge(2 * hwblk.size());
+      for (size_t b = 0; b < hwblk.size(); ++b) {
+        const uint64_t a = hwblk[b] & 0xFFFFFFFFu, c = hwblk[b] >> 32;
+        ge[2 * b] = hrp_n[a];
+        ge[2 * b + 1] = hrp_n[a + c];
+      }
+      b_wgat.alloc(ge.size() * 8);
+      HIPCHECK(hipMemcpy(b_wgat.p, ge.data(), ge.size() * 8, hipMemcpyHostToDevice));
+      net.wgat = b_wgat.as<uint64_t>();
+    }
   }
   // fused step: a group of gather_lanes lanes per neuron
   int step_grid() const {
@@ -981,16 +1013,24 @@ struct Engine final : EngineBase {
     if (timed) HIPCHECK(hipEventRecord(ev[0], stream));
     const dim3 g((unsigned)hwblk.size()), b(sched.update_block);
     const size_t lds = fsk::window_lds_bytes<T>(sched.update_block / sched.gather_lanes, w,
-                                                __builtin_popcount((uint32_t)net.lin_mask));
+                                                __builtin_popcount((uint32_t)net.lin_mask), B == 1);
     // replicas in whole waves with one lane per row: the row tables are wave-uniform
     const bool uni = (B % 64) == 0 && sched.gather_lanes == 1;
+    // the AMPA + GABA (+ NMDA) layout with external AMPA drive and no STP
+    const bool prof = net.lin_mask == 3 && net.ext_mask == 1 && net.stp_mode == 0;
+#define FS_WIN(R, U)                                                                      \
+  do {                                                                                    \
+    if (prof) hipLaunchKernelGGL((fsk::k_window<T, R, U, 1>), g, b, lds, stream, net, k0, w); \
+    else hipLaunchKernelGGL((fsk::k_window<T, R, U, 0>), g, b, lds, stream, net, k0, w);      \
+  } while (0)
     if (net.ext_mode == FS_EXT_REPLAY) {
-      if (uni) hipLaunchKernelGGL((fsk::k_window<T, true, true>), g, b, lds, stream, net, k0, w);
-      else hipLaunchKernelGGL((fsk::k_window<T, true, false>), g, b, lds, stream, net, k0, w);
+      if (uni) FS_WIN(true, true);
+      else FS_WIN(true, false);
     } else {
-      if (uni) hipLaunchKernelGGL((fsk::k_window<T, false, true>), g, b, lds, stream, net, k0, w);
-      else hipLaunchKernelGGL((fsk::k_window<T, false, false>), g, b, lds, stream, net, k0, w);
+      if (uni) FS_WIN(false, true);
+      else FS_WIN(false, false);
     }
+#undef FS_WIN
     HIPCHECK(hipGetLastError());
     if (timed) {
       HIPCHECK(hipEventRecord(ev[1], stream));

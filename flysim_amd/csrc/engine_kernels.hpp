@@ -167,6 +167,8 @@ struct Net {
   int32_t rec_on;           // 1: the batch's spikes go to the record (readback asked for)
   const uint64_t *wblk;     // windowed path: block b covers neurons [first, first + count),
                             // first | count << 32, at most blockDim / gather_lanes of them
+  const uint64_t *wgat;     // windowed path, one copy: [2 * b], [2 * b + 1] = block b's range
+                            // of gather CSR entries
   int32_t push_waves;       // waves of k_deliver dedicated to the push
   int32_t push_blocks;      // fused step: leading blocks that push; the rest gather + update
 };
@@ -571,10 +573,16 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PT &P, const NCon
   } else {
     const T i0 = g[0] * (P.vrev[0] - v);
     const T i1 = g[1] * (P.vrev[1] - v);
-    T ex;
-    if constexpr (sizeof(T) == 4) ex = expf(T(-0.062f) * v);
-    else ex = exp(T(-0.062f) * v);
-    const T i2 = g[2] * (P.vrev[2] - v) / (T(1) + P.mg * ex / T(3.57f));
+    T i2;
+    if constexpr (sizeof(T) == 4) {
+      // FP32 mode: hardware exp2 and reciprocal (about 1 ulp each) in place
+      // of the correctly rounded expf and divisions
+      const float ex = __expf(-0.062f * v);
+      i2 = g[2] * (P.vrev[2] - v) * __builtin_amdgcn_rcpf(1.0f + P.mg * ex * (1.0f / 3.57f));
+    } else {
+      const T ex = exp(T(-0.062f) * v);
+      i2 = g[2] * (P.vrev[2] - v) / (T(1) + P.mg * ex / T(3.57f));
+    }
     const T i3 = g[3] * (P.vrev[3] - v);
     const T i4 = g[4] * (P.vrev[4] - v);
     if (net.irep) {  // Nets[].Irep (code/EqsGnl2.cpp:196-200)
@@ -586,7 +594,8 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PT &P, const NCon
       ir[4 * (size_t)N] = i4;
     }
     T I = i0 + i1 + i2 + i3 + i4 + isyn * T(1e3f) + P.leak_drive;
-    I = T(1e-3f) * I / P.cm;
+    if constexpr (sizeof(T) == 4) I = (1e-3f * I) * __builtin_amdgcn_rcpf(P.cm);
+    else I = T(1e-3f) * I / P.cm;
     cs &= ~(F_OVF | F_UDF);
     if (I > P.dvmax) { I = P.dvmax; cs |= F_OVF; }
     else if (I < -P.dvmax) { I = -P.dvmax; cs |= F_UDF; }
@@ -1122,12 +1131,130 @@ __device__ __forceinline__ void gather_window(const Net<T> &net, int64_t p, int3
   }
 }
 
-// Bytes of dynamic LDS k_window takes for npb neurons per block, W steps and
-// nlu linear receptors in use (the carve-up at the top of k_window).
+// Copy of 4 bytes per lane from global memory straight into LDS (the LDS
+// DMA of the vector memory unit): lane i of the wave writes lds_wave[i], so
+// lds_wave must be the same for every lane of the wave.  Completion is
+// counted by vmcnt, like a load.
+__device__ __forceinline__ void lds_dma4(const void *src, void *lds_wave) {
+  __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)src,
+                                   (void __attribute__((address_space(3))) *)lds_wave, 4, 0, 0);
+}
+
+// Waits until this wave's memory operations, LDS DMA included, are done,
+// then for the whole block.
+__device__ __forceinline__ void dma_barrier() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+}
+
+// NMDA sums of the block's rows nb .. nb+nbc-1 (one network copy) for the W
+// steps of a window, into s_gn[s * npb + r].  The block's entries are
+// consecutive in the gather CSR; they are taken in chunks of CH: (1) their
+// columns and weights are copied into LDS; (2) every thread copies (entry,
+// step) gates of the chunk, entry-major, into LDS -- the W gates of one
+// entry are W consecutive history slots of its presynaptic neuron, so
+// consecutive lanes read consecutive words -- as LDS DMA, which takes no
+// registers, so each wave keeps all its copies of the chunk in flight; (3)
+// thread (row, step) adds its row's entries of the chunk in entry order with
+// fused multiply-adds, carrying the sum across chunks in s_gn.  The sum is
+// then scaled by the NMDA gain: the expression of gather_row with one lane
+// per row, so every path gives the same bits, and a block's time follows its
+// entry count rather than its longest row.  FP64 gates travel as their low
+// and high words (s_x: [2][CH * W] words).  hb: history slot of step k0 - 1.
+// All threads must call it.
 template <typename T>
-__host__ __device__ constexpr size_t window_lds_bytes(int npb, int W, int nlu) {
-  return (size_t)8 * npb + sizeof(T) * ((size_t)W * npb * (2 + nlu) + (size_t)((npb + 63) / 64) * 64 * W) +
-         (size_t)4 * ((size_t)npb + 2 * (size_t)W * npb + (size_t)((npb + 63) / 64) * 64 * W);
+__device__ __forceinline__ void gather_block(const Net<T> &net, int32_t nb, int nbc, uint64_t e0, uint64_t e1,
+                                             int32_t hb, int32_t W, T *s_gn, int npb, uint64_t *s_rpn,
+                                             uint32_t *s_m, float *s_w, uint32_t *s_x, int CH) {
+  const uint32_t tid = threadIdx.x, bd = blockDim.x, wb = tid & ~63u;
+  const uint32_t DN = (uint32_t)net.DN, dmask = DN - 1u;
+  // q / W as umulhi(q, mw), exact for q < 2^26 (W == 1: q itself)
+  const uint32_t mw = 0xFFFFFFFFu / (uint32_t)W + 1u;
+  auto divw = [&](uint32_t q) -> uint32_t { return W == 1 ? q : __umulhi(q, mw); };
+  const uint32_t npair = (uint32_t)nbc * (uint32_t)W;
+  for (uint32_t q = tid; q < npair; q += bd) {
+    const uint32_t r = divw(q), s = q - r * (uint32_t)W;
+    s_gn[s * npb + r] = T(0);
+  }
+  const uint32_t *sh = (const uint32_t *)net.shist;
+  constexpr uint32_t TW = sizeof(T) / 4;  // words per gate
+  for (uint64_t c0 = e0; c0 < e1; c0 += (uint64_t)CH) {
+    const uint32_t nc = (uint32_t)(e1 - c0 < (uint64_t)CH ? e1 - c0 : (uint64_t)CH);
+    const uint32_t nf = nc * (uint32_t)W;
+    for (uint32_t e = tid, ew = wb; ew < nc; e += bd, ew += bd) {
+      if (e < nc) {
+        lds_dma4(net.meta_n + c0 + e, s_m + ew);
+        lds_dma4(net.w_n + c0 + e, s_w + ew);
+      }
+    }
+    if (c0 == e0)  // the rows' starts, fetched beside the first chunk
+      for (uint32_t r = tid; r <= (uint32_t)nbc; r += bd) s_rpn[r] = ldc(net.rp_n + nb + r);
+    dma_barrier();
+    for (uint32_t f = tid, fw = wb; fw < nf; f += bd, fw += bd) {
+      if (f < nf) {
+        const uint32_t e = divw(f), s = f - e * (uint32_t)W;
+        const uint32_t m = s_m[e];
+        const uint32_t *g = sh + ((size_t)(m & 0x00FFFFFFu) * DN + (((uint32_t)hb + s - (m >> 24)) & dmask)) * TW;
+        lds_dma4(g, s_x + fw);
+        if constexpr (TW == 2) lds_dma4(g + 1, s_x + (size_t)CH * W + fw);
+      }
+    }
+    dma_barrier();
+    auto gate = [&](uint32_t f) -> T {
+      if constexpr (TW == 2) return __hiloint2double((int)s_x[(size_t)CH * W + f], (int)s_x[f]);
+      else return __uint_as_float(s_x[f]);
+    };
+    for (uint32_t q = tid; q < npair; q += bd) {
+      const uint32_t r = divw(q), s = q - r * (uint32_t)W;
+      const uint64_t a = s_rpn[r], b = s_rpn[r + 1];
+      const uint64_t lo = a > c0 ? a : c0, hi = b < c0 + nc ? b : c0 + nc;  // the row's part of the chunk
+      if (lo < hi) {
+        T acc = s_gn[s * npb + r];
+        uint32_t e = (uint32_t)(lo - c0);
+        const uint32_t eh = (uint32_t)(hi - c0);
+        // four entries' LDS reads ahead of their (ordered) multiply-adds
+        for (; e + 4 <= eh; e += 4) {
+          const float w0 = s_w[e], w1 = s_w[e + 1], w2 = s_w[e + 2], w3 = s_w[e + 3];
+          const T x0 = gate(e * (uint32_t)W + s), x1 = gate((e + 1) * (uint32_t)W + s),
+                  x2 = gate((e + 2) * (uint32_t)W + s), x3 = gate((e + 3) * (uint32_t)W + s);
+          acc = fma(T(w0), x0, acc);
+          acc = fma(T(w1), x1, acc);
+          acc = fma(T(w2), x2, acc);
+          acc = fma(T(w3), x3, acc);
+        }
+        for (; e < eh; ++e) acc = fma(T(s_w[e]), gate(e * (uint32_t)W + s), acc);
+        s_gn[s * npb + r] = acc;
+      }
+    }
+    lds_barrier();
+  }
+  const T gain = T(net.gainr[2]);
+  for (uint32_t q = tid; q < npair; q += bd) {
+    const uint32_t r = divw(q), s = q - r * (uint32_t)W;
+    s_gn[s * npb + r] *= gain;
+  }
+}
+
+// Entries per chunk of gather_block: W gates of each staged in LDS, 32 KiB.
+template <typename T>
+__host__ __device__ constexpr int window_chunk(int W) {
+  return (int)(32768 / sizeof(T)) / W < 1024 ? (int)(32768 / sizeof(T)) / W : 1024;
+}
+
+// Bytes of dynamic LDS k_window takes for npb neurons per block, W steps,
+// nlu linear receptors in use and one network copy (flat, gather_block's
+// staging) or several (the carve-up at the top of k_window): the arrays of
+// phases A and B, then a region phase A uses for the gather's staging and
+// phases B and C for the spike staging.
+template <typename T>
+__host__ __device__ constexpr size_t window_lds_bytes(int npb, int W, int nlu, bool flat) {
+  const size_t nst = (size_t)((npb + 63) / 64) * 64 * W;
+  const size_t fixed = ((size_t)8 * npb + sizeof(T) * ((size_t)W * npb * (1 + nlu)) +
+                        (size_t)4 * ((size_t)npb + 2 * (size_t)W * npb) + 7) & ~(size_t)7;
+  const size_t bc = sizeof(T) * ((size_t)W * npb + nst) + (size_t)4 * nst;
+  const int ch = window_chunk<T>(W);
+  const size_t stage = flat ? (size_t)8 * (npb + 1) + sizeof(T) * (size_t)ch * W + (size_t)8 * ch : 0;
+  return fixed + (bc > stage ? bc : stage);
 }
 
 // Temporal blocking over the smallest synaptic delay: one launch runs the W
@@ -1156,8 +1283,18 @@ __host__ __device__ constexpr size_t window_lds_bytes(int npb, int W, int nlu) {
 // delay + 1) are deep enough that no write of the launch lands in a slot some
 // block still has to read.  UNI: replicas in whole waves with L == 1 (the
 // row's tables are scalar loads).
-template <typename T, bool REPLAY, bool UNI>
-__global__ void __launch_bounds__(512) k_window(Net<T> net, int32_t k0, int32_t W) {
+// PROF 1: the network's receptor layout is AMPA and GABA linear (lin_mask 3),
+// external drive on AMPA only (ext_mask 1) and no short-term plasticity; the
+// instance folds those fields to constants, dropping the other receptors'
+// code from the update loop.  PROF 0: any layout.
+template <typename T, bool REPLAY, bool UNI, int PROF>
+__global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t W) {
+  Net<T> net = net_;
+  if constexpr (PROF == 1) {
+    net.lin_mask = 3;
+    net.ext_mask = 1;
+    net.stp_mode = 0;
+  }
   extern __shared__ __align__(16) unsigned char wsm[];
   const int L = net.gather_lanes;
   const int npb = (int)blockDim.x / L;  // neurons a block can hold
@@ -1169,16 +1306,26 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net, int32_t k0, int32_t 
   const int wpb = (int)(blockDim.x >> 6);
   const int nlu = __builtin_popcount((uint32_t)net.lin_mask);
   const int nst = ((npb + 63) / 64) * 64 * W;  // spike staging entries (64 W per update wave)
+  const bool flat = !UNI && B == 1;           // gather_block (else gather_window)
   // LDS carve-up (window_lds_bytes)
   uint64_t *s_prow = (uint64_t *)wsm;                         // [npb] push row start
   T *s_gn = (T *)(s_prow + npb);                              // [W][npb] NMDA sums
   T *s_inc = s_gn + (size_t)W * npb;                          // [nlu][W][npb] ring slots
-  T *s_hs = s_inc + (size_t)nlu * W * npb;                    // [W][npb] NMDA gates of each step
-  T *s_amp = s_hs + (size_t)W * npb;                          // [nst] staged push amplitudes
-  uint32_t *s_pdeg = (uint32_t *)(s_amp + nst);               // [npb] push row length
+  uint32_t *s_pdeg = (uint32_t *)(s_inc + (size_t)nlu * W * npb);  // [npb] push row length
   uint32_t *s_draw = s_pdeg + npb;                            // [W][npb] event outcomes
   float *s_inj = (float *)(s_draw + (size_t)W * npb);         // [W][npb] injection draws
-  uint32_t *s_spk = (uint32_t *)(s_inj + (size_t)W * npb);    // [nst] step << 24 | neuron
+  unsigned char *s_u = (unsigned char *)(s_inj + (size_t)W * npb);
+  s_u += (uintptr_t)(-(intptr_t)s_u) & 7u;                    // 8-byte aligned shared region
+  // phases B and C
+  T *s_hs = (T *)s_u;                                         // [W][npb] NMDA gates of each step
+  T *s_amp = s_hs + (size_t)W * npb;                          // [nst] staged push amplitudes
+  uint32_t *s_spk = (uint32_t *)(s_amp + nst);                // [nst] step << 24 | neuron
+  // phase A, gather_block's staging
+  const int CH = window_chunk<T>(W);
+  uint64_t *s_rpn = (uint64_t *)s_u;                          // [npb + 1] gather row starts
+  uint32_t *s_x = (uint32_t *)(s_rpn + npb + 1);              // [CH][W] staged gates (words)
+  uint32_t *s_m = s_x + (size_t)CH * W * (sizeof(T) / 4);     // [CH] staged columns | delays
+  float *s_w = (float *)(s_m + CH);                           // [CH] staged weights
   __shared__ uint32_t s_wcnt[8], s_base;
   if (tid < 8) s_wcnt[tid] = 0u;
   const int64_t tb = *net.t0;
@@ -1231,9 +1378,14 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net, int32_t k0, int32_t 
     locate<false>(net.npop, B, nr, nlr, repr, pr);
     load_const(net, *(PopC<T> *)(net.pops + pr), nlr, repr, pr, Cr);
   }
-  if (net.has_nmda_syn)
-    gather_window<T, UNI>(net, tid / L < nbc ? (int64_t)nb + tid / L : (int64_t)N, slot_of(tm1, k0 - 1, net.DN),
-                          W, L, tid % L, s_gn, npb, tid / L);
+  if (net.has_nmda_syn) {
+    if (flat)
+      gather_block<T>(net, nb, nbc, ldc(net.wgat + 2 * blockIdx.x), ldc(net.wgat + 2 * blockIdx.x + 1),
+                      slot_of(tm1, k0 - 1, net.DN), W, s_gn, npb, s_rpn, s_m, s_w, s_x, CH);
+    else
+      gather_window<T, UNI>(net, tid / L < nbc ? (int64_t)nb + tid / L : (int64_t)N,
+                            slot_of(tm1, k0 - 1, net.DN), W, L, tid % L, s_gn, npb, tid / L);
+  }
   if (rl) {
     int c = 0;
 #pragma unroll

@@ -18,6 +18,7 @@ from __future__ import annotations
 import hashlib
 import itertools
 import json
+import math
 import os
 import statistics
 import time
@@ -84,10 +85,11 @@ def tune(eng: Engine, steps: int = 200, trial_s: float = 0.03, cands: Optional[I
 
     The trials run one after another on the live state, whose activity can
     drift (a network that ignites runs several times slower per step
-    afterwards), so each candidate is timed right after the engine's
-    starting schedule and ranked by the ratio of the two times: every
-    candidate once, the `keep` best again in `passes - 1` more passes, by
-    the median of their ratios.  A trial runs up to `steps` steps, fewer on
+    afterwards), so each candidate is timed between two runs of the engine's
+    starting schedule and ranked by the ratio to them (a trial during which
+    the starting schedule's time changed by more than 25 % is repeated once
+    at the end): every candidate once, the `keep` best again in
+    `passes - 1` more passes, by the median of their ratios.  A trial runs up to `steps` steps, fewer on
     networks whose steps are slow (about `trial_s` seconds per timed run).
     The trial runs advance the engine's state: reset it afterwards.
     steps_per_graph stays the engine's own unless a candidate sets it.
@@ -104,32 +106,50 @@ def tune(eng: Engine, steps: int = 200, trial_s: float = 0.03, cands: Optional[I
     base = dict(eng.schedule)
     probe = time_schedule(eng, base, 10)
     n = int(min(steps, max(20, trial_s / max(probe, 1e-9))))
+    t_base = [time_schedule(eng, base, n)]   # the starting schedule before each trial
 
     def trial(sched):
-        t0 = time_schedule(eng, base, n)
-        t1 = time_schedule(eng, sched, n)
-        return t1 / t0, t1
+        """Candidate time over the geometric mean of the starting schedule's
+        times just before and just after it; None when those two differ by
+        more than 25 % (the activity changed during the trial)."""
+        try:
+            t1 = time_schedule(eng, sched, n)
+            applied = dict(eng.schedule)
+        finally:
+            t_base.append(time_schedule(eng, base, n))
+        a, b = t_base[-2], t_base[-1]
+        if max(a, b) > 1.25 * min(a, b):
+            return applied, None, t1
+        return applied, t1 / math.sqrt(a * b), t1
 
     runs = []   # [applied schedule, [ratio, ...], [s_per_step, ...]]
+    unsettled = []
     for sc in cands:
         try:
-            r, dt = trial(dict(base, **sc))
+            applied, r, dt = trial(dict(base, **sc))
         except Exception as e:  # an invalid candidate is skipped, not fatal
             if verbose:
                 print("schedule", sc, "failed:", e)
-            eng.set_schedule(**base)
             continue
-        runs.append([dict(eng.schedule), [r], [dt]])
+        if r is None:
+            unsettled.append(applied)
+            continue
+        runs.append([applied, [r], [dt]])
         if verbose:
-            print("%-100s %.2f us/step (%.3f x start)" % (eng.schedule, dt * 1e6, r), flush=True)
+            print("%-100s %.2f us/step (%.3f x start)" % (applied, dt * 1e6, r), flush=True)
+    for sc in unsettled:   # once more, after the activity changed
+        applied, r, dt = trial(sc)
+        if r is not None:
+            runs.append([applied, [r], [dt]])
     if not runs:
         raise RuntimeError("no schedule candidate ran")
     top = sorted(runs, key=lambda q: q[1][0])[:max(1, keep)]
     for p in range(1, passes if len(runs) > 1 else 1):
         for q in (top[::-1] if p % 2 else top):
-            r, dt = trial(q[0])
-            q[1].append(r)
-            q[2].append(dt)
+            _, r, dt = trial(q[0])
+            if r is not None:
+                q[1].append(r)
+                q[2].append(dt)
     rank = lambda q: float(statistics.median(q[1]))
     best = min(top, key=rank)
     eng.set_schedule(**best[0])
