@@ -556,24 +556,23 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PT &P, const NCon
     if (REPLAY) isyn = isyn + T(injv);
     else isyn = isyn + (P.inj_mean + P.inj_std * T(injv));
   }
-  // (3) membrane (VmEul / firing)
-  auto firing = [&]() {
-    if ((int32_t)cnt < P.rfped) {
-      v = cnt == 0 ? T(0) : P.vreset;
-      if ((int32_t)cnt == P.spidy) cs |= F_SPIKE;
-      else cs &= ~F_SPIKE;
-      cnt++;
-    } else {
-      cnt = 0;
-      cs &= ~F_FIRE;
-    }
+  // (3) membrane (VmEul / firing), as selects: the Euler step is evaluated
+  // for every lane and kept where the neuron was not firing; firing() (the
+  // refractory machine) applies where it was, or where the step crossed the
+  // threshold.  The values are those of the branches of the reference.
+  // Receptors with no conductance source in this network contribute -0 (the
+  // reference's 0 * (E - V) is +-0; adding either leaves a sum unchanged).
+  const bool f0 = (cs & F_FIRE) != 0;
+  auto live = [&](int r) -> bool {
+    if (!WIN && net.ltp_mode) return true;
+    if (net.ext_mask & (1 << r)) return true;
+    if (r == 2) return net.has_nmda_syn != 0;
+    return (net.lin_mask & (1 << (r < 2 ? r : r - 1))) != 0;
   };
-  if (cs & F_FIRE) {
-    firing();
-  } else {
-    const T i0 = g[0] * (P.vrev[0] - v);
-    const T i1 = g[1] * (P.vrev[1] - v);
-    T i2;
+  const T i0 = live(0) ? g[0] * (P.vrev[0] - v) : T(-0.0);
+  const T i1 = live(1) ? g[1] * (P.vrev[1] - v) : T(-0.0);
+  T i2 = T(-0.0);
+  if (live(2)) {
     if constexpr (sizeof(T) == 4) {
       // FP32 mode: hardware exp2 and reciprocal (about 1 ulp each) in place
       // of the correctly rounded expf and divisions
@@ -583,29 +582,36 @@ __device__ __forceinline__ void neuron_core(const Net<T> &net, PT &P, const NCon
       const T ex = exp(T(-0.062f) * v);
       i2 = g[2] * (P.vrev[2] - v) / (T(1) + P.mg * ex / T(3.57f));
     }
-    const T i3 = g[3] * (P.vrev[3] - v);
-    const T i4 = g[4] * (P.vrev[4] - v);
-    if (net.irep) {  // Nets[].Irep (code/EqsGnl2.cpp:196-200)
-      T *ir = net.irep + n;
-      ir[0] = i0;
-      ir[(size_t)N] = i1;
-      ir[2 * (size_t)N] = i2;
-      ir[3 * (size_t)N] = i3;
-      ir[4 * (size_t)N] = i4;
-    }
-    T I = i0 + i1 + i2 + i3 + i4 + isyn * T(1e3f) + P.leak_drive;
-    if constexpr (sizeof(T) == 4) I = (1e-3f * I) * __builtin_amdgcn_rcpf(P.cm);
-    else I = T(1e-3f) * I / P.cm;
-    cs &= ~(F_OVF | F_UDF);
-    if (I > P.dvmax) { I = P.dvmax; cs |= F_OVF; }
-    else if (I < -P.dvmax) { I = -P.dvmax; cs |= F_UDF; }
-    v = P.df * v + P.ef * I;
-    if (v > P.vth) {
-      cs |= F_FIRE;
-      firing();
-      if (!WIN && net.ltp_mode) cs |= F_LPFIRE;
-    }
   }
+  const T i3 = live(3) ? g[3] * (P.vrev[3] - v) : T(-0.0);
+  const T i4 = live(4) ? g[4] * (P.vrev[4] - v) : T(-0.0);
+  if (net.irep && !f0) {  // Nets[].Irep (code/EqsGnl2.cpp:196-200)
+    T *ir = net.irep + n;
+    ir[0] = g[0] * (P.vrev[0] - v);
+    ir[(size_t)N] = g[1] * (P.vrev[1] - v);
+    ir[2 * (size_t)N] = live(2) ? i2 : g[2] * (P.vrev[2] - v);
+    ir[3 * (size_t)N] = g[3] * (P.vrev[3] - v);
+    ir[4 * (size_t)N] = g[4] * (P.vrev[4] - v);
+  }
+  T I = i0 + i1 + i2 + i3 + i4 + isyn * T(1e3f) + P.leak_drive;
+  if constexpr (sizeof(T) == 4) I = (1e-3f * I) * __builtin_amdgcn_rcpf(P.cm);
+  else I = T(1e-3f) * I / P.cm;
+  const bool ovf = I > P.dvmax, udf = !ovf && I < -P.dvmax;
+  I = ovf ? P.dvmax : (udf ? -P.dvmax : I);
+  const T vm = P.df * v + P.ef * I;
+  const uint32_t csm = (cs & ~(F_OVF | F_UDF)) | (ovf ? F_OVF : 0u) | (udf ? F_UDF : 0u);
+  const bool cross = !f0 && vm > P.vth;
+  // firing() on the state it starts from: the old one if firing, else the
+  // Euler step's with F_FIRE set
+  const uint32_t csf = f0 ? cs : (csm | F_FIRE);
+  const bool inref = (int32_t)cnt < P.rfped;
+  const T vfire = inref ? (cnt == 0 ? T(0) : P.vreset) : (f0 ? v : vm);
+  const uint32_t csfire = inref ? (((int32_t)cnt == P.spidy) ? (csf | F_SPIKE) : (csf & ~F_SPIKE)) : (csf & ~F_FIRE);
+  const bool fire = f0 || cross;
+  v = fire ? vfire : vm;
+  cs = fire ? csfire : csm;
+  cnt = fire ? (inref ? cnt + 1u : 0u) : cnt;
+  if (!WIN && net.ltp_mode && cross) cs |= F_LPFIRE;
   spk = (cs & F_SPIKE) != 0;
   // (4) presynaptic gates (the reference's per-synapse gate rule, shared);
   // a spike adds 1, or D*F under STP and D under STD, the presynaptic
@@ -1426,23 +1432,39 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
       todo &= ~__ballot(here);
       const DevPop<T> P = net.pops[pc];
       if (here) hn = P.has_nmda_out;
+      // step s's inputs from LDS, read one step ahead of their use
+      struct In {
+        T inc[4];
+        T gn;
+        uint32_t db;
+        float iv;
+      };
+      auto fetch = [&](int32_t s, In &x) {
+        int c = 0;
+#pragma unroll
+        for (int li = 0; li < 4; ++li) {
+          x.inc[li] = T(0);
+          if (!(net.lin_mask & (1 << li))) continue;
+          x.inc[li] = s_inc[((size_t)c * W + s) * npb + ul];
+          ++c;
+        }
+        x.gn = net.has_nmda_syn ? s_gn[s * npb + ul] : T(0);
+        x.db = s_draw[s * npb + ul];
+        x.iv = s_inj[s * npb + ul];
+      };
+      In cur;
+      if (here) fetch(0, cur);
       for (int32_t s = 0; s < W; ++s) {
         const int32_t k = k0 + s;
         bool spk = false;
         T a_push = T(0);
         if (here) {
-          T inc[4] = {T(0), T(0), T(0), T(0)};
-          int c = 0;
-#pragma unroll
-          for (int li = 0; li < 4; ++li) {
-            if (!(net.lin_mask & (1 << li))) continue;
-            inc[li] = s_inc[((size_t)c * W + s) * npb + ul];
-            ++c;
-          }
-          const T gn = net.has_nmda_syn ? s_gn[s * npb + ul] : T(0);
-          neuron_core<T, REPLAY, true, const DevPop<T>>(net, P, C, n, tb + k, 0, gn, inc, s_draw[s * npb + ul],
-                                                        s_inj[s * npb + ul], S, spk, a_push);
+          In nxt = cur;
+          if (s + 1 < W) fetch(s + 1, nxt);
+          neuron_core<T, REPLAY, true, const DevPop<T>>(net, P, C, n, tb + k, 0, cur.gn, cur.inc, cur.db, cur.iv,
+                                                        S, spk, a_push);
           s_hs[s * npb + ul] = S.sn;
+          cur = nxt;
         }
         const uint64_t mask = __ballot(spk);
         if (mask) {
