@@ -8,6 +8,10 @@ Stages (each a command; files are numpy .npz):
   screen  predict M sweep points, flag predicted mean rate > threshold-margin,
           re-simulate every flagged point plus a random audit of the rest
           (optionally a slice of that list per call) -> verified rates
+  band    the threshold-locating screen end to end: simulate only the points
+          predicted near the threshold plus a 1 % audit, classify the rest by
+          the surrogate
+  moderr  per-module error of the surrogate against repeated simulator runs
   report  merge the screen slices: hyperactive points found, surrogate error
           against the simulator, points/s of plain per-instance GPU simulation,
           batched simulation and batched + surrogate
@@ -133,6 +137,117 @@ def screen(model: Dict[str, np.ndarray], points: int, seed: int, replicas: int,
             "t_predict_s": np.float64(t_pred),
             "predict_gpu_ms": np.float64(sur.mlp.last_ms), "t_sim_s": np.float64(t_sim),
             "sim_points_per_s": np.float64(mine.size / t_sim if t_sim > 0 else 0.0)}
+
+
+def band(model: Dict[str, np.ndarray], points: int, seed: int, replicas: int, lo: float = 45.0,
+         hi: float = 60.0, audit_frac: float = 0.01, log=print) -> Dict[str, np.ndarray]:
+    """The threshold-locating screen, run end to end: predict every point,
+    simulate the points whose predicted mean rate lies in the uncertain band
+    (lo, hi] around the threshold plus a random audit of `audit_frac` of the
+    others, and classify every point -- by its simulated rate where it was
+    simulated, by the surrogate elsewhere (hyperactive above the band, quiet
+    below).  The audit measures how often the surrogate's call outside the
+    band is wrong."""
+    space = SweepSpace()
+    X = space.sample(points, np.random.default_rng(seed))
+    sur = load_surrogate(model, X.shape[1], space.n_mod)
+    t0 = time.perf_counter()
+    pred = sur.predict(X)
+    t_pred = time.perf_counter() - t0
+    mean = pred.mean(axis=1)
+    in_band = (mean > lo) & (mean <= hi)
+    bidx = np.nonzero(in_band)[0]
+    rest = np.nonzero(~in_band)[0]
+    rng = np.random.default_rng(seed + 1)
+    aidx = np.sort(rng.choice(rest, size=int(round(audit_frac * rest.size)), replace=False))
+    todo = np.concatenate([bidx, aidx])
+    sim = make_sim(replicas)
+    rates = np.zeros((todo.size, space.n_mod), np.float32)
+    chunk = replicas * 16
+    t1 = time.perf_counter()
+    for a in range(0, todo.size, chunk):
+        b = min(todo.size, a + chunk)
+        rates[a:b] = sim.run(X[todo[a:b]], seed=seed * 7919 + int(todo[a]))
+        log("band: simulated %d/%d points, %.1f points/s" % (b, todo.size, b / (time.perf_counter() - t1)))
+    t_sim = time.perf_counter() - t1
+    call = mean > THRESHOLD_HZ                        # the surrogate's call
+    sim_mean = rates.mean(axis=1)
+    call[todo] = sim_mean > THRESHOLD_HZ              # the simulator's, where it ran
+    return {"points": np.int64(points), "seed": np.int64(seed), "band_hz": np.array([lo, hi]),
+            "band_idx": bidx, "audit_idx": aidx, "rates": rates.astype(np.float16),
+            "pred": pred[todo].astype(np.float16), "pred_mean_all": mean.astype(np.float32),
+            "hyperactive": call, "t_predict_s": np.float64(t_pred), "t_sim_s": np.float64(t_sim),
+            "predict_gpu_ms": np.float64(sur.mlp.last_ms),
+            "sim_points_per_s": np.float64(todo.size / t_sim if t_sim > 0 else 0.0)}
+
+
+def band_report(train_meta: Dict, model: Dict[str, np.ndarray], bd: Dict[str, np.ndarray],
+                plain_pts_s: float) -> Dict:
+    """Wall time of the end-to-end band screen (training data, fit,
+    prediction, simulation of the band and the audit) against simulating
+    every point plainly, and the surrogate's wrong calls outside the band as
+    the audit measures them."""
+    points = int(bd["points"])
+    lo, hi = (float(v) for v in bd["band_hz"])
+    nb, na = bd["band_idx"].size, bd["audit_idx"].size
+    pm = bd["pred_mean_all"].astype(np.float64)
+    aud_sim = bd["rates"][nb:].astype(np.float64).mean(axis=1)
+    aud_pm = pm[bd["audit_idx"]]
+    above, below = aud_pm > hi, aud_pm <= lo
+    false_hyper = int((aud_sim[above] <= THRESHOLD_HZ).sum())
+    missed = int((aud_sim[below] > THRESHOLD_HZ).sum())
+    n_above, n_below = int((pm > hi).sum()), int((pm <= lo).sum())
+    t_gen, t_fit = float(train_meta["wall_s"]), float(model["fit_s"])
+    t_pred, t_sim = float(bd["t_predict_s"]), float(bd["t_sim_s"])
+    t_total = t_gen + t_fit + t_pred + t_sim
+    t_plain = points / plain_pts_s
+    return {"points": points, "threshold_hz": THRESHOLD_HZ, "band_hz": [lo, hi],
+            "band_points": nb, "band_frac": nb / points, "audit_points": na,
+            "hyperactive_points": int(np.asarray(bd["hyperactive"]).sum()),
+            "above_band": n_above, "below_band": n_below,
+            "audit_above": int(above.sum()), "audit_false_hyperactive": false_hyper,
+            "audit_below": int(below.sum()), "audit_missed_hyperactive": missed,
+            "est_wrong_calls_above": false_hyper / max(1, int(above.sum())) * n_above,
+            "est_wrong_calls_below": missed / max(1, int(below.sum())) * n_below,
+            "train_points": int(train_meta["points"]), "train_gen_s": t_gen, "fit_s": t_fit,
+            "predict_s": t_pred, "predict_gpu_ms": float(bd["predict_gpu_ms"]),
+            "verify_s": t_sim, "verify_points_per_s": float(bd["sim_points_per_s"]),
+            "total_s": t_total, "plain_points_per_s": plain_pts_s, "plain_total_s": t_plain,
+            "speedup_vs_plain": t_plain / t_total,
+            "speedup_vs_batched_all": points / float(bd["sim_points_per_s"]) / t_total}
+
+
+def module_error(model: Dict[str, np.ndarray], points: int, seed: int, replicas: int, repeats: int = 8,
+                 log=print) -> Dict:
+    """Per-module error of the surrogate against the simulator on held-out
+    points: each point is simulated `repeats` times with independent input
+    streams, and the surrogate's prediction is compared with the mean of the
+    runs (the quantity it estimates); one run's deviation from that mean is
+    the simulator's own noise at this protocol."""
+    space = SweepSpace()
+    X = space.sample(points, np.random.default_rng(seed))
+    sur = load_surrogate(model, X.shape[1], space.n_mod)
+    pred = sur.predict(X).astype(np.float64)
+    sim = make_sim(replicas)
+    runs = np.zeros((repeats, points, space.n_mod))
+    for k in range(repeats):
+        runs[k] = sim.run(X, seed=seed * 1000 + 17 * k + 1)
+        log("module error: repeat %d/%d" % (k + 1, repeats))
+    ref = runs.mean(axis=0)
+    err = np.abs(pred - ref)
+    noise = np.abs(runs[0] - ref) * np.sqrt(repeats / (repeats - 1.0))
+    sub = ref.mean(axis=1) <= THRESHOLD_HZ
+    out = {"points": points, "repeats": repeats,
+           "module_mae_hz": err.mean(axis=0).tolist(),
+           "module_mae_hz_max": float(err.mean(axis=0).max()),
+           "mae_hz": float(err.mean()),
+           "subthreshold_points": int(sub.sum()),
+           "module_mae_hz_max_subthreshold": float(err[sub].mean(axis=0).max()) if sub.any() else None,
+           "mae_hz_subthreshold": float(err[sub].mean()) if sub.any() else None,
+           "one_run_noise_mae_hz": float(noise.mean()),
+           "one_run_noise_module_max_hz": float(noise.mean(axis=0).max()),
+           "modules_within_2hz": int((err.mean(axis=0) <= 2.0).sum())}
+    return out
 
 
 def plain_rate(points: int, seed: int) -> float:
@@ -267,6 +382,28 @@ def main(argv=None):
     s.add_argument("--part", type=int, default=0)
     s.add_argument("--parts", type=int, default=1)
     s.add_argument("--out", required=True)
+    bd = sub.add_parser("band")
+    bd.add_argument("--model", required=True)
+    bd.add_argument("--points", type=int, default=1000000)
+    bd.add_argument("--seed", type=int, default=2)
+    bd.add_argument("--replicas", type=int, default=128)
+    bd.add_argument("--lo", type=float, default=45.0)
+    bd.add_argument("--hi", type=float, default=60.0)
+    bd.add_argument("--out", required=True)
+    br = sub.add_parser("bandreport")
+    br.add_argument("--train", required=True, help="json written by the gen command")
+    br.add_argument("--model", required=True)
+    br.add_argument("--band", required=True)
+    br.add_argument("--plain", required=True)
+    br.add_argument("--moderr", default="", help="json written by the moderr command")
+    br.add_argument("--out", default="")
+    me = sub.add_parser("moderr")
+    me.add_argument("--model", required=True)
+    me.add_argument("--points", type=int, default=1024)
+    me.add_argument("--seed", type=int, default=5)
+    me.add_argument("--replicas", type=int, default=128)
+    me.add_argument("--repeats", type=int, default=8)
+    me.add_argument("--out", required=True)
     pl = sub.add_parser("plain")
     pl.add_argument("--points", type=int, default=64)
     pl.add_argument("--out", required=True)
@@ -293,6 +430,21 @@ def main(argv=None):
     elif a.cmd == "screen":
         m = dict(np.load(a.model))
         np.savez(a.out, **screen(m, a.points, a.seed, a.replicas, part=a.part, parts=a.parts, log=log))
+    elif a.cmd == "band":
+        m = dict(np.load(a.model))
+        np.savez(a.out, **band(m, a.points, a.seed, a.replicas, a.lo, a.hi, log=log))
+    elif a.cmd == "moderr":
+        m = dict(np.load(a.model))
+        json.dump(module_error(m, a.points, a.seed, a.replicas, a.repeats, log=log), open(a.out, "w"), indent=1)
+    elif a.cmd == "bandreport":
+        rep = band_report(json.load(open(a.train)), dict(np.load(a.model)), dict(np.load(a.band)),
+                          float(json.load(open(a.plain))["points_per_s"]))
+        if a.moderr:
+            rep["module_error"] = {k: v for k, v in json.load(open(a.moderr)).items() if k != "module_mae_hz"}
+        txt = json.dumps(rep, indent=1)
+        print(txt)
+        if a.out:
+            open(a.out, "w").write(txt)
     elif a.cmd == "plain":
         rate = plain_rate(a.points, 99)
         log("plain per-instance simulation: %.2f points/s" % rate)

@@ -38,18 +38,21 @@ def signature(eng: Engine, extra: str = "") -> str:
 
 def candidates(stats: Dict[str, int]) -> List[Dict]:
     """Block sizes x gather lanes per row x the step paths the network
-    admits (windowed and fused need a delay of at least one step on every
-    synapse).  Replicas in whole waves read one CSR row per wave, so one or
-    two lanes per row suit them."""
+    admits: windowed (as many steps per launch as the smallest delay allows,
+    one step per launch without delays), fused (needs a delay of at least one
+    step on every synapse) and two launches per step.  Replicas in whole
+    waves read one CSR row per wave, so one or two lanes per row suit them."""
     delayed = stats["min_delay"] >= 1
     has_nmda = stats["syn_nmda"] > 0
     reps = stats["replicas"]
     out = []
+    lanes = [1, 2] if reps >= 64 else ([2, 4, 8, 16, 32] if has_nmda else [8])
+    if not delayed:  # one step per launch: a few shapes
+        lanes = lanes[:2] if reps >= 64 else [4, 8]
+    for b, L in itertools.product([128, 256, 512] if delayed else [256], lanes):
+        if L * 2 <= b:
+            out.append(dict(update_block=b, gather_lanes=L, fused=0, window=64))
     if delayed:
-        lanes = [1, 2] if reps >= 64 else ([2, 4, 8, 16, 32] if has_nmda else [8])
-        for b, L in itertools.product([128, 256, 512], lanes):
-            if L * 2 <= b:
-                out.append(dict(update_block=b, gather_lanes=L, fused=0, window=64))
         for b, L in itertools.product([256, 512], [1, 2] if reps >= 64 else ([1, 4, 8] if has_nmda else [8])):
             out.append(dict(update_block=b, gather_lanes=L, fused=1, window=0))
     lanes = [1, 2] if reps >= 64 else ([1, 4, 8, 16] if has_nmda else [8])

@@ -67,3 +67,34 @@ def test_band_design_accounting():
     assert b["missed_hyperactive_below_simulated"] == 1
     assert abs(b["audit_miss_rate_below"] - 0.25) < 1e-12
     assert abs(b["est_missed_below_total"] - (1 + 0.25 * 60)) < 1e-9
+
+
+def test_band_report_end_to_end_accounting():
+    """End-to-end band screen: band and audit simulated, wrong calls outside
+    the band estimated from the audit on each side, the time of every stage
+    (training data, fit, prediction, simulation) against plain simulation."""
+    from flysim_amd.sweep import band_report
+    points = 1000
+    pm = np.full(points, 10.0, np.float32)
+    pm[:300] = 90.0                       # above the band (45, 60]
+    pm[300:350] = 50.0                    # in the band
+    bidx = np.arange(300, 350)
+    aidx = np.array([0, 1, 2, 3, 500, 501, 502, 503, 504])   # 4 above, 5 below
+    sim_b = np.full(bidx.size, 52.0)
+    sim_a = np.array([80.0, 80.0, 40.0, 80.0, 10.0, 10.0, 70.0, 10.0, 10.0])
+    rates = np.repeat(np.concatenate([sim_b, sim_a])[:, None], 50, axis=1)
+    hyper = pm > 50.0
+    hyper[bidx] = True
+    bd = {"points": np.int64(points), "band_hz": np.array([45.0, 60.0]), "band_idx": bidx,
+          "audit_idx": aidx, "rates": rates.astype(np.float16), "pred_mean_all": pm,
+          "hyperactive": hyper, "t_predict_s": np.float64(1.0), "t_sim_s": np.float64(5.9),
+          "predict_gpu_ms": np.float64(2.0), "sim_points_per_s": np.float64(59 / 5.9)}
+    rep = band_report({"wall_s": 20.0, "points": 100}, {"fit_s": np.float64(3.0)}, bd, plain_pts_s=2.0)
+    assert rep["band_points"] == 50 and rep["audit_points"] == 9
+    assert rep["audit_above"] == 4 and rep["audit_false_hyperactive"] == 1
+    assert rep["audit_below"] == 5 and rep["audit_missed_hyperactive"] == 1
+    assert abs(rep["est_wrong_calls_above"] - 300 / 4) < 1e-9
+    assert abs(rep["est_wrong_calls_below"] - 650 / 5) < 1e-9
+    assert abs(rep["total_s"] - (20.0 + 3.0 + 1.0 + 5.9)) < 1e-9
+    assert abs(rep["speedup_vs_plain"] - 500.0 / 29.9) < 1e-9
+    assert rep["hyperactive_points"] == 350

@@ -64,6 +64,8 @@ def test_ratio_ranking_survives_an_activity_jump(clocked):
     assert eng.schedule["gather_lanes"] == 32
     assert res["vs_start"] == pytest.approx(0.94 / 0.96)
     assert len(res["trials"]) == 6
+    for t in res["trials"]:   # the trial across the jump was repeated, not kept
+        assert t["vs_start"] == pytest.approx(cost(t["schedule"]) / cost(BASE), rel=0.1)
     # ranked by absolute first-pass times the early, pre-ignition candidates
     # would have won
     assert min(res["trials"], key=lambda t: t["s_per_step"])["schedule"]["gather_lanes"] < 32
@@ -106,7 +108,9 @@ def test_candidates_follow_the_network():
     delayed = tune.candidates(STATS)
     assert any(c["window"] for c in delayed) and any(c["fused"] for c in delayed)
     undelayed = tune.candidates(dict(STATS, min_delay=0))
-    assert not any(c["window"] or c["fused"] for c in undelayed)
+    assert not any(c["fused"] for c in undelayed)
+    assert any(c["window"] for c in undelayed)  # one step per launch
+    assert any(not c["window"] and not c["fused"] for c in undelayed)
     reps = tune.candidates(dict(STATS, replicas=128))
     assert {c["gather_lanes"] for c in reps} <= {1, 2}
     for c in delayed:
