@@ -227,15 +227,16 @@ class RateSurrogate:
         return ((np.asarray(X, np.float32) - self.mu) / self.sd).astype(np.float32)
 
     def fit(self, X: np.ndarray, Y: np.ndarray, epochs: int = 60, batch: int = 512,
-            lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+            lr: float = 1e-3, seed: int = 0, decay: Sequence[float] = (1.0, 0.3, 0.1)) -> np.ndarray:
+        """Adam on the MSE of rates / 100 Hz; the learning rate steps down
+        through the factors of `decay`, equal shares of the epochs each."""
         X = np.asarray(X, np.float32)
         self.mu = X.mean(axis=0)
         self.sd = X.std(axis=0) + 1e-6
         self.mlp.set_data(self._x(X), np.asarray(Y, np.float32) / Y_SCALE)
         losses = []
-        # step decay of the learning rate over three phases
-        for i, f in enumerate((1.0, 0.3, 0.1)):
-            losses.append(self.mlp.train(max(1, epochs // 3), batch, lr=lr * f, seed=seed + i))
+        for i, f in enumerate(decay):
+            losses.append(self.mlp.train(max(1, epochs // len(decay)), batch, lr=lr * f, seed=seed + i))
         return np.concatenate(losses)
 
     def predict(self, X: np.ndarray) -> np.ndarray:
