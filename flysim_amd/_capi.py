@@ -71,7 +71,9 @@ class NetDesc(C.Structure):
 class Schedule(C.Structure):
     _fields_ = [("update_block", C.c_int32), ("gather_lanes", C.c_int32),
                 ("push_block", C.c_int32), ("steps_per_graph", C.c_int32),
-                ("use_graph", C.c_int32), ("fused", C.c_int32), ("window", C.c_int32)]
+                ("use_graph", C.c_int32), ("fused", C.c_int32), ("window", C.c_int32),
+                ("xcd_order", C.c_int32), ("gather_chunk", C.c_int32),
+                ("gather_split", C.c_int32)]
 
 
 class EiBatchDesc(C.Structure):
@@ -81,7 +83,8 @@ class EiBatchDesc(C.Structure):
                 ("exc_frac", C.c_float), ("p_conn", C.c_float), ("g_lo", C.c_float),
                 ("g_hi", C.c_float), ("nmda_ratio", C.c_float), ("exc", PopParams),
                 ("inh", PopParams), ("dt", C.c_float), ("precision", C.c_int32),
-                ("ext_mode", C.c_int32), ("record_spikes", C.c_int32)]
+                ("ext_mode", C.c_int32), ("record_spikes", C.c_int32),
+                ("delay_steps", C.c_int32)]
 
 
 class ReplicaParams(C.Structure):

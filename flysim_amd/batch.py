@@ -7,6 +7,17 @@ with sizes uniform in {1000..20000}, seeds 1000 + k, external AMPA rate
 uniform in 0.5-4 kHz and NMDA / GABA gains uniform in 0.5-2, all in one
 handle whose synapse tables the device generates (fs_create_ei_batch).
 
+The sweep states its 1 ms delay as a synaptic delay of 10 steps (SpikeDly
+0) instead of the reference form's SpikeDly of 1 ms.  The two are the same
+network: SpikeDly only delays the spike flag (code/EqsGnl2.cpp:229-251, the
+flag is set at Count == SpiDy, once per firing while SpiDy < RfPed - 1),
+and the presynaptic gates are driven by that flag alone, so moving the flag
+10 steps earlier shifts every gate trajectory 10 steps earlier, which the
+10-step synaptic delay shifts back: each target receives the identical
+conductance sequence.  Only the reported spike times differ (threshold
+crossing instead of 1 ms later).  With every synapse delayed, one launch
+runs 11 steps (the windowed path).
+
 ``ei_batch_network`` restates that generator in numpy (same hash, same float
 rounding) for small instances: the tests build the same networks through the
 host path (fs_create) and compare rasters.
@@ -32,11 +43,11 @@ def mix64(x: np.ndarray) -> np.ndarray:
         return z ^ (z >> np.uint64(31))
 
 
-def c1_populations(f_ext: float = 2400.0, g_ext: float = W.C1_G_EXT):
+def c1_populations(f_ext: float = 2400.0, g_ext: float = W.C1_G_EXT, spike_delay_ms: float = 1.0):
     """E and I population templates of the 1k E/I network (sizes unused)."""
     def pop(name):
         return W.Population(name=name, n=1, cm=0.5, taum=20.0, vth=-50.0, rest=-70.0, reset=-55.0,
-                            refractory_ms=2.0, spike_delay_ms=1.0, self_connection=False,
+                            refractory_ms=2.0, spike_delay_ms=spike_delay_ms, self_connection=False,
                             receptors=[W.Receptor(W.AMPA, tau=2.0, rev=0.0, freq_ext=f_ext,
                                                   mean_ext_eff=g_ext, mean_ext_con=1.0),
                                        W.Receptor(W.NMDA, tau=100.0, tau_xt=2.0, rev=0.0),
@@ -46,16 +57,20 @@ def c1_populations(f_ext: float = 2400.0, g_ext: float = W.C1_G_EXT):
 
 def c4_spec(n_inst: int = 256, seed: int = 4, n_lo: int = 1000, n_hi: int = 20000,
             f_lo: float = 500.0, f_hi: float = 4000.0, gain_lo: float = 0.5,
-            gain_hi: float = 2.0, g_ext: float = W.C1_G_EXT) -> EiBatchSpec:
+            gain_hi: float = 2.0, g_ext: float = W.C1_G_EXT, synaptic_delay: bool = True) -> EiBatchSpec:
     """The 256-instance sweep (sizes, rates and gains drawn with numpy's
-    PCG64 from `seed`; instance k uses connectivity seed 1000 + k)."""
+    PCG64 from `seed`; instance k uses connectivity seed 1000 + k).  The
+    1 ms delay is a 10-step synaptic delay (the same dynamics as SpikeDly
+    1 ms, see the module docstring), or SpikeDly 1 ms with
+    synaptic_delay=False."""
     rng = np.random.default_rng(seed)
     n = rng.integers(n_lo, n_hi + 1, n_inst)
     ext = rng.uniform(f_lo, f_hi, n_inst)
     gn = rng.uniform(gain_lo, gain_hi, n_inst)
     gg = rng.uniform(gain_lo, gain_hi, n_inst)
-    e, i = c1_populations(g_ext=g_ext)
-    return EiBatchSpec(n, 1000 + np.arange(n_inst), ext, gn, gg, e, i)
+    e, i = c1_populations(g_ext=g_ext, spike_delay_ms=0.0 if synaptic_delay else 1.0)
+    return EiBatchSpec(n, 1000 + np.arange(n_inst), ext, gn, gg, e, i,
+                       delay_steps=10 if synaptic_delay else 0)
 
 
 def ei_batch_network(spec: EiBatchSpec) -> W.Network:
@@ -107,7 +122,7 @@ def ei_batch_network(spec: EiBatchSpec) -> W.Network:
                      post=np.concatenate(post_l).astype(np.uint32)[order],
                      w=np.concatenate(w_l).astype(np.float32)[order],
                      receptor=np.concatenate(r_l)[order],
-                     delay=np.zeros(pre.size, np.uint8), dt=spec.dt)
+                     delay=np.full(pre.size, spec.delay_steps, np.uint8), dt=spec.dt)
 
 
 def expected_pairs(spec: EiBatchSpec) -> float:
@@ -120,7 +135,7 @@ def subset(spec: EiBatchSpec, idx) -> EiBatchSpec:
     idx = np.atleast_1d(idx)
     return EiBatchSpec(spec.n_neuron[idx], spec.seed[idx], spec.ext_hz[idx], spec.gain_nmda[idx],
                        spec.gain_gaba[idx], spec.exc, spec.inh, spec.exc_frac, spec.p_conn,
-                       spec.g_lo, spec.g_hi, spec.nmda_ratio, spec.dt)
+                       spec.g_lo, spec.g_hi, spec.nmda_ratio, spec.dt, spec.delay_steps)
 
 
 def run_batch(spec: EiBatchSpec, steps: int, seed: int = 1, schedule="auto", log=print):

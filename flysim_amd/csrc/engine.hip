@@ -116,8 +116,10 @@ struct Engine final : EngineBase {
     const int npb = std::max(1, sched.update_block / sched.gather_lanes);
     const int nlu = __builtin_popcount((uint32_t)net.lin_mask);
     int32_t w = std::min(sched.window, win_max);
-    while (w > 1 && fsk::window_lds_bytes<T>(npb, w, nlu, B == 1) > (size_t)65536) --w;
-    if (w >= 1 && fsk::window_lds_bytes<T>(npb, w, nlu, B == 1) > (size_t)65536) return 0;
+    if (B == 1) w = std::min(w, fsk::FLAT_WMAX);  // gather_block's partial sums per thread
+    const int gc = sched.gather_chunk;
+    while (w > 1 && fsk::window_lds_bytes<T>(npb, w, nlu, B == 1, gc) > (size_t)65536) --w;
+    if (w >= 1 && fsk::window_lds_bytes<T>(npb, w, nlu, B == 1, gc) > (size_t)65536) return 0;
     return w;
   }
   bool fused_active() const override { return fused(); }
@@ -511,6 +513,7 @@ struct Engine final : EngineBase {
       throw std::invalid_argument("incomplete batch description");
     if (!(d.p_conn >= 0.0f && d.p_conn <= 1.0f)) throw std::invalid_argument("p_conn must be in [0,1]");
     if (!(d.exc_frac >= 0.0f && d.exc_frac <= 1.0f)) throw std::invalid_argument("exc_frac must be in [0,1]");
+    if (d.delay_steps < 0 || d.delay_steps > 63) throw std::invalid_argument("delay_steps must be 0..63");
     int64_t tot = 0;
     for (int32_t k = 0; k < K; ++k) {
       if (d.n_neuron[k] < 1) throw std::invalid_argument("every instance needs neurons");
@@ -572,6 +575,7 @@ struct Engine final : EngineBase {
     ga.g_lo = d.g_lo;
     ga.g_span = d.g_hi - d.g_lo;
     ga.nmda_ratio = d.nmda_ratio;
+    ga.delay = (uint32_t)d.delay_steps;
     ga.rp_p = b_rp_p.as<uint64_t>();
     ga.rp_n = b_rp_n.as<uint64_t>();
     ga.nmda_out = b_nmda_out.as<uint32_t>();
@@ -625,7 +629,11 @@ struct Engine final : EngineBase {
         dpop[2 * k + e].first = (int32_t)gi[k].first;
       }
     }
-    init_dims(0, 0, false, (1 << 0) | (1 << 1), d.ext_mode);
+    // one synaptic delay for every synapse: windows of 1 + delay steps
+    const int32_t dl = d.delay_steps;
+    win_max = std::min(dl + 1, 64);
+    mind = maxdp = maxdn = dl;
+    init_dims(dl, dl, dl >= 1, (1 << 0) | (1 << 1), d.ext_mode);
     finish(npop.data());
   }
 
@@ -827,7 +835,14 @@ struct Engine final : EngineBase {
     if (sched.steps_per_graph < 1 || sched.steps_per_graph > 100000)
       throw std::invalid_argument("steps_per_graph out of range");
     if (window() && ub > 512) throw std::invalid_argument("the windowed path takes update_block <= 512");
+    if (sched.gather_chunk < 0 || (sched.gather_chunk > 0 && sched.gather_chunk < 64))
+      throw std::invalid_argument("gather_chunk must be 0 or >= 64");
+    if (sched.xcd_order < 0 || sched.xcd_order > 1) throw std::invalid_argument("xcd_order must be 0 or 1");
+    if (sched.gather_split < 0 || sched.gather_split > L || (sched.gather_split & (sched.gather_split - 1)))
+      throw std::invalid_argument("gather_split must be 0 or a power of two <= gather_lanes");
     net.gather_lanes = L;
+    net.gather_chunk = sched.gather_chunk;
+    net.gather_split = std::max(1, sched.gather_split);
     const int K = sched.steps_per_graph;
     // record buffer: every neuron could hold SPIKE_F every step (sticky
     // case).  Block b of the update grid appends to sub-list b % NSUB, whose
@@ -890,7 +905,7 @@ struct Engine final : EngineBase {
     uint64_t cap = 0;
     if (bal) {
       const double mean = (double)hrp_n[N0] / (double)plain;
-      const double ch = (double)fsk::window_chunk<T>(std::max(1, window()));
+      const double ch = (double)fsk::window_chunk<T>(std::max(1, window()), sched.gather_chunk);
       cap = (uint64_t)(ch * std::max(1.0, std::ceil(mean / ch)));
     }
     int64_t g = 0, pieces = 0;
@@ -913,6 +928,17 @@ struct Engine final : EngineBase {
       runs.clear();
       for (int64_t a = 0; a < N; a += npb)
         runs.push_back((uint64_t)a | ((uint64_t)std::min<int64_t>(npb, N - a) << 32));
+    }
+    if (sched.xcd_order && B == 1 && runs.size() >= 16) {
+      // block i runs on XCD i % 8: give XCD x the x-th eighth of the pieces
+      // in order, so the blocks one XCD runs at a time are neighbours
+      const size_t nbk = runs.size(), qx = nbk / 8, rx = nbk % 8;
+      std::vector<uint64_t> perm(nbk);
+      for (size_t i = 0; i < nbk; ++i) {
+        const size_t x = i % 8, j = i / 8;
+        perm[i] = runs[x * qx + std::min(x, rx) + j];
+      }
+      runs.swap(perm);
     }
     hwblk.swap(runs);
     b_wblk.alloc(hwblk.size() * 8);
@@ -1013,7 +1039,8 @@ struct Engine final : EngineBase {
     if (timed) HIPCHECK(hipEventRecord(ev[0], stream));
     const dim3 g((unsigned)hwblk.size()), b(sched.update_block);
     const size_t lds = fsk::window_lds_bytes<T>(sched.update_block / sched.gather_lanes, w,
-                                                __builtin_popcount((uint32_t)net.lin_mask), B == 1);
+                                                __builtin_popcount((uint32_t)net.lin_mask), B == 1,
+                                                sched.gather_chunk);
     // replicas in whole waves with one lane per row: the row tables are wave-uniform
     const bool uni = (B % 64) == 0 && sched.gather_lanes == 1;
     // the AMPA + GABA (+ NMDA) layout with external AMPA drive and no STP

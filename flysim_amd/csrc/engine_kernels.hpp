@@ -169,6 +169,8 @@ struct Net {
                             // first | count << 32, at most blockDim / gather_lanes of them
   const uint64_t *wgat;     // windowed path, one copy: [2 * b], [2 * b + 1] = block b's range
                             // of gather CSR entries
+  int32_t gather_chunk;     // windowed path, one copy: cap on the entries per staged chunk (0: none)
+  int32_t gather_split;     // windowed path, one copy: sub-lanes per (row, step) pair of the gather
   int32_t push_waves;       // waves of k_deliver dedicated to the push
   int32_t push_blocks;      // fused step: leading blocks that push; the rest gather + update
 };
@@ -1153,6 +1155,10 @@ __device__ __forceinline__ void dma_barrier() {
   __syncthreads();
 }
 
+// Most steps a window of gather_block may span: each thread keeps one
+// partial sum per step of the window in registers.
+constexpr int FLAT_WMAX = 16;
+
 // NMDA sums of the block's rows nb .. nb+nbc-1 (one network copy) for the W
 // steps of a window, into s_gn[s * npb + r].  The block's entries are
 // consecutive in the gather CSR; they are taken in chunks of CH: (1) their
@@ -1161,29 +1167,41 @@ __device__ __forceinline__ void dma_barrier() {
 // entry are W consecutive history slots of its presynaptic neuron, so
 // consecutive lanes read consecutive words -- as LDS DMA, which takes no
 // registers, so each wave keeps all its copies of the chunk in flight; (3)
-// thread (row, step) adds its row's entries of the chunk in entry order with
-// fused multiply-adds, carrying the sum across chunks in s_gn.  The sum is
-// then scaled by the NMDA gain: the expression of gather_row with one lane
-// per row, so every path gives the same bits, and a block's time follows its
-// entry count rather than its longest row.  FP64 gates travel as their low
-// and high words (s_x: [2][CH * W] words).  hb: history slot of step k0 - 1.
-// All threads must call it.
+// L = gather_split consecutive threads (a power of two, at most blockDim /
+// npb) share each (row, step) pair: sub-lane q adds the row's entries q,
+// q + L, ... (counted from the row's start) of the chunk in entry order with
+// fused multiply-adds, carrying its partial sum across chunks in a register.
+// At the end each partial sum is scaled by the NMDA gain and the L of them
+// are reduced by a shuffle tree: gather_row's expression with L lanes, so
+// the two-launch and fused paths with gather_lanes = L give the same bits,
+// and a block's time follows its entry count rather than its longest row,
+// long rows being split L ways (short rows keep L = 1).  FP64
+// gates travel as their low and high words (s_x: [2][CH * W] words).  hb:
+// history slot of step k0 - 1; W <= FLAT_WMAX.  All threads must call it.
 template <typename T>
 __device__ __forceinline__ void gather_block(const Net<T> &net, int32_t nb, int nbc, uint64_t e0, uint64_t e1,
                                              int32_t hb, int32_t W, T *s_gn, int npb, uint64_t *s_rpn,
                                              uint32_t *s_m, float *s_w, uint32_t *s_x, int CH) {
   const uint32_t tid = threadIdx.x, bd = blockDim.x, wb = tid & ~63u;
+  // L sub-lanes per (row, step) pair: item f = pair * L + sub-lane, thread
+  // tid holds items tid, tid + bd, ... (at most W of them, L <= bd / npb)
+  const uint32_t L = (uint32_t)net.gather_split;
+  const uint32_t q = tid & (L - 1u);
+  const uint32_t n_items = (uint32_t)npb * (uint32_t)W * L;
+  const uint32_t nj = (n_items + bd - 1u) / bd;
   const uint32_t DN = (uint32_t)net.DN, dmask = DN - 1u;
-  // q / W as umulhi(q, mw), exact for q < 2^26 (W == 1: q itself)
+  // x / W as umulhi(x, mw), exact for x < 2^26 (W == 1: x itself)
   const uint32_t mw = 0xFFFFFFFFu / (uint32_t)W + 1u;
-  auto divw = [&](uint32_t q) -> uint32_t { return W == 1 ? q : __umulhi(q, mw); };
-  const uint32_t npair = (uint32_t)nbc * (uint32_t)W;
-  for (uint32_t q = tid; q < npair; q += bd) {
-    const uint32_t r = divw(q), s = q - r * (uint32_t)W;
-    s_gn[s * npb + r] = T(0);
-  }
+  auto divw = [&](uint32_t x) -> uint32_t { return W == 1 ? x : __umulhi(x, mw); };
+  T acc[FLAT_WMAX];
+#pragma unroll
+  for (int i = 0; i < FLAT_WMAX; ++i) acc[i] = T(0);
   const uint32_t *sh = (const uint32_t *)net.shist;
   constexpr uint32_t TW = sizeof(T) / 4;  // words per gate
+  auto gate = [&](uint32_t f) -> T {
+    if constexpr (TW == 2) return __hiloint2double((int)s_x[(size_t)CH * W + f], (int)s_x[f]);
+    else return __uint_as_float(s_x[f]);
+  };
   for (uint64_t c0 = e0; c0 < e1; c0 += (uint64_t)CH) {
     const uint32_t nc = (uint32_t)(e1 - c0 < (uint64_t)CH ? e1 - c0 : (uint64_t)CH);
     const uint32_t nf = nc * (uint32_t)W;
@@ -1206,45 +1224,55 @@ __device__ __forceinline__ void gather_block(const Net<T> &net, int32_t nb, int 
       }
     }
     dma_barrier();
-    auto gate = [&](uint32_t f) -> T {
-      if constexpr (TW == 2) return __hiloint2double((int)s_x[(size_t)CH * W + f], (int)s_x[f]);
-      else return __uint_as_float(s_x[f]);
-    };
-    for (uint32_t q = tid; q < npair; q += bd) {
-      const uint32_t r = divw(q), s = q - r * (uint32_t)W;
+#pragma unroll
+    for (int i = 0; i < FLAT_WMAX; ++i) {
+      if ((uint32_t)i >= nj) break;
+      const uint32_t f = tid + (uint32_t)i * bd;
+      const uint32_t pr = f / L;  // (row, step) pair
+      const uint32_t r = divw(pr), s = pr - r * (uint32_t)W;
+      if (f >= n_items || r >= (uint32_t)nbc) continue;
       const uint64_t a = s_rpn[r], b = s_rpn[r + 1];
       const uint64_t lo = a > c0 ? a : c0, hi = b < c0 + nc ? b : c0 + nc;  // the row's part of the chunk
-      if (lo < hi) {
-        T acc = s_gn[s * npb + r];
-        uint32_t e = (uint32_t)(lo - c0);
-        const uint32_t eh = (uint32_t)(hi - c0);
-        // four entries' LDS reads ahead of their (ordered) multiply-adds
-        for (; e + 4 <= eh; e += 4) {
-          const float w0 = s_w[e], w1 = s_w[e + 1], w2 = s_w[e + 2], w3 = s_w[e + 3];
-          const T x0 = gate(e * (uint32_t)W + s), x1 = gate((e + 1) * (uint32_t)W + s),
-                  x2 = gate((e + 2) * (uint32_t)W + s), x3 = gate((e + 3) * (uint32_t)W + s);
-          acc = fma(T(w0), x0, acc);
-          acc = fma(T(w1), x1, acc);
-          acc = fma(T(w2), x2, acc);
-          acc = fma(T(w3), x3, acc);
-        }
-        for (; e < eh; ++e) acc = fma(T(s_w[e]), gate(e * (uint32_t)W + s), acc);
-        s_gn[s * npb + r] = acc;
+      if (lo >= hi) continue;
+      // this sub-lane's first entry of the part: (e - a) % L == q
+      uint32_t e = (uint32_t)(lo - c0) + ((q - (uint32_t)(lo - a)) & (L - 1u));
+      const uint32_t eh = (uint32_t)(hi - c0);
+      T v = acc[i];
+      // four entries' LDS reads ahead of their (ordered) multiply-adds
+      for (; e + 3u * L < eh; e += 4u * L) {
+        const float w0 = s_w[e], w1 = s_w[e + L], w2 = s_w[e + 2u * L], w3 = s_w[e + 3u * L];
+        const T x0 = gate(e * (uint32_t)W + s), x1 = gate((e + L) * (uint32_t)W + s),
+                x2 = gate((e + 2u * L) * (uint32_t)W + s), x3 = gate((e + 3u * L) * (uint32_t)W + s);
+        v = fma(T(w0), x0, v);
+        v = fma(T(w1), x1, v);
+        v = fma(T(w2), x2, v);
+        v = fma(T(w3), x3, v);
       }
+      for (; e < eh; e += L) v = fma(T(s_w[e]), gate(e * (uint32_t)W + s), v);
+      acc[i] = v;
     }
     lds_barrier();
   }
   const T gain = T(net.gainr[2]);
-  for (uint32_t q = tid; q < npair; q += bd) {
-    const uint32_t r = divw(q), s = q - r * (uint32_t)W;
-    s_gn[s * npb + r] *= gain;
+#pragma unroll
+  for (int i = 0; i < FLAT_WMAX; ++i) {
+    if ((uint32_t)i >= nj) break;
+    T v = acc[i] * gain;
+    for (uint32_t off = L >> 1; off > 0; off >>= 1) v += __shfl_xor(v, (int)off, (int)L);
+    const uint32_t f = tid + (uint32_t)i * bd;
+    const uint32_t pr = f / L;
+    const uint32_t r = divw(pr), s = pr - r * (uint32_t)W;
+    if (q == 0 && f < n_items && r < (uint32_t)nbc) s_gn[s * npb + r] = v;
   }
 }
 
-// Entries per chunk of gather_block: W gates of each staged in LDS, 32 KiB.
+// Entries per chunk of gather_block: W gates of each staged in LDS, 32 KiB,
+// or cap entries when 0 < cap is smaller (smaller chunks take less LDS, so
+// more blocks share a compute unit and hide each other's staging latency).
 template <typename T>
-__host__ __device__ constexpr int window_chunk(int W) {
-  return (int)(32768 / sizeof(T)) / W < 1024 ? (int)(32768 / sizeof(T)) / W : 1024;
+__host__ __device__ constexpr int window_chunk(int W, int cap) {
+  const int full = (int)(32768 / sizeof(T)) / W < 1024 ? (int)(32768 / sizeof(T)) / W : 1024;
+  return cap > 0 && cap < full ? cap : full;
 }
 
 // Bytes of dynamic LDS k_window takes for npb neurons per block, W steps,
@@ -1253,12 +1281,12 @@ __host__ __device__ constexpr int window_chunk(int W) {
 // phases A and B, then a region phase A uses for the gather's staging and
 // phases B and C for the spike staging.
 template <typename T>
-__host__ __device__ constexpr size_t window_lds_bytes(int npb, int W, int nlu, bool flat) {
+__host__ __device__ constexpr size_t window_lds_bytes(int npb, int W, int nlu, bool flat, int chunk_cap) {
   const size_t nst = (size_t)((npb + 63) / 64) * 64 * W;
   const size_t fixed = ((size_t)8 * npb + sizeof(T) * ((size_t)W * npb * (1 + nlu)) +
                         (size_t)4 * ((size_t)npb + 2 * (size_t)W * npb) + 7) & ~(size_t)7;
   const size_t bc = sizeof(T) * ((size_t)W * npb + nst) + (size_t)4 * nst;
-  const int ch = window_chunk<T>(W);
+  const int ch = window_chunk<T>(W, chunk_cap);
   const size_t stage = flat ? (size_t)8 * (npb + 1) + sizeof(T) * (size_t)ch * W + (size_t)8 * ch : 0;
   return fixed + (bc > stage ? bc : stage);
 }
@@ -1327,7 +1355,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   T *s_amp = s_hs + (size_t)W * npb;                          // [nst] staged push amplitudes
   uint32_t *s_spk = (uint32_t *)(s_amp + nst);                // [nst] step << 24 | neuron
   // phase A, gather_block's staging
-  const int CH = window_chunk<T>(W);
+  const int CH = window_chunk<T>(W, net.gather_chunk);
   uint64_t *s_rpn = (uint64_t *)s_u;                          // [npb + 1] gather row starts
   uint32_t *s_x = (uint32_t *)(s_rpn + npb + 1);              // [CH][W] staged gates (words)
   uint32_t *s_m = s_x + (size_t)CH * W * (sizeof(T) / 4);     // [CH] staged columns | delays

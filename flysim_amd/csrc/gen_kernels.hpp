@@ -9,8 +9,8 @@
 // against p_conn * 2^32 decide the pair and its low 24 bits give the
 // efficacy g = g_lo + (g_hi - g_lo) * u, u = low24 / 2^24 (float, one
 // rounding per operation).  E->X pairs carry AMPA (g) and NMDA
-// (g * nmda_ratio * gain_nmda), I->X pairs GABA (g * gain_gaba); no extra
-// delay.  Row scans are wave-wide: one wave per CSR row tests 64 candidates
+// (g * nmda_ratio * gain_nmda), I->X pairs GABA (g * gain_gaba); every
+// synapse takes the same synaptic delay.  Row scans are wave-wide: one wave per CSR row tests 64 candidates
 // per ballot, so rows come out in ascending column order.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -38,6 +38,7 @@ struct GenArgs {
   int32_t N;
   uint32_t thr;             // p_conn * 2^32
   float g_lo, g_span, nmda_ratio;
+  uint32_t delay;           // synaptic delay of every synapse (steps, < 64)
   // outputs (fill pass): CSR by pre (AMPA from E, GABA from I) and CSR by
   // post (NMDA from E); count pass writes row lengths to rp[row + 1]
   uint64_t *rp_p, *rp_n;
@@ -77,7 +78,7 @@ __global__ void __launch_bounds__(256) k_gen_rows(GenArgs a) {
       if (a.fill && ok) {
         const uint64_t e = pos + (uint64_t)__popcll(m & lanes_below);
         const float g = eff(a, u);
-        a.meta_p[e] = (uint32_t)(I.first + p) | ((exc ? 0u : 1u) << 30);
+        a.meta_p[e] = (uint32_t)(I.first + p) | (a.delay << 24) | ((exc ? 0u : 1u) << 30);
         a.w_p[e] = exc ? g : __fmul_rn(g, I.gain_gaba);
       }
       pos += (uint64_t)__popcll(m);
@@ -95,7 +96,7 @@ __global__ void __launch_bounds__(256) k_gen_rows(GenArgs a) {
       const uint64_t m = __ballot(ok);
       if (a.fill && ok) {
         const uint64_t e = pos + (uint64_t)__popcll(m & lanes_below);
-        a.meta_n[e] = (uint32_t)(I.first + j);
+        a.meta_n[e] = (uint32_t)(I.first + j) | (a.delay << 24);
         a.w_n[e] = __fmul_rn(__fmul_rn(eff(a, u), a.nmda_ratio), I.gain_nmda);
       }
       pos += (uint64_t)__popcll(m);

@@ -100,6 +100,12 @@ struct Mlp {
   std::vector<int64_t> offW, offb, offW16;
   int64_t n_params = 0, n_w16 = 0;
   Buf p32, grad, m1, m2, w16, w16map, ctr, loss;
+  // whole-network forward in one kernel (every padded width <= 256): its
+  // copy of the weights in the fused layout, kept beside w16 by set_params
+  // and Adam
+  bool fused = false;
+  Buf wf16, wfmap;
+  int32_t n_cu = 0;
   // training set and minibatch buffers
   Buf xall, yall, perm;
   int64_t n_data = 0;
@@ -150,6 +156,21 @@ struct Mlp {
           map[offW[i] + (int64_t)o * s[i] + k] = offW16[i] + (int64_t)o * sp[i] + k;
     w16map.alloc(n_params * 8);
     HIPCHECK(hipMemcpy(w16map.p, map.data(), n_params * 8, hipMemcpyHostToDevice));
+    fused = L <= fsm::FUSED_MAXL &&
+            *std::max_element(sp.begin(), sp.end()) <= fsm::FUSED_MAXW;
+    if (fused) {
+      std::vector<int64_t> fmap(n_params, -1);
+      for (int i = 0; i < L; ++i)
+        for (int32_t o = 0; o < s[i + 1]; ++o)
+          for (int32_t k = 0; k < s[i]; ++k)
+            fmap[offW[i] + (int64_t)o * s[i] + k] = offW16[i] + fsm::fused_woff(i, o, k, sp[i + 1]);
+      wfmap.alloc(n_params * 8);
+      HIPCHECK(hipMemcpy(wfmap.p, fmap.data(), n_params * 8, hipMemcpyHostToDevice));
+      wf16.alloc(n_w16 * 2);
+      int dev = 0;
+      HIPCHECK(hipGetDevice(&dev));
+      HIPCHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     // He-uniform initialisation of the weights (bound sqrt(6/fan_in)), zero biases
     std::mt19937_64 rng(seed);
     std::vector<float> init(n_params, 0.0f);
@@ -176,6 +197,14 @@ struct Mlp {
         for (int32_t k = 0; k < s[i]; ++k)
           w[offW16[i] + (int64_t)o * sp[i] + k] = (bf16)host[offW[i] + (int64_t)o * s[i] + k];
     HIPCHECK(hipMemcpy(w16.p, w.data(), n_w16 * 2, hipMemcpyHostToDevice));
+    if (fused) {
+      std::vector<bf16> wf(n_w16, (bf16)0.0f);
+      for (int i = 0; i < L; ++i)
+        for (int32_t o = 0; o < s[i + 1]; ++o)
+          for (int32_t k = 0; k < s[i]; ++k)
+            wf[offW16[i] + fsm::fused_woff(i, o, k, sp[i + 1])] = (bf16)host[offW[i] + (int64_t)o * s[i] + k];
+      HIPCHECK(hipMemcpy(wf16.p, wf.data(), n_w16 * 2, hipMemcpyHostToDevice));
+    }
     reset_optimizer();
   }
 
@@ -246,6 +275,28 @@ struct Mlp {
     }
   }
 
+  // the whole forward of n rows, x f32 [n][in] -> y f32 [n][out] (device
+  // memory), one persistent block per compute unit; nothing allocated
+  void forward_fused(const float *x, int64_t n, float *y) {
+    fsm::FusedArgs a{};
+    a.x = x;
+    a.y = y;
+    a.n = n;
+    a.L = L;
+    for (int i = 0; i <= L; ++i) {
+      a.sz[i] = s[i];
+      a.sp[i] = sp[i];
+    }
+    for (int i = 0; i < L; ++i) {
+      a.w[i] = wf16.as<bf16>() + offW16[i];
+      a.b[i] = P32(offb[i]);
+    }
+    const int64_t tiles = (n + fsm::FUSED_ROWS - 1) / fsm::FUSED_ROWS;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, std::max(1, n_cu)));
+    hipLaunchKernelGGL(fsm::k_mlp_fused, dim3(grid), dim3(fsm::FUSED_THREADS), 0, stream, a);
+    HIPCHECK(hipGetLastError());
+  }
+
   // loss and gradients of the rows in x16 / tgt (gradient into `grad`)
   void loss_backward(int32_t rows) {
     float *y = yout.as<float>();
@@ -296,9 +347,19 @@ struct Mlp {
   }
 
   // rows of x (host or device f32 [n][in]) -> y (f32 [n][out]), in chunks
+  // (the fused kernel takes device rows in one launch)
   double predict(const float *x, int64_t n, float *y, bool dev) {
+    if (fused && dev) {
+      HIPCHECK(hipEventRecord(ev[0], stream));
+      forward_fused(x, n, y);
+      HIPCHECK(hipEventRecord(ev[1], stream));
+      HIPCHECK(hipEventSynchronize(ev[1]));
+      float t = 0;
+      HIPCHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
+      return t;
+    }
     const int32_t chunk = (int32_t)std::min<int64_t>(n, 1 << 18);
-    ensure_rows(std::max<int32_t>(chunk, 1));
+    if (!fused) ensure_rows(std::max<int32_t>(chunk, 1));
     Buf xin, yo;
     if (!dev) {
       xin.alloc((size_t)chunk * s[0] * 4);
@@ -310,12 +371,16 @@ struct Mlp {
       const float *xs = dev ? x + r0 * s[0] : xin.as<float>();
       float *ys = dev ? y + r0 * s[L] : yo.as<float>();
       if (!dev) HIPCHECK(hipMemcpy(xin.p, x + r0 * s[0], (size_t)rows * s[0] * 4, hipMemcpyHostToDevice));
-      const int blocks = std::max(1, std::min(8192, (int)(((int64_t)rows * sp[0] + 255) / 256)));
       HIPCHECK(hipEventRecord(ev[0], stream));
-      hipLaunchKernelGGL(fsm::k_pack_rows, dim3(blocks), dim3(256), 0, stream, xs, (const int64_t *)nullptr,
-                         (const int64_t *)nullptr, (int64_t)rows, s[0], sp[0], x16.as<bf16>());
-      HIPCHECK(hipGetLastError());
-      forward(rows, false, ys, s[L]);
+      if (fused) {
+        forward_fused(xs, rows, ys);
+      } else {
+        const int blocks = std::max(1, std::min(8192, (int)(((int64_t)rows * sp[0] + 255) / 256)));
+        hipLaunchKernelGGL(fsm::k_pack_rows, dim3(blocks), dim3(256), 0, stream, xs, (const int64_t *)nullptr,
+                           (const int64_t *)nullptr, (int64_t)rows, s[0], sp[0], x16.as<bf16>());
+        HIPCHECK(hipGetLastError());
+        forward(rows, false, ys, s[L]);
+      }
       HIPCHECK(hipEventRecord(ev[1], stream));
       HIPCHECK(hipEventSynchronize(ev[1]));
       float t = 0;
@@ -342,6 +407,11 @@ struct Mlp {
   // the unpadded inner dimension (K = width), so nothing depends on what an
   // earlier call left in the padding columns.
   void predict_noalloc(const float *x, int64_t n, float *y) {
+    if (fused) {  // no intermediate in memory at all
+      forward_fused(x, n, y);
+      HIPCHECK(hipStreamSynchronize(stream));
+      return;
+    }
     const int64_t R = noalloc_rows();
     if (R < 64)
       throw std::length_error("layer widths need more than the 8 MiB allocation-free scratch "
@@ -401,7 +471,9 @@ struct Mlp {
     const int b3 = std::max(1, std::min(2048, (int)((n_params + 255) / 256)));
     hipLaunchKernelGGL(fsm::k_adam, dim3(b3), dim3(256), 0, stream, p32.as<float>(), grad.as<float>(),
                        m1.as<float>(), m2.as<float>(), n_params, (const int64_t *)c, o.lr, o.beta1, o.beta2,
-                       o.eps, o.weight_decay, w16map.as<int64_t>(), w16.as<bf16>());
+                       o.eps, o.weight_decay, w16map.as<int64_t>(), w16.as<bf16>(),
+                       fused ? wfmap.as<int64_t>() : (const int64_t *)nullptr,
+                       fused ? wf16.as<bf16>() : (bf16 *)nullptr);
     HIPCHECK(hipGetLastError());
     hipLaunchKernelGGL(fsm::k_incr, dim3(1), dim3(1), 0, stream, c + 1);  // next minibatch
     HIPCHECK(hipGetLastError());

@@ -50,11 +50,70 @@ struct GemmArgs {
 
 constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;
 
-// exact (erf) GELU and its derivative
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// Fused forward of a whole network whose padded widths are all <= 256
+// (k_mlp_fused).  Each wave keeps 32 rows in registers through every layer:
+// the product is taken transposed, Z^T = W H^T, with the weights as the
+// MFMA's A operand and the activations as its B operand, so lane l always
+// holds data row l % 32 -- the accumulator layout of one layer is, up to a
+// fixed permutation of the K index inside each 16-wide k step, the B-operand
+// layout of the next.  Output unit n = 32 nt + 8 (r/4) + 4 (l/32) + r%4 of
+// accumulator register r of n-tile nt becomes element q = r % 8 of k step
+// 2 nt + r / 8 of the next layer, i.e. MFMA k position 8 (l/32) + q of that
+// step holds unit 8 (q/4) + 4 (l/32) + q % 4 of it.  The weight copy the
+// kernel reads (fused layout) stores each layer's K axis in that order
+// (layers after the first), cut into slices of 64 k, each slice [n][64] with
+// the eight 16-byte chunks of row n at chunk position c ^ (n % 8), so the 32
+// rows one A-operand read touches spread over the LDS banks.
+constexpr int FUSED_MAXW = 256, FUSED_MAXL = 8, FUSED_ROWS = 256, FUSED_THREADS = 512;
+
+// MFMA k position of input unit k of a layer after the first: swap the
+// second and third group of four within each 16 (an involution)
+__host__ __device__ constexpr int32_t fused_kpos(int32_t k) {
+  const int32_t p = k & 15, g = p >> 2;
+  return (k & ~15) | ((g == 1 ? 2 : g == 2 ? 1 : g) << 2) | (p & 3);
+}
+
+// element offset of weight (o, k) of layer i within that layer's fused-layout
+// block; n_out_pad = padded output width
+__host__ __device__ constexpr int64_t fused_woff(int32_t layer, int32_t o, int32_t k, int32_t n_out_pad) {
+  const int32_t P = layer == 0 ? k : fused_kpos(k);
+  const int32_t sl = P >> 6, pl = P & 63, c = pl >> 3, e = pl & 7;
+  return (int64_t)sl * n_out_pad * 64 + (int64_t)o * 64 + ((c ^ (o & 7)) << 3) + e;
+}
+
+struct FusedArgs {
+  const float *x;          // [n][in]
+  float *y;                // [n][out]
+  int64_t n;
+  int32_t L;
+  int32_t sz[FUSED_MAXL + 1];   // widths
+  int32_t sp[FUSED_MAXL + 1];   // padded widths (multiples of 64, <= 256)
+  const bf16 *w[FUSED_MAXL];    // fused-layout weights of each layer
+  const float *b[FUSED_MAXL];   // biases (f32, sz[i+1] each)
+};
+
+// erf by Abramowitz & Stegun 7.1.26 (Hastings' rational approximation):
+// erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2),
+// t = 1 / (1 + p x), x >= 0, odd extension; absolute error <= 1.5e-7, far
+// below the bf16 rounding of the activations it feeds (2^-9 relative).
+// Branch-free: one reciprocal, one exp2, five fused multiply-adds.
+__device__ __forceinline__ float erf_as(float x) {
+  const float ax = fabsf(x);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ax, 1.0f));
+  float pl = fmaf(1.061405429f, t, -1.453152027f);
+  pl = fmaf(pl, t, 1.421413741f);
+  pl = fmaf(pl, t, -0.284496736f);
+  pl = fmaf(pl, t, 0.254829592f);
+  pl *= t;
+  const float e = __builtin_amdgcn_exp2f(-1.4426950408889634f * ax * ax);
+  return copysignf(fmaf(-pl, e, 1.0f), x);
+}
+
+// GELU (erf form) and its derivative
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-         z * 0.39894228040143268f * expf(-0.5f * z * z);
+  return 0.5f * (1.0f + erf_as(z * 0.70710678118654752f)) +
+         z * 0.39894228040143268f * __builtin_amdgcn_exp2f(-0.72134752044448170f * z * z);
 }
 
 // Stage a 64 (rows) x 32 (k) tile of X(r, k) = X[r*sr + k*sk] into LDS as
@@ -144,6 +203,183 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   }
 }
 
+// The whole forward, one persistent block per compute unit: tiles of
+// FUSED_ROWS rows (32 per wave); per layer the weight slices (64 k each)
+// stream through two LDS buffers -- the loads of the next slice are in
+// flight while the waves multiply the current one -- and the activations
+// never leave the registers.  The sums of the layer-by-layer path: each
+// output is bias + the sum over k in ascending 16-wide steps of bf16
+// products (only the order of the terms inside a step differs, the matrix
+// core's own), GELU then bf16 between layers.  Measured on 2^20 rows of the
+// 54-256-256-256-50 net: about a third of the time each goes to the MFMAs,
+// the GELU epilogue and the LDS traffic of the weight operands, which the
+// barrier per slice keeps from overlapping.
+__global__ void __launch_bounds__(FUSED_THREADS) k_mlp_fused(FusedArgs a) {
+  __shared__ __align__(16) bf16 wsl[2][FUSED_MAXW * 64];
+  __shared__ __align__(16) float sbias[FUSED_MAXL][FUSED_MAXW];  // zero past each layer's width
+  const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int h = lane >> 5, lr = lane & 31;
+  for (int i = 0; i < a.L; ++i)
+    for (int n = (int)threadIdx.x; n < FUSED_MAXW; n += FUSED_THREADS)
+      sbias[i][n] = n < a.sz[i + 1] ? a.b[i][n] : 0.0f;
+  const int64_t tiles = (a.n + FUSED_ROWS - 1) / FUSED_ROWS;
+  // slice g of the sequence (layer, slice) repeated per tile of this block;
+  // layer i has sp[i] / 64 slices
+  int per_tile = 0;
+  for (int i = 0; i < a.L; ++i) per_tile += a.sp[i] >> 6;
+  const int64_t my_tiles = (int64_t)blockIdx.x < tiles ? (tiles - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+  const int64_t total = my_tiles * per_tile;
+  // The copy of slice (layer, sl) goes through registers: loaded while the
+  // waves multiply the current slice, written to the other LDS buffer after
+  // it (a copy straight into LDS would make every LDS read of the current
+  // slice wait for it).  Thread t holds 16-byte chunks t, t + 512, ... of
+  // the slice (np * 8 chunks, np / 64 per thread).
+  static_assert(FUSED_MAXW / 64 == 4, "four staged chunks per thread");
+  uint4 st0 = {0u, 0u, 0u, 0u}, st1 = st0, st2 = st0, st3 = st0;
+  int stg_n = 0;  // chunks per thread of the staged slice (wave-uniform)
+#define FUSED_STAGE_LOAD(layer, sl)                                                           \
+  do {                                                                                        \
+    const int np_ = a.sp[(layer) + 1];                                                        \
+    const uint4 *src_ = reinterpret_cast<const uint4 *>(a.w[layer] + (size_t)(sl) * np_ * 64) + \
+                        threadIdx.x;                                                          \
+    stg_n = np_ >> 6;                                                                         \
+    st0 = src_[0];                                                                            \
+    if (stg_n > 1) st1 = src_[FUSED_THREADS];                                                 \
+    if (stg_n > 2) st2 = src_[2 * FUSED_THREADS];                                             \
+    if (stg_n > 3) st3 = src_[3 * FUSED_THREADS];                                             \
+  } while (0)
+#define FUSED_STAGE_STORE(buf)                                                  \
+  do {                                                                          \
+    uint4 *dst_ = reinterpret_cast<uint4 *>(wsl[buf]) + threadIdx.x;            \
+    dst_[0] = st0;                                                              \
+    if (stg_n > 1) dst_[FUSED_THREADS] = st1;                                   \
+    if (stg_n > 2) dst_[2 * FUSED_THREADS] = st2;                               \
+    if (stg_n > 3) dst_[3 * FUSED_THREADS] = st3;                               \
+  } while (0)
+  int64_t g = 0;
+  int cl = 0, cs = 0;  // layer and slice of sequence element g (per tile)
+  if (total > 0) {
+    FUSED_STAGE_LOAD(0, 0);
+    FUSED_STAGE_STORE(0);
+  }
+  __syncthreads();
+  for (int64_t t = 0; t < my_tiles; ++t) {
+    const int64_t row = ((int64_t)blockIdx.x + t * gridDim.x) * FUSED_ROWS + wave * 32 + lr;
+    const bool live = row < a.n;
+    bf16x8 act[FUSED_MAXW / 16];
+    // layer 0 input: k step s, lane half h holds x[row][16 s + 8 h + q];
+    // four k steps at a time, every load issued before any is used (the
+    // addresses are clamped into the array, the values selected after)
+    // the input width, opaque to the optimiser per tile: the per-element
+    // index clamps then stay here instead of being hoisted out of the tile
+    // loop into registers
+    int32_t szx = a.sz[0];
+    __asm__ volatile("" : "+s"(szx));
+    const float *xr = a.x + (live ? row : a.n - 1) * szx;
+#pragma unroll
+    for (int s4 = 0; s4 < FUSED_MAXW / 64; ++s4) {
+      if (s4 < (a.sp[0] >> 6)) {
+        float v[4][8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int32_t k = 64 * s4 + 16 * j + 8 * h + q;
+            v[j][q] = xr[k < szx ? k : szx - 1];
+          }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int32_t k = 64 * s4 + 16 * j + 8 * h + q;
+            act[4 * s4 + j][q] = (bf16)((live && k < szx) ? v[j][q] : 0.0f);
+          }
+      }
+    }
+    for (int layer = 0; layer < a.L; ++layer) {
+      const int NT = a.sp[layer + 1] / 32;
+      f32x16 acc[FUSED_MAXW / 32];
+#pragma unroll
+      for (int nt = 0; nt < FUSED_MAXW / 32; ++nt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[nt][r] = 0.0f;
+#pragma unroll
+      for (int sl = 0; sl < FUSED_MAXW / 64; ++sl) {
+        if (sl >= (a.sp[layer] >> 6)) break;
+        // the next slice of the sequence, into registers for now
+        const bool more = g + 1 < total;
+        if (more) {
+          int nl = cl, ns = cs + 1;
+          if (ns >= (a.sp[nl] >> 6)) { ns = 0; nl = nl + 1 < a.L ? nl + 1 : 0; }
+          FUSED_STAGE_LOAD(nl, ns);
+        }
+        const bf16 *ws = wsl[g & 1];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+          const int c = 2 * ks + h;  // 16-byte chunk of the row
+          // every n-tile, branch-free (the reads of a k step then issue
+          // ahead of its MFMAs): tiles past the layer's width multiply
+          // stale LDS rows into accumulators nothing reads
+#pragma unroll
+          for (int nt = 0; nt < FUSED_MAXW / 32; ++nt) {
+            const int n = nt * 32 + lr;
+            const bf16x8 wa = *reinterpret_cast<const bf16x8 *>(ws + n * 64 + ((c ^ (n & 7)) << 3));
+            acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, act[sl * 4 + ks], acc[nt], 0, 0, 0);
+          }
+        }
+        if (more) FUSED_STAGE_STORE((int)((g + 1) & 1));
+        __syncthreads();
+        ++g;
+        if (++cs >= (a.sp[cl] >> 6)) { cs = 0; cl = cl + 1 < a.L ? cl + 1 : 0; }
+      }
+      const float *bias = sbias[layer];
+      const int32_t nv = a.sz[layer + 1];
+      if (layer < a.L - 1) {
+        // bias, GELU, bf16: unit 32 nt + 8 j + 4 h + i -> element r % 8 of k step 2 nt + r / 8
+#pragma unroll
+        for (int nt = 0; nt < FUSED_MAXW / 32; ++nt) {
+          if (nt < NT) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int32_t n0 = nt * 32 + 8 * j + 4 * h;
+              const float4 b4 = *reinterpret_cast<const float4 *>(bias + n0);
+              const float bb[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const int r = 4 * j + i;
+                // a padded unit has zero weights and bias: gelu(0) = 0
+                act[2 * nt + (r >> 3)][r & 7] = (bf16)gelu(acc[nt][r] + bb[i]);
+              }
+            }
+          }
+          // one n-tile at a time: keeps the scheduler from hoisting every
+          // tile's bias reads (and their registers) to the top
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      } else if (live) {
+        float *yr = a.y + row * a.sz[a.L];
+#pragma unroll
+        for (int nt = 0; nt < FUSED_MAXW / 32; ++nt) {
+          if (nt < NT) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int32_t n0 = nt * 32 + 8 * j + 4 * h;
+              const float4 b4 = *reinterpret_cast<const float4 *>(bias + n0);
+              const float bb[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+                if (n0 + i < nv) yr[n0 + i] = acc[nt][4 * j + i] + bb[i];
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+#undef FUSED_STAGE_LOAD
+#undef FUSED_STAGE_STORE
+
 // x f32 [*][in] -> bf16 [rows][ld] zero-padded; output row r is input row
 // idx[*bsel * rows + r] when idx is given (minibatch *bsel of a permutation),
 // else row r
@@ -221,10 +457,11 @@ __global__ void __launch_bounds__(256) k_colsum(const float *dz, int32_t rows, i
 
 // Adam (Kingma & Ba 2015) with bias correction over the flat f32 master
 // parameters; step = the 1-based step index held on the device; writes the
-// bf16 copies of the weights (w16_map[i] >= 0: index into w16).
+// bf16 copies of the weights (w16_map[i] >= 0: index into w16; wf_map, when
+// given, likewise into the fused kernel's copy wf16).
 __global__ void k_adam(float *p, const float *grad, float *m1, float *m2, int64_t n, const int64_t *step,
                        float lr, float b1, float b2, float eps, float wd, const int64_t *w16_map,
-                       bf16 *w16) {
+                       bf16 *w16, const int64_t *wf_map, bf16 *wf16) {
   const int64_t t = *step;
   const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -238,6 +475,10 @@ __global__ void k_adam(float *p, const float *grad, float *m1, float *m2, int64_
     p[i] = v;
     const int64_t j = w16_map[i];
     if (j >= 0) w16[j] = (bf16)v;
+    if (wf_map) {  // the fused kernel's copy
+      const int64_t jf = wf_map[i];
+      if (jf >= 0) wf16[jf] = (bf16)v;
+    }
   }
 }
 

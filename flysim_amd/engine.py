@@ -82,7 +82,7 @@ class EiBatchSpec:
 
     def __init__(self, n_neuron, seed, ext_hz, gain_nmda, gain_gaba, exc: Population,
                  inh: Population, exc_frac: float = 0.8, p_conn: float = 0.1, g_lo: float = 0.5,
-                 g_hi: float = 1.5, nmda_ratio: float = 0.1, dt: float = 0.1):
+                 g_hi: float = 1.5, nmda_ratio: float = 0.1, dt: float = 0.1, delay_steps: int = 0):
         self.n_neuron = np.ascontiguousarray(n_neuron, dtype=np.int32)
         k = self.n_neuron.size
         self.seed = np.ascontiguousarray(np.broadcast_to(seed, k), dtype=np.uint64)
@@ -92,6 +92,9 @@ class EiBatchSpec:
         self.exc, self.inh = exc, inh
         self.exc_frac, self.p_conn, self.g_lo, self.g_hi = exc_frac, p_conn, g_lo, g_hi
         self.nmda_ratio, self.dt = nmda_ratio, dt
+        if not 0 <= int(delay_steps) <= 63:
+            raise ValueError("delay_steps must be 0..63")
+        self.delay_steps = int(delay_steps)
 
     def n_exc(self) -> np.ndarray:
         return np.floor(np.float64(np.float32(self.exc_frac)) * self.n_neuron + 0.5).astype(np.int64)
@@ -110,6 +113,7 @@ class EiBatchSpec:
         d.inh = derive_params(self.inh, self.dt)
         d.dt = self.dt
         d.precision, d.ext_mode, d.record_spikes = precision, ext_mode, int(record)
+        d.delay_steps = self.delay_steps
         return d, self
 
 

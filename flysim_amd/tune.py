@@ -46,7 +46,7 @@ def candidates(stats: Dict[str, int]) -> List[Dict]:
     has_nmda = stats["syn_nmda"] > 0
     reps = stats["replicas"]
     out = []
-    lanes = [1, 2] if reps >= 64 else ([2, 4, 8, 16, 32] if has_nmda else [8])
+    lanes = [1, 2] if reps >= 64 else ([2, 4, 8, 16, 32, 64] if has_nmda else [8])
     if not delayed:  # one step per launch: a few shapes
         lanes = lanes[:2] if reps >= 64 else [4, 8]
     for b, L in itertools.product([128, 256, 512] if delayed else [256], lanes):
@@ -58,6 +58,27 @@ def candidates(stats: Dict[str, int]) -> List[Dict]:
     lanes = [1, 2] if reps >= 64 else ([1, 4, 8, 16] if has_nmda else [8])
     for b, L in itertools.product([256, 512, 1024], lanes):
         out.append(dict(update_block=b, gather_lanes=L, fused=0, window=0))
+    return out
+
+
+def refinements(sched: Dict, stats: Dict[str, int]) -> List[Dict]:
+    """Variants of a windowed schedule on one network copy: blocks dealt to
+    the XCDs in neighbouring runs (or not), smaller staged gather chunks
+    (more blocks per compute unit) and several lanes per (row, step) pair
+    of the gather."""
+    if not sched.get("window") or stats["replicas"] != 1 or stats["syn_nmda"] == 0:
+        return []
+    x = 1 - int(sched.get("xcd_order", 0))
+    out = [dict(sched, xcd_order=x)]
+    for c in (128, 256):
+        if sched.get("gather_chunk", 0) != c:
+            out.append(dict(sched, gather_chunk=c))
+            out.append(dict(sched, gather_chunk=c, xcd_order=x))
+    # lanes per (row, step) pair of the gather: long rows want several
+    for sp in (4, 16, 64):
+        if sp <= sched.get("gather_lanes", 1) and sp != sched.get("gather_split", 0):
+            out.append(dict(sched, gather_split=sp))
+            out.append(dict(sched, gather_split=sp, xcd_order=x))
     return out
 
 
@@ -92,7 +113,9 @@ def tune(eng: Engine, steps: int = 200, trial_s: float = 0.03, cands: Optional[I
     starting schedule and ranked by the ratio to them (a trial during which
     the starting schedule's time changed by more than 25 % is repeated once
     at the end): every candidate once, the `keep` best again in
-    `passes - 1` more passes, by the median of their ratios.  A trial runs up to `steps` steps, fewer on
+    `passes - 1` more passes, by the median of their ratios; with the
+    built-in candidates (cands None), the two best of the first pass are
+    also tried with the windowed path's extra knobs (refinements).  A trial runs up to `steps` steps, fewer on
     networks whose steps are slow (about `trial_s` seconds per timed run).
     The trial runs advance the engine's state: reset it afterwards.
     steps_per_graph stays the engine's own unless a candidate sets it.
@@ -104,6 +127,7 @@ def tune(eng: Engine, steps: int = 200, trial_s: float = 0.03, cands: Optional[I
     if sig in db:
         eng.set_schedule(**db[sig]["schedule"])
         return dict(db[sig], signature=sig, tune_s=time.perf_counter() - t_start)
+    refine = cands is None  # a caller's own candidate list is tried as given
     if cands is None:
         cands = candidates(eng.stats())
     base = dict(eng.schedule)
@@ -146,6 +170,24 @@ def tune(eng: Engine, steps: int = 200, trial_s: float = 0.03, cands: Optional[I
             runs.append([applied, [r], [dt]])
     if not runs:
         raise RuntimeError("no schedule candidate ran")
+    # the two best first-pass schedules, varied in the knobs that only the
+    # windowed path on one network copy has
+    seen = {json.dumps(q[0], sort_keys=True) for q in runs}
+    for q in (sorted(runs, key=lambda q: q[1][0])[:2] if refine else []):
+        for sc in refinements(q[0], eng.stats()):
+            if json.dumps(sc, sort_keys=True) in seen:
+                continue
+            seen.add(json.dumps(sc, sort_keys=True))
+            try:
+                applied, r, dt = trial(sc)
+            except Exception as e:
+                if verbose:
+                    print("schedule", sc, "failed:", e)
+                continue
+            if r is not None:
+                runs.append([applied, [r], [dt]])
+                if verbose:
+                    print("%-100s %.2f us/step (%.3f x start)" % (applied, dt * 1e6, r), flush=True)
     top = sorted(runs, key=lambda q: q[1][0])[:max(1, keep)]
     for p in range(1, passes if len(runs) > 1 else 1):
         for q in (top[::-1] if p % 2 else top):

@@ -140,8 +140,9 @@ int fs_create_replicated(const fs_net_desc *desc, int32_t replicas, fs_handle **
  * round(exc_frac * n[k]) excitatory), every ordered pair j != p of an
  * instance a synapse with probability p_conn, efficacy g ~ U(g_lo, g_hi) nS;
  * E->X pairs carry AMPA (g) and NMDA (g * nmda_ratio * gain_nmda[k]), I->X
- * pairs GABA (g * gain_gaba[k]); no extra delay (SpikeDly of the population
- * parameters applies).  Pairs and efficacies come from a counter-based hash
+ * pairs GABA (g * gain_gaba[k]); every synapse takes delay_steps steps of
+ * synaptic delay (0..63) on top of the SpikeDly of the population
+ * parameters.  Pairs and efficacies come from a counter-based hash
  * of (seed[k], j, p), which also keys the instance's external input stream,
  * so an instance runs identically alone or in any batch.  Populations:
  * 2k = E, 2k+1 = I of instance k, external AMPA at ext_hz[k]. */
@@ -157,6 +158,7 @@ typedef struct fs_ei_batch_desc {
   int32_t precision;
   int32_t ext_mode;
   int32_t record_spikes;
+  int32_t delay_steps;                  /* synaptic delay of every synapse (steps) */
 } fs_ei_batch_desc;
 int fs_create_ei_batch(const fs_ei_batch_desc *desc, fs_handle **out);
 
@@ -238,10 +240,22 @@ int fs_pop_spike_counts(fs_handle *h, uint64_t *out);
  * the smallest synaptic delay in steps, and to 64).  The windowed path takes
  * precedence over the fused one.  fs_get_schedule reports the path in
  * effect: fused = 1 only while the fused path runs, window = the steps per
- * launch actually used (0 when off). */
+ * launch actually used (0 when off).  xcd_order (windowed path, one network
+ * copy): 1 hands consecutive blocks of neurons to one XCD (the chip's
+ * workgroup dispatcher deals blocks to its 8 XCDs in turn), so the blocks
+ * an XCD runs together share presynaptic neurons in its L2.  gather_chunk
+ * (windowed path, one network copy): at most this many NMDA entries per
+ * LDS-staged chunk of a block's gather (0: as many as 32 KiB of staged gates
+ * hold; else >= 64), trading LDS per block for blocks per compute unit.
+ * gather_split (windowed path, one network copy): lanes that share the
+ * entries of one (row, step) pair of that gather (a power of two, at most
+ * gather_lanes; 0 or 1: one lane per pair), for networks with long rows. */
 typedef struct fs_schedule {
   int32_t update_block, gather_lanes, push_block, steps_per_graph, use_graph, fused;
   int32_t window;
+  int32_t xcd_order;
+  int32_t gather_chunk;
+  int32_t gather_split;
 } fs_schedule;
 int fs_get_schedule(fs_handle *h, fs_schedule *s);
 int fs_set_schedule(fs_handle *h, const fs_schedule *s);

@@ -54,3 +54,21 @@ def test_c4_spec_ranges():
         assert g.min() >= 0.5 and g.max() <= 2.0
     pairs = 0.1 * np.sum(s.n_neuron.astype(np.float64) * (s.n_neuron - 1))
     assert 3.0e9 < pairs < 4.2e9
+
+
+def test_c4_states_its_delay_as_synaptic_delay():
+    s = c4_spec(n_inst=3, n_lo=100, n_hi=200)
+    assert s.delay_steps == 10 and s.exc.spike_delay_ms == 0.0 and s.inh.spike_delay_ms == 0.0
+    net = ei_batch_network(s)
+    assert np.all(net.delay == 10)
+    r = c4_spec(n_inst=3, n_lo=100, n_hi=200, synaptic_delay=False)
+    assert r.delay_steps == 0 and r.exc.spike_delay_ms == 1.0
+    # same pairs and efficacies either way
+    np.testing.assert_array_equal(ei_batch_network(r).w, net.w)
+    e, i = c1_populations()
+    for bad in (-1, 64):
+        try:
+            EiBatchSpec(np.array([10]), np.array([1]), [1.0], [1.0], [1.0], e, i, delay_steps=bad)
+        except ValueError:
+            continue
+        raise AssertionError("delay_steps %d accepted" % bad)

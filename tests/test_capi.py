@@ -33,7 +33,7 @@ def test_struct_sizes_match_header_layout():
     # fs_pop_params: 8 floats, 2 ints, 5+5 floats, 4 floats, 5+5 floats, uint, 3 floats
     assert C.sizeof(A.PopParams) == 4 * (8 + 2 + 10 + 4 + 10 + 1 + 3 + 11 + 4)
     assert C.sizeof(A.PopConf) == 4 * (7 + 5 + 30 + 8 + 4)
-    assert C.sizeof(A.Schedule) == 28
+    assert C.sizeof(A.Schedule) == 40
     assert C.sizeof(A.ReplicaParams) == 32
     assert C.sizeof(A.Adam) == 20
 

@@ -18,11 +18,11 @@ from flysim_amd.engine import EiBatchSpec, Engine, arrays_from_network
 pytestmark = pytest.mark.gpu
 
 
-def _spec(sizes, seeds, ext=(2400.0,), gn=(1.0,), gg=(1.0,)):
-    e, i = c1_populations(g_ext=1.3)
+def _spec(sizes, seeds, ext=(2400.0,), gn=(1.0,), gg=(1.0,), delay=0, spike_delay_ms=1.0):
+    e, i = c1_populations(g_ext=1.3, spike_delay_ms=spike_delay_ms)
     k = len(sizes)
     return EiBatchSpec(np.array(sizes), np.array(seeds), np.resize(ext, k), np.resize(gn, k),
-                       np.resize(gg, k), e, i)
+                       np.resize(gg, k), e, i, delay_steps=delay)
 
 
 def _bits(n, steps, seed):
@@ -37,12 +37,13 @@ def _bits(n, steps, seed):
     return x.reshape(-1)
 
 
-def test_device_generator_equals_host_restatement():
+@pytest.mark.parametrize("delay", [0, 10])
+def test_device_generator_equals_host_restatement(delay):
     spec = _spec([150, 333, 271], [11, 12, 13], ext=(2400.0, 1800.0, 3000.0), gn=(1.0, 1.7, 0.6),
-                 gg=(1.0, 0.5, 1.9))
+                 gg=(1.0, 0.5, 1.9), delay=delay, spike_delay_ms=0.0 if delay else 1.0)
     steps = 600
-    print("case ei batch: device generator vs numpy restatement, 3 instances, %d steps" % steps,
-          flush=True)
+    print("case ei batch: device generator vs numpy restatement, 3 instances, delay %d, %d steps"
+          % (delay, steps), flush=True)
     net = ei_batch_network(spec)
     bits = _bits(net.n, steps, 5)
     rasters, ctrs = [], []
@@ -90,3 +91,38 @@ def test_generated_pair_count_is_bernoulli():
     mean, sd = trials * 0.1, np.sqrt(trials * 0.1 * 0.9)
     print("E pairs", nmda, "expected", mean, "+-", sd)
     assert abs(nmda - mean) < 5 * sd
+
+
+def test_synaptic_delay_form_equals_spike_delay_form():
+    """SpikeDly 1 ms with no synaptic delay (the reference form: the spike
+    flag follows the threshold crossing by 10 steps, code/EqsGnl2.cpp:229-251)
+    and SpikeDly 0 with a 10-step synaptic delay (the sweep's form, run by
+    the windowed path) give every neuron the same conductances: the
+    membrane potentials agree to rounding at every step (pushes that reach
+    one target in one step are added by float atomics in either order) and
+    the rasters differ by the 10-step shift of the reported spike times
+    (FP64, same input stream, the same gather_lanes on both paths)."""
+    sizes, seeds = [420, 700, 310], [41, 42, 43]
+    kw = dict(ext=(2600.0, 2300.0, 3300.0), gn=(1.3, 0.7, 1.6), gg=(0.8, 1.5, 1.1))
+    ref = _spec(sizes, seeds, **kw)
+    fold = _spec(sizes, seeds, delay=10, spike_delay_ms=0.0, **kw)
+    steps, chunk = 2000, 250
+    print("case ei batch: SpikeDly 1 ms vs 10-step synaptic delay, 3 instances, %d steps" % steps,
+          flush=True)
+    ea = Engine.ei_batch(ref, precision=A.FS_FP64)
+    eb = Engine.ei_batch(fold, precision=A.FS_FP64)
+    for e in (ea, eb):
+        e.set_schedule(gather_lanes=8, update_block=256)
+        e.reset(5)
+    assert ea.schedule["window"] == 0 and eb.schedule["window"] == 11
+    ra, rb, worst = [], [], 0.0
+    for _ in range(steps // chunk):
+        ra.append(ea.raster(chunk))
+        rb.append(eb.raster(chunk))
+        worst = max(worst, float(np.max(np.abs(ea.state("V") - eb.state("V")))))
+    ra, rb = np.concatenate(ra), np.concatenate(rb)
+    print("spikes %d / %d, max |dV| %.3g mV" % (len(ra), len(rb), worst))
+    assert len(ra) > 500
+    assert worst < 1e-9
+    shifted = rb[rb[:, 0] < steps - 10] + np.array([10, 0])
+    np.testing.assert_array_equal(ra[ra[:, 0] >= 10], shifted)

@@ -35,17 +35,30 @@ def _layer_slices(sizes):
     return out
 
 
-@pytest.mark.parametrize("rows", [1, 37, 256, 1000])
-def test_forward_matches_oracle(rows):
-    print("case mlp forward rows=%d" % rows, flush=True)
-    m = Mlp(SIZES, seed=3)
+# the c5 network and a smaller one run the fused whole-network kernel (every
+# padded width <= 256); a 320-wide layer takes the layer-by-layer GEMMs
+@pytest.mark.parametrize("sizes,rows", [(SIZES, 1), (SIZES, 37), (SIZES, 256), (SIZES, 1000),
+                                        (SIZES, 70001), ([20, 128, 64, 7], 999), ([30, 320, 10], 300)])
+def test_forward_matches_oracle(sizes, rows):
+    print("case mlp forward sizes=%s rows=%d" % (sizes, rows), flush=True)
+    m = Mlp(sizes, seed=3)
     p = m.params.astype(np.float64)
-    x, _ = _data(rows, rows)
+    rng = np.random.default_rng(rows)
+    x = rng.normal(size=(rows, sizes[0])).astype(np.float32)
     y = m.predict(x).astype(np.float64)
-    ref, _ = M.forward(p, SIZES, x, bf16=True)
+    ref, _ = M.forward(p, sizes, x, bf16=True)
     scale = np.max(np.abs(ref)) + 1e-12
-    assert np.max(np.abs(y - ref)) < 2e-3 * scale
-    ref64, _ = M.forward(p, SIZES, x)
+    dev = np.abs(y - ref) / scale
+    err = float(np.max(dev))
+    print("max |y - bf16 oracle| / scale = %.3g, 99.9th percentile %.3g (scale %.3g)"
+          % (err, np.quantile(dev, 0.999), scale))
+    # A hidden activation within 1e-7 of a bf16 rounding boundary can round
+    # the other way than in the oracle (float64 erf against the GPU's
+    # float erf): over millions of outputs the largest deviation grows
+    # with the row count, the bulk does not.
+    assert err < (2e-3 if rows <= 1000 else 3e-3)
+    assert np.quantile(dev, 0.999) < 1e-3
+    ref64, _ = M.forward(p, sizes, x)
     assert np.max(np.abs(y - ref64)) < 3e-2 * scale
 
 
@@ -101,6 +114,9 @@ def test_training_reduces_loss():
     assert losses[-1] < 0.1 * losses[0]
     y = m.predict(x[:512])
     assert np.mean((y - t[:512]) ** 2) < 2 * losses[-1] + 1e-3
+    # the trained weights reach the forward kernel's own copy (Adam writes it)
+    ref, _ = M.forward(m.params.astype(np.float64), SIZES, x[:512], bf16=True)
+    assert np.max(np.abs(y - ref)) < 2e-3 * (np.max(np.abs(ref)) + 1e-12)
 
 
 def test_predict_device_pointers_equal_host():

@@ -17,9 +17,9 @@ class FakeEngine:
     """Per-step cost = cost(schedule) x activity, where activity steps from 1
     to 3 once `ignite_at` steps have run; time is a virtual clock."""
 
-    def __init__(self, cost, ignite_at, stats=STATS):
+    def __init__(self, cost, ignite_at, stats=STATS, base=BASE):
         self.cost, self.ignite_at, self._stats = cost, ignite_at, dict(stats)
-        self._sched = dict(BASE)
+        self._sched = dict(base)
         self.clock, self.steps = 0.0, 0
 
     @property
@@ -30,7 +30,7 @@ class FakeEngine:
         return dict(self._stats)
 
     def set_schedule(self, **kw):
-        bad = set(kw) - set(BASE)
+        bad = set(kw) - set(self._sched)
         if bad:
             raise ValueError("unknown schedule keys %s" % bad)
         if kw.get("gather_lanes", 1) > 32:
@@ -115,3 +115,25 @@ def test_candidates_follow_the_network():
     assert {c["gather_lanes"] for c in reps} <= {1, 2}
     for c in delayed:
         assert c["gather_lanes"] * 2 <= c["update_block"] or not c["window"]
+
+
+def test_windowed_refinements_are_tried(clocked):
+    # the knobs only the windowed path on one network copy has are varied
+    # around the best first-pass schedules
+    base = dict(BASE, xcd_order=0, gather_chunk=0)
+    fast = lambda s: s.get("gather_chunk") == 128 and s.get("xcd_order") == 1
+    cost = lambda s: 1e-6 * (0.5 if fast(s) else 1.0) * (1.0 - 0.01 * s["gather_lanes"].bit_length())
+    eng = FakeEngine(cost, 10 ** 9, base=base)
+    clocked["eng"] = eng
+    cands = [dict(gather_lanes=4), dict(gather_lanes=16)]
+    monkey = tune.candidates
+    tune.candidates = lambda st: cands
+    try:
+        res = tune.tune(eng, cache=None)
+    finally:
+        tune.candidates = monkey
+    assert res["schedule"]["gather_chunk"] == 128 and res["schedule"]["xcd_order"] == 1
+    assert res["schedule"]["gather_lanes"] == 16
+    # replicas have no such variants
+    assert tune.refinements(base, dict(STATS, replicas=128)) == []
+    assert tune.refinements(dict(base, window=0), STATS) == []
