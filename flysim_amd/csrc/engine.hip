@@ -111,6 +111,28 @@ struct Engine final : EngineBase {
   // and at most what the block's LDS holds (window_lds_bytes <= 64 KiB)
   int32_t win_max = 1;
   int32_t mind = 0, maxdp = 0, maxdn = 0;  // smallest delay; largest linear / NMDA delay (steps)
+  size_t lds_max = 65536;  // dynamic LDS a block may take (the device's limit)
+  // The windowed kernel may take nearly the device's whole LDS per block
+  // (160 KiB on gfx950) when a window of many neurons needs it.
+  void init_lds_limit() {
+    int dev = 0, v = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    // the kernel's own static LDS (a few counters) comes out of the same limit
+    lds_max = std::max<size_t>(65536, (size_t)v - 1024);
+    const int bytes = (int)lds_max;
+#define FS_LDS(R, U, PR)                                                                          \
+  HIPCHECK(hipFuncSetAttribute((const void *)fsk::k_window<T, R, U, PR>,                         \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes))
+    FS_LDS(false, false, 0); FS_LDS(false, false, 1); FS_LDS(false, true, 0); FS_LDS(false, true, 1);
+    FS_LDS(true, false, 0); FS_LDS(true, false, 1); FS_LDS(true, true, 0); FS_LDS(true, true, 1);
+#undef FS_LDS
+  }
+  bool inj_any() const {
+    for (const auto &p : hpop)
+      if (p.act & FS_ACT_INJ) return true;
+    return false;
+  }
   int32_t window() const {
     if (sched.window <= 0) return 0;
     const int npb = std::max(1, sched.update_block / sched.gather_lanes);
@@ -118,8 +140,9 @@ struct Engine final : EngineBase {
     int32_t w = std::min(sched.window, win_max);
     if (B == 1) w = std::min(w, fsk::FLAT_WMAX);  // gather_block's partial sums per thread
     const int gc = sched.gather_chunk;
-    while (w > 1 && fsk::window_lds_bytes<T>(npb, w, nlu, B == 1, gc) > (size_t)65536) --w;
-    if (w >= 1 && fsk::window_lds_bytes<T>(npb, w, nlu, B == 1, gc) > (size_t)65536) return 0;
+    const bool inj = inj_any();
+    while (w > 1 && fsk::window_lds_bytes<T>(npb, w, nlu, B == 1, gc, inj) > lds_max) --w;
+    if (w >= 1 && fsk::window_lds_bytes<T>(npb, w, nlu, B == 1, gc, inj) > lds_max) return 0;
     return w;
   }
   bool fused_active() const override { return fused(); }
@@ -308,6 +331,7 @@ struct Engine final : EngineBase {
     N = N0 * B;
     n_pop = P * B;
     HIPCHECK(hipStreamCreate(&stream));
+    init_lds_limit();
     for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
     hpop.assign(d.pops, d.pops + P);
     for (int32_t i = 0; i < N0; ++i)
@@ -532,6 +556,7 @@ struct Engine final : EngineBase {
     precision = d.precision;
     record = d.record_spikes != 0;
     HIPCHECK(hipStreamCreate(&stream));
+    init_lds_limit();
     for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
     std::vector<fsg::GenInst> gi(K);
     std::vector<uint32_t> npop(N);
@@ -806,7 +831,10 @@ struct Engine final : EngineBase {
     sched.update_block = 256;
     sched.gather_lanes = 8;
     sched.push_block = 256;
-    sched.steps_per_graph = 100;
+    // steps per graph: 100, or fewer where the spike record of the
+    // one-step-per-launch paths (every neuron every step at worst) would
+    // pass 2^26 entries
+    sched.steps_per_graph = (int32_t)std::max<int64_t>(1, std::min<int64_t>(100, ((int64_t)1 << 26) / N));
     sched.use_graph = 1;
     sched.fused = 1;
     // the windowed path when synapses have delays (several steps per
@@ -814,11 +842,13 @@ struct Engine final : EngineBase {
     sched.window = win_max >= 2 ? 64 : 0;
     if (sched.window) {
       // up to 100 windows per graph: each batch ends in a host round trip
-      // (overflow check, spike readback), so long batches amortise it; the
-      // spike record (worst case every neuron every step) stays within 2^26
-      // entries, down to one window per graph for the largest networks
+      // (overflow check, spike readback), so long batches amortise it; with
+      // spike recording the record (worst case every neuron every step)
+      // stays within 2^26 entries, down to one window per graph for the
+      // largest networks
       const int64_t w = std::min(sched.window, win_max);
-      const int64_t per = std::max<int64_t>(1, std::min<int64_t>(100, ((int64_t)1 << 26) / ((int64_t)N * w)));
+      const int64_t per =
+          record ? std::max<int64_t>(1, std::min<int64_t>(100, ((int64_t)1 << 26) / ((int64_t)N * w))) : 100;
       sched.steps_per_graph = (int32_t)(per * w);
     }
     apply_schedule();
@@ -856,14 +886,19 @@ struct Engine final : EngineBase {
       const int64_t nblk = ((int64_t)N + npb - 1) / npb;
       for (int64_t b = 0; b < nblk; ++b) cov[b % fsk::NSUB] += std::min<int64_t>(npb, N - b * npb);
     }
-    uint64_t cap = (uint64_t)N * (uint64_t)K;
-    const uint64_t cap_max = ((uint64_t)1 << 31) - 1;
+    // The windowed path writes the record only for readback, so a handle
+    // created without spike recording needs none there; the other paths
+    // push from it (it is their per-step spike list).
+    const bool rec_needed = record || !window();
+    uint64_t cap = rec_needed ? (uint64_t)N * (uint64_t)K : 0;
+    if (cap > ((uint64_t)1 << 27))
+      throw std::invalid_argument("steps_per_graph too large for the spike record of this network "
+                                  "(neurons x steps_per_graph > 2^27)");
     hsub_off.assign(fsk::NSUB, 0u);
     hsub_cap.assign(fsk::NSUB, 0u);
     uint64_t acc = 0;
     for (int s = 0; s < fsk::NSUB; ++s) {
-      uint64_t c = cov[s] * (uint64_t)K;
-      if (cap > cap_max) c = c * cap_max / cap;  // beyond 2^31 entries: overflow is detected
+      const uint64_t c = rec_needed ? cov[s] * (uint64_t)K : 0;
       hsub_off[s] = (uint32_t)acc;
       hsub_cap[s] = (uint32_t)c;
       acc += c;
@@ -929,7 +964,7 @@ struct Engine final : EngineBase {
       for (int64_t a = 0; a < N; a += npb)
         runs.push_back((uint64_t)a | ((uint64_t)std::min<int64_t>(npb, N - a) << 32));
     }
-    if (sched.xcd_order && B == 1 && runs.size() >= 16) {
+    if (sched.xcd_order && runs.size() >= 16) {
       // block i runs on XCD i % 8: give XCD x the x-th eighth of the pieces
       // in order, so the blocks one XCD runs at a time are neighbours
       const size_t nbk = runs.size(), qx = nbk / 8, rx = nbk % 8;
@@ -1038,9 +1073,10 @@ struct Engine final : EngineBase {
   void launch_window(int32_t k0, int32_t w, bool timed) {
     if (timed) HIPCHECK(hipEventRecord(ev[0], stream));
     const dim3 g((unsigned)hwblk.size()), b(sched.update_block);
+    net.inj_on = inj_any() ? 1 : 0;
     const size_t lds = fsk::window_lds_bytes<T>(sched.update_block / sched.gather_lanes, w,
                                                 __builtin_popcount((uint32_t)net.lin_mask), B == 1,
-                                                sched.gather_chunk);
+                                                sched.gather_chunk, net.inj_on != 0);
     // replicas in whole waves with one lane per row: the row tables are wave-uniform
     const bool uni = (B % 64) == 0 && sched.gather_lanes == 1;
     // the AMPA + GABA (+ NMDA) layout with external AMPA drive and no STP
@@ -1169,7 +1205,10 @@ struct Engine final : EngineBase {
     dpop[i] = to_dev(p, dpop[i]);
     HIPCHECK(hipMemcpyAsync(b_pops.as<fsk::DevPop<T>>() + i, &dpop[i], sizeof(fsk::DevPop<T>),
                             hipMemcpyHostToDevice, stream));
-    if (inj_switch) zig_dirty = true;  // refreshed once before the next step (events come in bulk)
+    if (inj_switch) {
+      zig_dirty = true;    // refreshed once before the next step (events come in bulk)
+      graph_dirty = true;  // the windowed kernel's LDS holds injection draws only while some is on
+    }
     pbr_from_pop(i, rep, p);
     for (int r = 0; r < 5; ++r) note_pb(r, pbr(r, i, rep));
     tables_dirty = true;
@@ -1233,6 +1272,8 @@ struct Engine final : EngineBase {
     if (off) off[0] = 0;
     // the windowed kernel writes the spike record only when it is read back
     const int32_t rec_on = (off || ids) ? 1 : 0;
+    if (rec_on && !record)
+      throw std::invalid_argument("spike readback needs a handle created with record_spikes = 1");
     if (net.rec_on != rec_on) {
       net.rec_on = rec_on;
       graph_dirty = true;  // a kernel argument

@@ -171,6 +171,7 @@ struct Net {
                             // of gather CSR entries
   int32_t gather_chunk;     // windowed path, one copy: cap on the entries per staged chunk (0: none)
   int32_t gather_split;     // windowed path, one copy: sub-lanes per (row, step) pair of the gather
+  int32_t inj_on;           // some population has Gaussian injection on (windowed path's LDS)
   int32_t push_waves;       // waves of k_deliver dedicated to the push
   int32_t push_blocks;      // fused step: leading blocks that push; the rest gather + update
 };
@@ -1281,10 +1282,11 @@ __host__ __device__ constexpr int window_chunk(int W, int cap) {
 // phases A and B, then a region phase A uses for the gather's staging and
 // phases B and C for the spike staging.
 template <typename T>
-__host__ __device__ constexpr size_t window_lds_bytes(int npb, int W, int nlu, bool flat, int chunk_cap) {
+__host__ __device__ constexpr size_t window_lds_bytes(int npb, int W, int nlu, bool flat, int chunk_cap,
+                                                     bool inj) {
   const size_t nst = (size_t)((npb + 63) / 64) * 64 * W;
-  const size_t fixed = ((size_t)8 * npb + sizeof(T) * ((size_t)W * npb * (1 + nlu)) +
-                        (size_t)4 * ((size_t)npb + 2 * (size_t)W * npb) + 7) & ~(size_t)7;
+  const size_t fixed = ((size_t)8 * npb + sizeof(T) * ((size_t)W * npb * (1 + nlu)) + (size_t)4 * npb +
+                        (inj ? (size_t)4 * W * npb : 0) + (size_t)W * npb + 7) & ~(size_t)7;
   const size_t bc = sizeof(T) * ((size_t)W * npb + nst) + (size_t)4 * nst;
   const int ch = window_chunk<T>(W, chunk_cap);
   const size_t stage = flat ? (size_t)8 * (npb + 1) + sizeof(T) * (size_t)ch * W + (size_t)8 * ch : 0;
@@ -1346,9 +1348,10 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   T *s_gn = (T *)(s_prow + npb);                              // [W][npb] NMDA sums
   T *s_inc = s_gn + (size_t)W * npb;                          // [nlu][W][npb] ring slots
   uint32_t *s_pdeg = (uint32_t *)(s_inc + (size_t)nlu * W * npb);  // [npb] push row length
-  uint32_t *s_draw = s_pdeg + npb;                            // [W][npb] event outcomes
-  float *s_inj = (float *)(s_draw + (size_t)W * npb);         // [W][npb] injection draws
-  unsigned char *s_u = (unsigned char *)(s_inj + (size_t)W * npb);
+  const bool inj = net.inj_on != 0;
+  float *s_inj = (float *)(s_pdeg + npb);                     // [W][npb] injection draws (inj only)
+  uint8_t *s_draw = (uint8_t *)(s_inj + (inj ? (size_t)W * npb : 0));  // [W][npb] event outcomes
+  unsigned char *s_u = (unsigned char *)(s_draw + (size_t)W * npb);
   s_u += (uintptr_t)(-(intptr_t)s_u) & 7u;                    // 8-byte aligned shared region
   // phases B and C
   T *s_hs = (T *)s_u;                                         // [W][npb] NMDA gates of each step
@@ -1438,8 +1441,8 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
       uint32_t db;
       float iv;
       neuron_draws<T, REPLAY>(net, Pr, Cr, nr, tb + k0 + s, db, iv);
-      s_draw[s * npb + ir] = db;
-      s_inj[s * npb + ir] = iv;
+      s_draw[s * npb + ir] = (uint8_t)db;
+      if (inj) s_inj[s * npb + ir] = iv;
     }
   }
   lds_barrier();
@@ -1478,7 +1481,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
         }
         x.gn = net.has_nmda_syn ? s_gn[s * npb + ul] : T(0);
         x.db = s_draw[s * npb + ul];
-        x.iv = s_inj[s * npb + ul];
+        x.iv = inj ? s_inj[s * npb + ul] : 0.0f;
       };
       In cur;
       if (here) fetch(0, cur);

@@ -72,7 +72,7 @@ def gen(points: int, seed: int, replicas: int, warm: int = WARM_STEPS, meas: int
 
 
 def train(X: np.ndarray, Y: np.ndarray, epochs: int = 60, batch: int = 512, lr: float = 1e-3,
-          holdout: float = 0.1, seed: int = 0) -> Dict[str, np.ndarray]:
+          holdout: float = 0.1, seed: int = 0, decay=(1.0, 0.3, 0.1)) -> Dict[str, np.ndarray]:
     n = X.shape[0]
     rng = np.random.default_rng(seed)
     idx = rng.permutation(n)
@@ -80,7 +80,7 @@ def train(X: np.ndarray, Y: np.ndarray, epochs: int = 60, batch: int = 512, lr: 
     val, tr = idx[:n_val], idx[n_val:]
     sur = RateSurrogate(X.shape[1], Y.shape[1], seed=seed)
     t0 = time.perf_counter()
-    losses = sur.fit(X[tr], Y[tr], epochs=epochs, batch=min(batch, len(tr)), lr=lr, seed=seed)
+    losses = sur.fit(X[tr], Y[tr], epochs=epochs, batch=min(batch, len(tr)), lr=lr, seed=seed, decay=decay)
     t_fit = time.perf_counter() - t0
     out = {"params": sur.mlp.params, "mu": sur.mu, "sd": sur.sd, "losses": losses,
            "fit_s": np.float64(t_fit), "val_idx": val}
@@ -373,6 +373,8 @@ def main(argv=None):
     t.add_argument("--epochs", type=int, default=60)
     t.add_argument("--batch", type=int, default=512)
     t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--decay", default="1,0.3,0.1",
+                   help="learning-rate factors, equal shares of the epochs each")
     t.add_argument("--out", required=True)
     s = sub.add_parser("screen")
     s.add_argument("--model", required=True)
@@ -423,7 +425,8 @@ def main(argv=None):
         json.dump(meta, open(a.out[:-4] + ".json" if a.out.endswith(".npz") else a.out + ".json", "w"))
     elif a.cmd == "train":
         d = np.load(a.data)
-        m = train(d["X"], d["Y"], epochs=a.epochs, batch=a.batch, lr=a.lr)
+        m = train(d["X"], d["Y"], epochs=a.epochs, batch=a.batch, lr=a.lr,
+                  decay=tuple(float(v) for v in a.decay.split(",")))
         log("trained: final loss %.4g, holdout MAE %.2f Hz, fit %.1f s"
             % (m["losses"][-1], m.get("val_mae_hz", np.nan), m["fit_s"]))
         np.savez(a.out, **m)

@@ -108,7 +108,9 @@ typedef struct fs_net_desc {
   float dt;                       /* ms */
   int32_t precision;              /* FS_FP32 | FS_FP64 */
   int32_t ext_mode;               /* FS_EXT_PHILOX | FS_EXT_REPLAY */
-  int32_t record_spikes;          /* 1: keep per-step spike lists for readback */
+  int32_t record_spikes;          /* 1: keep per-step spike lists for readback; 0: fs_run
+                                     takes no spike buffers (FS_ERR_ARG), and the windowed
+                                     path keeps no spike record at all */
 } fs_net_desc;
 
 typedef struct fs_handle fs_handle;
@@ -240,8 +242,8 @@ int fs_pop_spike_counts(fs_handle *h, uint64_t *out);
  * the smallest synaptic delay in steps, and to 64).  The windowed path takes
  * precedence over the fused one.  fs_get_schedule reports the path in
  * effect: fused = 1 only while the fused path runs, window = the steps per
- * launch actually used (0 when off).  xcd_order (windowed path, one network
- * copy): 1 hands consecutive blocks of neurons to one XCD (the chip's
+ * launch actually used (0 when off).  xcd_order (windowed path): 1 hands
+ * consecutive blocks of neurons to one XCD (the chip's
  * workgroup dispatcher deals blocks to its 8 XCDs in turn), so the blocks
  * an XCD runs together share presynaptic neurons in its L2.  gather_chunk
  * (windowed path, one network copy): at most this many NMDA entries per
