@@ -868,6 +868,13 @@ struct Engine final : EngineBase {
     if (sched.gather_chunk < 0 || (sched.gather_chunk > 0 && sched.gather_chunk < 64))
       throw std::invalid_argument("gather_chunk must be 0 or >= 64");
     if (sched.xcd_order < 0 || sched.xcd_order > 1) throw std::invalid_argument("xcd_order must be 0 or 1");
+    // The windowed path writes the record only for readback, so a handle
+    // created without spike recording needs none there; the other paths
+    // push from it (it is their per-step spike list).
+    const bool rec_needed = record || !window();
+    if (rec_needed && (uint64_t)N * (uint64_t)sched.steps_per_graph > ((uint64_t)1 << 27))
+      throw std::invalid_argument("steps_per_graph too large for the spike record of this network "
+                                  "(neurons x steps_per_graph > 2^27)");
     if (sched.gather_split < 0 || sched.gather_split > L || (sched.gather_split & (sched.gather_split - 1)))
       throw std::invalid_argument("gather_split must be 0 or a power of two <= gather_lanes");
     net.gather_lanes = L;
@@ -886,14 +893,6 @@ struct Engine final : EngineBase {
       const int64_t nblk = ((int64_t)N + npb - 1) / npb;
       for (int64_t b = 0; b < nblk; ++b) cov[b % fsk::NSUB] += std::min<int64_t>(npb, N - b * npb);
     }
-    // The windowed path writes the record only for readback, so a handle
-    // created without spike recording needs none there; the other paths
-    // push from it (it is their per-step spike list).
-    const bool rec_needed = record || !window();
-    uint64_t cap = rec_needed ? (uint64_t)N * (uint64_t)K : 0;
-    if (cap > ((uint64_t)1 << 27))
-      throw std::invalid_argument("steps_per_graph too large for the spike record of this network "
-                                  "(neurons x steps_per_graph > 2^27)");
     hsub_off.assign(fsk::NSUB, 0u);
     hsub_cap.assign(fsk::NSUB, 0u);
     uint64_t acc = 0;
@@ -1674,7 +1673,11 @@ int fs_set_schedule(fs_handle *h, const fs_schedule *s) {
       if (auto *e32 = dynamic_cast<Engine<float> *>(h->e.get())) e32->apply_schedule();
       else if (auto *e64 = dynamic_cast<Engine<double> *>(h->e.get())) e64->apply_schedule();
     } catch (...) {
+      // back to the old schedule, re-applied: a failure part-way may have
+      // rebuilt tables (window blocks) for the rejected one
       h->e->sched = old;
+      if (auto *e32 = dynamic_cast<Engine<float> *>(h->e.get())) e32->apply_schedule();
+      else if (auto *e64 = dynamic_cast<Engine<double> *>(h->e.get())) e64->apply_schedule();
       throw;
     }
   })
