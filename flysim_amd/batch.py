@@ -184,11 +184,13 @@ def main(argv=None):
                     help="instances also run one by one (0: skip)")
     ap.add_argument("--lanes", type=int, default=0,
                     help="NMDA gather lanes per row (0: autotuned schedule)")
+    ap.add_argument("--schedule", default="",
+                    help='fs_schedule fields as JSON, e.g. \'{"gather_lanes": 32}\' (instead of the autotuner)')
     ap.add_argument("--out", default="")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     spec = c4_spec(n_inst=a.instances)
-    sched = {"gather_lanes": a.lanes} if a.lanes else "auto"
+    sched = json.loads(a.schedule) if a.schedule else ({"gather_lanes": a.lanes} if a.lanes else "auto")
     n = spec.n_neuron.astype(np.float64)
     ne = spec.n_exc().astype(np.float64)
     pairs_expect = float(np.sum(n * (n - 1)) * spec.p_conn)

@@ -12,6 +12,7 @@ Stages (each a command; files are numpy .npz):
           predicted near the threshold plus a 1 % audit, classify the rest by
           the surrogate
   moderr  per-module error of the surrogate against repeated simulator runs
+  predict GPU time of the surrogate's forward over device-resident points
   report  merge the screen slices: hyperactive points found, surrogate error
           against the simulator, points/s of plain per-instance GPU simulation,
           batched simulation and batched + surrogate
@@ -250,6 +251,25 @@ def module_error(model: Dict[str, np.ndarray], points: int, seed: int, replicas:
     return out
 
 
+def predict_rate(points: int, repeats: int = 5) -> Dict:
+    """GPU time of the surrogate's forward over `points` sweep points already
+    in device memory (an untrained network of the sweep's shape): best of
+    `repeats` runs, and the matrix-core rate it reaches."""
+    from ._capi import DeviceBuffer
+    from .surrogate import Mlp
+    space = SweepSpace()
+    sizes = [space.dim, 256, 256, 256, space.n_mod]
+    mlp = Mlp(sizes, seed=0)
+    x = np.ascontiguousarray(space.sample(points, np.random.default_rng(7)), dtype=np.float32)
+    xd, yd = DeviceBuffer(x.nbytes), DeviceBuffer(points * space.n_mod * 4)
+    xd.upload(x)
+    mlp.predict_device(xd.ptr, min(points, 4096), yd.ptr)
+    ms = min(mlp.predict_device(xd.ptr, points, yd.ptr) for _ in range(repeats))
+    flop = 2.0 * sum(sizes[i] * sizes[i + 1] for i in range(len(sizes) - 1)) * points
+    return {"points": points, "gpu_ms": ms, "points_per_s": points / (ms * 1e-3),
+            "tflops": flop / (ms * 1e-3) / 1e12}
+
+
 def plain_rate(points: int, seed: int) -> float:
     """Points/s of plain per-instance GPU simulation (one network per run)."""
     X = SweepSpace().sample(points, np.random.default_rng(seed))
@@ -406,6 +426,9 @@ def main(argv=None):
     me.add_argument("--replicas", type=int, default=128)
     me.add_argument("--repeats", type=int, default=8)
     me.add_argument("--out", required=True)
+    pr = sub.add_parser("predict")
+    pr.add_argument("--points", type=int, default=1 << 20)
+    pr.add_argument("--repeats", type=int, default=5)
     pl = sub.add_parser("plain")
     pl.add_argument("--points", type=int, default=64)
     pl.add_argument("--out", required=True)
@@ -448,6 +471,8 @@ def main(argv=None):
         print(txt)
         if a.out:
             open(a.out, "w").write(txt)
+    elif a.cmd == "predict":
+        print(json.dumps(predict_rate(a.points, a.repeats)))
     elif a.cmd == "plain":
         rate = plain_rate(a.points, 99)
         log("plain per-instance simulation: %.2f points/s" % rate)

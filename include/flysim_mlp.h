@@ -40,17 +40,21 @@ int fs_mlp_loss_grad(fs_mlp *m, const float *x, const float *y, int32_t rows, fl
 
 /* y[n][out] = network(x[n][in]); device_ptrs = 1: x and y are device
  * pointers (inputs resident in HBM).  *ms (optional) = GPU time of the
- * packing + forward kernels measured with HIP events. */
+ * forward kernels measured with HIP events.  Networks whose padded widths
+ * (multiples of 64) are all <= 256, at most 8 layers, run one fused kernel
+ * for every layer; others one GEMM per layer. */
 int fs_mlp_predict(fs_mlp *m, const float *x, int64_t n, float *y, int32_t device_ptrs, double *ms);
 
 /* Allocation-free forward.  No counterpart in the reference; for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  x (f32 [n][in]) and y
- * (f32 [n][out]) are device pointers; the packed input and the hidden
- * activations of each chunk of rows live in one fixed 8 MiB device array per
- * GPU (resolved at fs_mlp_create), so the call allocates nothing.  Any n is
- * processed in chunks of that array's size (a multiple of 64 rows; the
- * widths fs_mlp_create admits always fit one chunk).  Output equals
+ * (f32 [n][out]) are device pointers.  Networks that fs_mlp_predict runs
+ * with its fused kernel run that kernel here too, which keeps every
+ * intermediate in registers and LDS; for the others the packed input and the
+ * hidden activations of each chunk of rows live in one fixed 8 MiB device
+ * array per GPU (resolved at fs_mlp_create), so the call allocates nothing.
+ * Any n is processed in chunks of that array's size (a multiple of 64 rows;
+ * the widths fs_mlp_create admits always fit one chunk).  Output equals
  * fs_mlp_predict's bit for bit.  Every call on one GPU shares the array and
  * leaves its contents behind: calls that overlap on one GPU (other threads or
  * other handles) race.  Returns after the GPU has finished. */
