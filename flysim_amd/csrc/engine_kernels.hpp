@@ -1376,41 +1376,44 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   const int uw = ul >> 6;                                            // update wave ordinal
   const bool act = ul < nbc;
   const int32_t n = act ? nb + ul : N;
-  int32_t nl = 0, rep = 0;
-  uint32_t p0 = 0;
+  // Every lane loads (a lane without a neuron loads the block's first
+  // neuron's values and never uses them): a load under `if (act)` would
+  // leave the other lanes' registers to be merged with defaults afterwards,
+  // and that merge waits for the load before the gather can start.  For the
+  // same reason the push rows reach LDS only after the gather.
+  const int32_t nld = act ? n : nb;
+  int32_t nl, rep;
+  uint32_t p0;
   NState<T> S;
   NConst<T> C;
-  if (act) {
-    locate<false>(net.npop, B, n, nl, rep, p0);
-    load_const(net, *(PopC<T> *)(net.pops + p0), nl, rep, p0, C);
-    load_state(net, nl, rep, n, slot_of(tm1, k0 - 1, net.DN), S);
-    const uint64_t r0 = ldc(net.rp_p + nl);
-    s_prow[ul] = r0;
-    s_pdeg[ul] = (uint32_t)(ldc(net.rp_p + nl + 1) - r0);
-  }
+  locate<false>(net.npop, B, nld, nl, rep, p0);
+  load_const(net, *(PopC<T> *)(net.pops + p0), nl, rep, p0, C);
+  load_state(net, nl, rep, nld, slot_of(tm1, k0 - 1, net.DN), S);
+  const uint64_t prow0 = ldc(net.rp_p + nl), prow1 = ldc(net.rp_p + nl + 1);
 
   // ---- A: prefetch the window's inputs.  Thread -> (neuron ir, steps s0,
   // s0 + L, ...) for the ring slots and the draws (blockDim = npb * L): their
   // loads are issued first and consumed after the NMDA gather, whose latency
   // they share.
   const int ir = tid % npb, s0 = tid / npb;
-  const int32_t nr = nb + ir;
   const bool rl = ir < nbc;  // this thread's neuron of the ring / draw work is in the block
+  const int32_t nr = nb + (rl ? ir : 0);  // (loads unconditional, as above)
   constexpr int RQ = 3;      // ring slots per receptor held in registers across the gather
   T rv[4][RQ];
   NConst<T> Cr;
-  uint32_t pr = 0;
-  if (rl) {
+  uint32_t pr;
 #pragma unroll
-    for (int li = 0; li < 4; ++li) {
+  for (int li = 0; li < 4; ++li) {
 #pragma unroll
-      for (int j = 0; j < RQ; ++j) {
-        const int32_t s = s0 + j * L;
-        rv[li][j] = ((net.lin_mask & (1 << li)) && s < W)
-                        ? net.ring[((size_t)li * net.D + slot_of(tm0, k0 + s - 1, net.D)) * (size_t)N + nr]
-                        : T(0);
-      }
+    for (int j = 0; j < RQ; ++j) {
+      // slots past the window read the step before it (a valid slot, unused)
+      const int32_t s = s0 + j * L < W ? s0 + j * L : 0;
+      rv[li][j] = (net.lin_mask & (1 << li))
+                      ? net.ring[((size_t)li * net.D + slot_of(tm0, k0 + s - 1, net.D)) * (size_t)N + nr]
+                      : T(0);
     }
+  }
+  {
     int32_t nlr, repr;
     locate<false>(net.npop, B, nr, nlr, repr, pr);
     load_const(net, *(PopC<T> *)(net.pops + pr), nlr, repr, pr, Cr);
@@ -1422,6 +1425,10 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
     else
       gather_window<T, UNI>(net, tid / L < nbc ? (int64_t)nb + tid / L : (int64_t)N,
                             slot_of(tm1, k0 - 1, net.DN), W, L, tid % L, s_gn, npb, tid / L);
+  }
+  if (act) {
+    s_prow[ul] = prow0;
+    s_pdeg[ul] = (uint32_t)(prow1 - prow0);
   }
   if (rl) {
     int c = 0;
