@@ -121,11 +121,13 @@ struct Engine final : EngineBase {
     // the kernel's own static LDS (a few counters) comes out of the same limit
     lds_max = std::max<size_t>(65536, (size_t)v - 1024);
     const int bytes = (int)lds_max;
-#define FS_LDS(R, U, PR)                                                                          \
-  HIPCHECK(hipFuncSetAttribute((const void *)fsk::k_window<T, R, U, PR>,                         \
+#define FS_LDS(R, U, PR, DUP)                                                                     \
+  HIPCHECK(hipFuncSetAttribute((const void *)fsk::k_window<T, R, U, PR, DUP>,                    \
                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes))
-    FS_LDS(false, false, 0); FS_LDS(false, false, 1); FS_LDS(false, true, 0); FS_LDS(false, true, 1);
-    FS_LDS(true, false, 0); FS_LDS(true, false, 1); FS_LDS(true, true, 0); FS_LDS(true, true, 1);
+    FS_LDS(false, false, 0, false); FS_LDS(false, false, 1, false); FS_LDS(true, false, 0, false);
+    FS_LDS(true, false, 1, false); FS_LDS(false, false, 0, true); FS_LDS(false, false, 1, true);
+    FS_LDS(true, false, 0, true); FS_LDS(true, false, 1, true); FS_LDS(false, true, 0, true);
+    FS_LDS(false, true, 1, true); FS_LDS(true, true, 0, true); FS_LDS(true, true, 1, true);
 #undef FS_LDS
   }
   bool inj_any() const {
@@ -1080,17 +1082,22 @@ struct Engine final : EngineBase {
     const bool uni = (B % 64) == 0 && sched.gather_lanes == 1;
     // the AMPA + GABA (+ NMDA) layout with external AMPA drive and no STP
     const bool prof = net.lin_mask == 3 && net.ext_mask == 1 && net.stp_mode == 0;
-#define FS_WIN(R, U)                                                                      \
-  do {                                                                                    \
-    if (prof) hipLaunchKernelGGL((fsk::k_window<T, R, U, 1>), g, b, lds, stream, net, k0, w); \
-    else hipLaunchKernelGGL((fsk::k_window<T, R, U, 0>), g, b, lds, stream, net, k0, w);      \
+    // every wave issues the prologue's loads when that duplicates them at
+    // most 8 times (the replicas' instance, one lane per row, always)
+    const bool dup = sched.gather_lanes <= 8;
+#define FS_WIN(R, U, DUP)                                                                          \
+  do {                                                                                             \
+    if (prof) hipLaunchKernelGGL((fsk::k_window<T, R, U, 1, DUP>), g, b, lds, stream, net, k0, w);   \
+    else hipLaunchKernelGGL((fsk::k_window<T, R, U, 0, DUP>), g, b, lds, stream, net, k0, w);        \
   } while (0)
     if (net.ext_mode == FS_EXT_REPLAY) {
-      if (uni) FS_WIN(true, true);
-      else FS_WIN(true, false);
+      if (uni) FS_WIN(true, true, true);
+      else if (dup) FS_WIN(true, false, true);
+      else FS_WIN(true, false, false);
     } else {
-      if (uni) FS_WIN(false, true);
-      else FS_WIN(false, false);
+      if (uni) FS_WIN(false, true, true);
+      else if (dup) FS_WIN(false, false, true);
+      else FS_WIN(false, false, false);
     }
 #undef FS_WIN
     HIPCHECK(hipGetLastError());

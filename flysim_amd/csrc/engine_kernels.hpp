@@ -1323,7 +1323,13 @@ __host__ __device__ constexpr size_t window_lds_bytes(int npb, int W, int nlu, b
 // external drive on AMPA only (ext_mask 1) and no short-term plasticity; the
 // instance folds those fields to constants, dropping the other receptors'
 // code from the update loop.  PROF 0: any layout.
-template <typename T, bool REPLAY, bool UNI, int PROF>
+// DUP: every wave issues the prologue's loads (waves without a neuron of
+// their own load the block's first neuron's values, unused) with no branch
+// around them, for blocks of few waves per neuron group (gather_lanes <= 8);
+// else only the waves that use them, behind a wave-uniform branch (which
+// the compiler schedules less well, but which spares blocks of many lanes
+// per neuron dozens of duplicate loads).
+template <typename T, bool REPLAY, bool UNI, int PROF, bool DUP>
 __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t W) {
   Net<T> net = net_;
   if constexpr (PROF == 1) {
@@ -1376,20 +1382,28 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   const int uw = ul >> 6;                                            // update wave ordinal
   const bool act = ul < nbc;
   const int32_t n = act ? nb + ul : N;
-  // Every lane loads (a lane without a neuron loads the block's first
-  // neuron's values and never uses them): a load under `if (act)` would
-  // leave the other lanes' registers to be merged with defaults afterwards,
-  // and that merge waits for the load before the gather can start.  For the
-  // same reason the push rows reach LDS only after the gather.
+  // Every lane of a wave that holds update lanes (with DUP: of every wave)
+  // loads (a lane without a neuron loads the block's first neuron's values
+  // and never uses them): a
+  // load under a per-lane condition would leave the other lanes' registers
+  // to be merged with defaults afterwards, and that merge waits for the
+  // load before the gather can start; a wave-uniform branch merges nothing.
+  // For the same reason the push rows reach LDS only after the gather.
+  const bool uwave = DUP || uw * 64 < npb;  // wave-uniform
   const int32_t nld = act ? n : nb;
   int32_t nl, rep;
   uint32_t p0;
   NState<T> S;
   NConst<T> C;
-  locate<false>(net.npop, B, nld, nl, rep, p0);
-  load_const(net, *(PopC<T> *)(net.pops + p0), nl, rep, p0, C);
-  load_state(net, nl, rep, nld, slot_of(tm1, k0 - 1, net.DN), S);
-  const uint64_t prow0 = ldc(net.rp_p + nl), prow1 = ldc(net.rp_p + nl + 1);
+  uint64_t prow0, prow1;
+  if (uwave) {
+    locate<false>(net.npop, B, nld, nl, rep, p0);
+    load_const(net, *(PopC<T> *)(net.pops + p0), nl, rep, p0, C);
+    load_state(net, nl, rep, nld, slot_of(tm1, k0 - 1, net.DN), S);
+    prow0 = ldc(net.rp_p + nl);
+    prow1 = ldc(net.rp_p + nl + 1);
+  }  // (no defaults for the other waves: they never read these, and a
+     // default would be merged with the loaded values, and so wait for them)
 
   // ---- A: prefetch the window's inputs.  Thread -> (neuron ir, steps s0,
   // s0 + L, ...) for the ring slots and the draws (blockDim = npb * L): their
@@ -1397,27 +1411,29 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   // they share.
   const int ir = tid % npb, s0 = tid / npb;
   const bool rl = ir < nbc;  // this thread's neuron of the ring / draw work is in the block
-  const int32_t nr = nb + (rl ? ir : 0);  // (loads unconditional, as above)
+  const int32_t nr = nb + (rl ? ir : 0);  // (loads by whole waves, as above)
+  // the wave's first thread has a step in the window (wave-uniform)
+  const bool rwave = DUP || (int)(((unsigned)tid & ~63u) / (unsigned)npb) < W;
   constexpr int RQ = 3;      // ring slots per receptor held in registers across the gather
   T rv[4][RQ];
   NConst<T> Cr;
   uint32_t pr;
+  if (rwave) {
 #pragma unroll
-  for (int li = 0; li < 4; ++li) {
+    for (int li = 0; li < 4; ++li) {
 #pragma unroll
-    for (int j = 0; j < RQ; ++j) {
-      // slots past the window read the step before it (a valid slot, unused)
-      const int32_t s = s0 + j * L < W ? s0 + j * L : 0;
-      rv[li][j] = (net.lin_mask & (1 << li))
-                      ? net.ring[((size_t)li * net.D + slot_of(tm0, k0 + s - 1, net.D)) * (size_t)N + nr]
-                      : T(0);
+      for (int j = 0; j < RQ; ++j) {
+        // slots past the window read the step before it (a valid slot, unused)
+        const int32_t s = s0 + j * L < W ? s0 + j * L : 0;
+        rv[li][j] = (net.lin_mask & (1 << li))
+                        ? net.ring[((size_t)li * net.D + slot_of(tm0, k0 + s - 1, net.D)) * (size_t)N + nr]
+                        : T(0);
+      }
     }
-  }
-  {
     int32_t nlr, repr;
     locate<false>(net.npop, B, nr, nlr, repr, pr);
     load_const(net, *(PopC<T> *)(net.pops + pr), nlr, repr, pr, Cr);
-  }
+  }  // (a wave with no step of the window stages no ring slot or draw)
   if (net.has_nmda_syn) {
     if (flat)
       gather_block<T>(net, nb, nbc, ldc(net.wgat + 2 * blockIdx.x), ldc(net.wgat + 2 * blockIdx.x + 1),
@@ -1443,11 +1459,10 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
       }
       ++c;
     }
-    PopC<T> &Pr = *(PopC<T> *)(net.pops + pr);
     for (int32_t s = s0; s < W; s += L) {
       uint32_t db;
       float iv;
-      neuron_draws<T, REPLAY>(net, Pr, Cr, nr, tb + k0 + s, db, iv);
+      neuron_draws<T, REPLAY>(net, *(PopC<T> *)(net.pops + pr), Cr, nr, tb + k0 + s, db, iv);
       s_draw[s * npb + ir] = (uint8_t)db;
       if (inj) s_inj[s * npb + ir] = iv;
     }
