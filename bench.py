@@ -324,10 +324,11 @@ def measured_traffic(workload, kernel, dtype):
     return rec["bytes"] if rec else None
 
 
-def sweep_figures(points_plain=16, replicas=128, mlp_points=1 << 20):
+def sweep_figures(points_plain=32, replicas=128, batches=6, mlp_points=1 << 20):
     """Sweep points/s on the fly20k topology (100 ms warm-up + 200 ms window
     per point): plain per-instance GPU simulation vs replicated launches of
-    `replicas` points, and the surrogate's forward over 2^20 points resident
+    `replicas` points (`batches` of them timed, after one untimed batch that
+    tunes the schedule and captures the graphs), and the surrogate's forward over 2^20 points resident
     in device memory (one BF16 matrix-core kernel for all four layers; GPU
     time from HIP events) with its MFMA roofline."""
     from flysim_amd._capi import DeviceBuffer
@@ -342,7 +343,7 @@ def sweep_figures(points_plain=16, replicas=128, mlp_points=1 << 20):
     batched = make_sim(replicas)
     batched.run(space.sample(replicas, rng), seed=3)
     batched.points, batched.wall_seconds = 0, 0.0
-    batched.run(space.sample(2 * replicas, rng), seed=4)
+    batched.run(space.sample(batches * replicas, rng), seed=4)
     sizes = [space.dim, 256, 256, 256, space.n_mod]
     mlp = Mlp(sizes, seed=0)
     x = np.ascontiguousarray(space.sample(mlp_points, rng), dtype=np.float32)
