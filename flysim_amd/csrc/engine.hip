@@ -93,7 +93,7 @@ struct Engine final : EngineBase {
       b_ens, b_stp, b_irep, b_rp_g, b_col_g, b_w_g, b_igap, b_lp, b_cntlp, b_lpre, b_lpost, b_lrec,
       b_lw, b_lw0, b_gl, b_rp_p, b_meta_p, b_w_p, b_rp_n, b_meta_n, b_w_n, b_ext, b_rec_ids, b_rec_a,
       b_total, b_rec_k, b_cum, b_overflow, b_t0, b_popcnt, b_counters, b_tmp, b_nmda_out, b_lcnt, b_lbase, b_suboff, b_subcap,
-      b_pbr, b_gain, b_skey, b_tmod, b_zig, b_wblk, b_wgat;
+      b_pbr, b_gain, b_skey, b_tmod, b_zig, b_wblk, b_wgat, b_wpop;
   std::vector<uint32_t> hnpop;   // population of each local neuron
   std::vector<uint64_t> hwblk;   // windowed path: first | count << 32 of each block
   std::vector<uint64_t> hrp_n;   // gather CSR row starts of one copy (block balancing)
@@ -689,7 +689,10 @@ struct Engine final : EngineBase {
     net.N = N;
     // the windowed path writes up to win_max steps ahead of the slots other
     // blocks still read (the fused step needs DN >= max NMDA delay + 2)
-    net.D = maxd_p + 1 + win_max;
+    // (a power of two of slots, like the gate history: slot arithmetic by mask)
+    int32_t d = 1;
+    while (d < maxd_p + 1 + win_max) d <<= 1;
+    net.D = d;
     // the gate history: a power of two of slots per neuron (slot arithmetic by mask)
     int32_t dn = 1;
     while (dn < maxd_n + 1 + win_max) dn <<= 1;
@@ -980,6 +983,18 @@ struct Engine final : EngineBase {
     b_wblk.alloc(hwblk.size() * 8);
     HIPCHECK(hipMemcpy(b_wblk.p, hwblk.data(), hwblk.size() * 8, hipMemcpyHostToDevice));
     net.wblk = b_wblk.as<uint64_t>();
+    // each block's population, or 0xFFFFFFFF where its neurons span several
+    std::vector<uint32_t> bp(hwblk.size());
+    for (size_t b = 0; b < hwblk.size(); ++b) {
+      const int64_t a = (int64_t)(hwblk[b] & 0xFFFFFFFFu), c = (int64_t)(hwblk[b] >> 32);
+      uint32_t p = hnpop[a / B];
+      for (int64_t g = a + 1; g < a + c && p != 0xFFFFFFFFu; ++g)
+        if (hnpop[g / B] != p) p = 0xFFFFFFFFu;
+      bp[b] = p;
+    }
+    b_wpop.alloc(bp.size() * 4);
+    HIPCHECK(hipMemcpy(b_wpop.p, bp.data(), bp.size() * 4, hipMemcpyHostToDevice));
+    net.wpop = b_wpop.as<uint32_t>();
     net.wgat = nullptr;
     if (B == 1 && hrp_n.size() == (size_t)N0 + 1) {
       std::vector<uint64_t> ge(2 * hwblk.size());

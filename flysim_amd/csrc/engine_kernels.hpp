@@ -169,6 +169,7 @@ struct Net {
                             // first | count << 32, at most blockDim / gather_lanes of them
   const uint64_t *wgat;     // windowed path, one copy: [2 * b], [2 * b + 1] = block b's range
                             // of gather CSR entries
+  const uint32_t *wpop;     // windowed path: population of block b's neurons (0xFFFFFFFF: several)
   int32_t gather_chunk;     // windowed path, one copy: cap on the entries per staged chunk (0: none)
   int32_t gather_split;     // windowed path, one copy: sub-lanes per (row, step) pair of the gather
   int32_t inj_on;           // some population has Gaussian injection on (windowed path's LDS)
@@ -303,8 +304,10 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // (t0 % M + k) mod M for a step offset k >= -M of the batch starting at t0
+// M (the ring depth D or the gate history depth DN) is a power of two, so
+// the remainder is a mask (any k: two's complement wraps the same way).
 __device__ __forceinline__ int32_t slot_of(int32_t t0mod, int32_t k, int32_t M) {
-  return (int32_t)((uint32_t)(t0mod + k + M) % (uint32_t)M);
+  return (int32_t)((uint32_t)(t0mod + k) & (uint32_t)(M - 1));
 }
 
 // Position of the NMDA gate of local neuron nl, replica rep, history slot
@@ -1187,6 +1190,7 @@ __device__ __forceinline__ void gather_block(const Net<T> &net, int32_t nb, int 
   // L sub-lanes per (row, step) pair: item f = pair * L + sub-lane, thread
   // tid holds items tid, tid + bd, ... (at most W of them, L <= bd / npb)
   const uint32_t L = (uint32_t)net.gather_split;
+  const uint32_t lgL = (uint32_t)__builtin_ctz(L);  // L is a power of two
   const uint32_t q = tid & (L - 1u);
   const uint32_t n_items = (uint32_t)npb * (uint32_t)W * L;
   const uint32_t nj = (n_items + bd - 1u) / bd;
@@ -1229,7 +1233,7 @@ __device__ __forceinline__ void gather_block(const Net<T> &net, int32_t nb, int 
     for (int i = 0; i < FLAT_WMAX; ++i) {
       if ((uint32_t)i >= nj) break;
       const uint32_t f = tid + (uint32_t)i * bd;
-      const uint32_t pr = f / L;  // (row, step) pair
+      const uint32_t pr = f >> lgL;  // (row, step) pair
       const uint32_t r = divw(pr), s = pr - r * (uint32_t)W;
       if (f >= n_items || r >= (uint32_t)nbc) continue;
       const uint64_t a = s_rpn[r], b = s_rpn[r + 1];
@@ -1261,7 +1265,7 @@ __device__ __forceinline__ void gather_block(const Net<T> &net, int32_t nb, int 
     T v = acc[i] * gain;
     for (uint32_t off = L >> 1; off > 0; off >>= 1) v += __shfl_xor(v, (int)off, (int)L);
     const uint32_t f = tid + (uint32_t)i * bd;
-    const uint32_t pr = f / L;
+    const uint32_t pr = f >> lgL;
     const uint32_t r = divw(pr), s = pr - r * (uint32_t)W;
     if (q == 0 && f < n_items && r < (uint32_t)nbc) s_gn[s * npb + r] = v;
   }
@@ -1377,8 +1381,9 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   // update lanes: npb consecutive threads, one neuron each, starting at a
   // wave that rotates with the block index, so the update waves of blocks
   // sharing a compute unit do not all land on one SIMD
-  const int ubase = (int)(blockIdx.x % (unsigned)wpb) * 64;
-  const int ul = (tid - ubase + (int)blockDim.x) % (int)blockDim.x;  // update-lane index
+  const int ubase = (wpb & (wpb - 1)) == 0 ? (int)(blockIdx.x & (unsigned)(wpb - 1)) * 64
+                                           : (int)(blockIdx.x % (unsigned)wpb) * 64;
+  const int ul = tid - ubase + (tid < ubase ? (int)blockDim.x : 0);  // update-lane index
   const int uw = ul >> 6;                                            // update wave ordinal
   const bool act = ul < nbc;
   const int32_t n = act ? nb + ul : N;
@@ -1391,13 +1396,32 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   // For the same reason the push rows reach LDS only after the gather.
   const bool uwave = DUP || uw * 64 < npb;  // wave-uniform
   const int32_t nld = act ? n : nb;
-  int32_t nl, rep;
-  uint32_t p0;
+  // (one network copy, B = 1, needs no division)
+  const int32_t nl = B == 1 ? nld : nld / B, rep = nld - nl * B;
+  // The populations of the update and ring/draw neurons come first, while
+  // nothing else is in flight: the block table's when its neurons share one
+  // (the windowed blocks are runs of one population: no lookup at all),
+  // else both lookups together, so the parameter loads below wait for one
+  // round trip at most.
+  const bool npb2 = (npb & (npb - 1)) == 0;  // (power-of-two blocks need no division)
+  const int ir = npb2 ? tid & (npb - 1) : tid % npb;
+  const int s0 = npb2 ? tid >> __builtin_ctz((unsigned)npb) : tid / npb;
+  const bool rl = ir < nbc;  // this thread's neuron of the ring / draw work is in the block
+  const int32_t nr = nb + (rl ? ir : 0);  // (loads by whole waves, as below)
+  const int32_t nlr = B == 1 ? nr : nr / B, repr = nr - nlr * B;
+  const uint32_t bpop = ldc(net.wpop + blockIdx.x);
+  uint32_t p0, pr;
+  if (bpop != 0xFFFFFFFFu) {
+    p0 = bpop;
+    pr = bpop;
+  } else {
+    p0 = ldc(net.npop + nl);
+    pr = ldc(net.npop + nlr);
+  }
   NState<T> S;
   NConst<T> C;
   uint64_t prow0, prow1;
   if (uwave) {
-    locate<false>(net.npop, B, nld, nl, rep, p0);
     load_const(net, *(PopC<T> *)(net.pops + p0), nl, rep, p0, C);
     load_state(net, nl, rep, nld, slot_of(tm1, k0 - 1, net.DN), S);
     prow0 = ldc(net.rp_p + nl);
@@ -1409,15 +1433,11 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   // s0 + L, ...) for the ring slots and the draws (blockDim = npb * L): their
   // loads are issued first and consumed after the NMDA gather, whose latency
   // they share.
-  const int ir = tid % npb, s0 = tid / npb;
-  const bool rl = ir < nbc;  // this thread's neuron of the ring / draw work is in the block
-  const int32_t nr = nb + (rl ? ir : 0);  // (loads by whole waves, as above)
   // the wave's first thread has a step in the window (wave-uniform)
   const bool rwave = DUP || (int)(((unsigned)tid & ~63u) / (unsigned)npb) < W;
   constexpr int RQ = 3;      // ring slots per receptor held in registers across the gather
   T rv[4][RQ];
   NConst<T> Cr;
-  uint32_t pr;
   if (rwave) {
 #pragma unroll
     for (int li = 0; li < 4; ++li) {
@@ -1430,8 +1450,6 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
                         : T(0);
       }
     }
-    int32_t nlr, repr;
-    locate<false>(net.npop, B, nr, nlr, repr, pr);
     load_const(net, *(PopC<T> *)(net.pops + pr), nlr, repr, pr, Cr);
   }  // (a wave with no step of the window stages no ring slot or draw)
   if (net.has_nmda_syn) {
@@ -1563,7 +1581,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
     const uint32_t pk = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_spk[ent]);
     const uint32_t j = pk & 0x00FFFFFFu, s = pk >> 24;
     const T amp = s_amp[ent];
-    const uint32_t jl = j / (uint32_t)B, jr = j - jl * (uint32_t)B;
+    const uint32_t jl = B == 1 ? j : j / (uint32_t)B, jr = j - jl * (uint32_t)B;
     const uint64_t r0 = s_prow[j - (uint32_t)nb];
     const uint64_t r1 = r0 + s_pdeg[j - (uint32_t)nb];
     const T ga = amp * T(net.gainr[0 * B + jr]), gg = amp * T(net.gainr[1 * B + jr]),
