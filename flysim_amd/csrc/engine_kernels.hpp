@@ -1163,6 +1163,17 @@ __device__ __forceinline__ void dma_barrier() {
 // partial sum per step of the window in registers.
 constexpr int FLAT_WMAX = 16;
 
+// x / W for a window length W <= 64 as __umulhi(x, kDivW.m[W]), exact for
+// x < 2^26 (m = floor(2^32 / W) + 1; W = 1 needs x itself): the kernels'
+// divisions by W become one table read.
+struct DivWTable {
+  uint32_t m[65];
+  constexpr DivWTable() : m() {
+    for (int w = 2; w <= 64; ++w) m[w] = 0xFFFFFFFFu / (uint32_t)w + 1u;
+  }
+};
+__device__ constexpr DivWTable kDivW{};
+
 // NMDA sums of the block's rows nb .. nb+nbc-1 (one network copy) for the W
 // steps of a window, into s_gn[s * npb + r].  The block's entries are
 // consecutive in the gather CSR; they are taken in chunks of CH: (1) their
@@ -1193,10 +1204,11 @@ __device__ __forceinline__ void gather_block(const Net<T> &net, int32_t nb, int 
   const uint32_t lgL = (uint32_t)__builtin_ctz(L);  // L is a power of two
   const uint32_t q = tid & (L - 1u);
   const uint32_t n_items = (uint32_t)npb * (uint32_t)W * L;
-  const uint32_t nj = (n_items + bd - 1u) / bd;
+  const uint32_t nj = (bd & (bd - 1u)) == 0 ? (n_items + bd - 1u) >> __builtin_ctz(bd)
+                                            : (n_items + bd - 1u) / bd;
   const uint32_t DN = (uint32_t)net.DN, dmask = DN - 1u;
   // x / W as umulhi(x, mw), exact for x < 2^26 (W == 1: x itself)
-  const uint32_t mw = 0xFFFFFFFFu / (uint32_t)W + 1u;
+  const uint32_t mw = kDivW.m[W];
   auto divw = [&](uint32_t x) -> uint32_t { return W == 1 ? x : __umulhi(x, mw); };
   T acc[FLAT_WMAX];
 #pragma unroll
@@ -1343,7 +1355,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   }
   extern __shared__ __align__(16) unsigned char wsm[];
   const int L = net.gather_lanes;
-  const int npb = (int)blockDim.x / L;  // neurons a block can hold
+  const int npb = (int)(blockDim.x >> __builtin_ctz((unsigned)L));  // neurons a block can hold (L: a power of two)
   const uint64_t blk = net.wblk[blockIdx.x];
   const int32_t nb = (int32_t)(uint32_t)blk;         // first neuron of the block
   const int nbc = (int)(uint32_t)(blk >> 32);        // its neurons (<= npb)
@@ -1368,7 +1380,10 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   T *s_amp = s_hs + (size_t)W * npb;                          // [nst] staged push amplitudes
   uint32_t *s_spk = (uint32_t *)(s_amp + nst);                // [nst] step << 24 | neuron
   // phase A, gather_block's staging
-  const int CH = window_chunk<T>(W, net.gather_chunk);
+  // window_chunk<T>(W, gather_chunk) with its division by W from the table
+  int CH = W == 1 ? (int)(32768 / sizeof(T)) : (int)__umulhi((uint32_t)(32768 / sizeof(T)), kDivW.m[W]);
+  CH = CH < 1024 ? CH : 1024;
+  if (net.gather_chunk > 0 && net.gather_chunk < CH) CH = net.gather_chunk;
   uint64_t *s_rpn = (uint64_t *)s_u;                          // [npb + 1] gather row starts
   uint32_t *s_x = (uint32_t *)(s_rpn + npb + 1);              // [CH][W] staged gates (words)
   uint32_t *s_m = s_x + (size_t)CH * W * (sizeof(T) / 4);     // [CH] staged columns | delays
