@@ -1411,8 +1411,10 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   // For the same reason the push rows reach LDS only after the gather.
   const bool uwave = DUP || uw * 64 < npb;  // wave-uniform
   const int32_t nld = act ? n : nb;
-  // (one network copy, B = 1, needs no division)
-  const int32_t nl = B == 1 ? nld : nld / B, rep = nld - nl * B;
+  // (one network copy, or a power-of-two number of copies, needs no division)
+  const bool b2 = (B & (B - 1)) == 0;
+  const int lgB = __builtin_ctz((unsigned)B);
+  const int32_t nl = b2 ? nld >> lgB : nld / B, rep = nld - nl * B;
   // The populations of the update and ring/draw neurons come first, while
   // nothing else is in flight: the block table's when its neurons share one
   // (the windowed blocks are runs of one population: no lookup at all),
@@ -1423,7 +1425,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   const int s0 = npb2 ? tid >> __builtin_ctz((unsigned)npb) : tid / npb;
   const bool rl = ir < nbc;  // this thread's neuron of the ring / draw work is in the block
   const int32_t nr = nb + (rl ? ir : 0);  // (loads by whole waves, as below)
-  const int32_t nlr = B == 1 ? nr : nr / B, repr = nr - nlr * B;
+  const int32_t nlr = b2 ? nr >> lgB : nr / B, repr = nr - nlr * B;
   const uint32_t bpop = ldc(net.wpop + blockIdx.x);
   uint32_t p0, pr;
   if (bpop != 0xFFFFFFFFu) {
@@ -1596,7 +1598,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
     const uint32_t pk = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_spk[ent]);
     const uint32_t j = pk & 0x00FFFFFFu, s = pk >> 24;
     const T amp = s_amp[ent];
-    const uint32_t jl = B == 1 ? j : j / (uint32_t)B, jr = j - jl * (uint32_t)B;
+    const uint32_t jl = b2 ? j >> lgB : j / (uint32_t)B, jr = j - jl * (uint32_t)B;
     const uint64_t r0 = s_prow[j - (uint32_t)nb];
     const uint64_t r1 = r0 + s_pdeg[j - (uint32_t)nb];
     const T ga = amp * T(net.gainr[0 * B + jr]), gg = amp * T(net.gainr[1 * B + jr]),

@@ -109,3 +109,21 @@ def test_window_fp64_replay_reproduces_reference_raster(golden, tmp_path):
     r = eng.raster(steps)
     ref = golden["runs"]["c1_s1_raster"].astype(np.int64)
     np.testing.assert_array_equal(r, ref[ref[:, 0] < steps])
+
+
+def test_window_blocks_of_several_populations():
+    """600 populations of 1-22 neurons: the windowed blocks fall back to plain
+    pieces of 32 neurons spanning several populations (600 population runs
+    against 94 plain pieces), so each lane looks its population up instead
+    of taking the block's from the block table.  FP64 results identical to
+    the fused one-launch-per-step path."""
+    net = W.c2_network(seed=6, n=3000, n_mod=300, mu=2.0, sigma=0.4, p_in=0.3, p_out=0.02,
+                       mod_edges=900, t_end_ms=100)
+    arr = arrays_from_network(net)
+    print("case window, mixed-population blocks: %d populations, %d neurons" % (len(net.pops), net.n),
+          flush=True)
+    ref = _run(arr, dict(fused=1, window=0, steps_per_graph=100), 1500)
+    got = _run(arr, dict(window=64, steps_per_graph=110), 1500)
+    assert ref["sched"]["fused"] == 1 and got["sched"]["window"] == 11
+    assert len(ref["raster"]) > 200
+    _same(got, ref, "mixed-population blocks")
