@@ -1193,10 +1193,12 @@ __device__ constexpr DivWTable kDivW{};
 // long rows being split L ways (short rows keep L = 1).  FP64
 // gates travel as their low and high words (s_x: [2][CH * W] words).  hb:
 // history slot of step k0 - 1; W <= FLAT_WMAX.  All threads must call it.
-template <typename T>
+// ov() runs once, while the first chunk's gates are in flight (work that
+// needs nothing loaded after the columns' barrier, and no staging array).
+template <typename T, typename Ov>
 __device__ __forceinline__ void gather_block(const Net<T> &net, int32_t nb, int nbc, uint64_t e0, uint64_t e1,
                                              int32_t hb, int32_t W, T *s_gn, int npb, uint64_t *s_rpn,
-                                             uint32_t *s_m, float *s_w, uint32_t *s_x, int CH) {
+                                             uint32_t *s_m, float *s_w, uint32_t *s_x, int CH, Ov &&ov) {
   const uint32_t tid = threadIdx.x, bd = blockDim.x, wb = tid & ~63u;
   // L sub-lanes per (row, step) pair: item f = pair * L + sub-lane, thread
   // tid holds items tid, tid + bd, ... (at most W of them, L <= bd / npb)
@@ -1240,6 +1242,7 @@ __device__ __forceinline__ void gather_block(const Net<T> &net, int32_t nb, int 
         if constexpr (TW == 2) lds_dma4(g + 1, s_x + (size_t)CH * W + fw);
       }
     }
+    if (c0 == e0) ov();
     dma_barrier();
 #pragma unroll
     for (int i = 0; i < FLAT_WMAX; ++i) {
@@ -1270,6 +1273,7 @@ __device__ __forceinline__ void gather_block(const Net<T> &net, int32_t nb, int 
     }
     lds_barrier();
   }
+  if (e1 <= e0) ov();  // no entries: no chunk ran it
   const T gain = T(net.gainr[2]);
 #pragma unroll
   for (int i = 0; i < FLAT_WMAX; ++i) {
@@ -1469,38 +1473,51 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
     }
     load_const(net, *(PopC<T> *)(net.pops + pr), nlr, repr, pr, Cr);
   }  // (a wave with no step of the window stages no ring slot or draw)
-  if (net.has_nmda_syn) {
-    if (flat)
-      gather_block<T>(net, nb, nbc, ldc(net.wgat + 2 * blockIdx.x), ldc(net.wgat + 2 * blockIdx.x + 1),
-                      slot_of(tm1, k0 - 1, net.DN), W, s_gn, npb, s_rpn, s_m, s_w, s_x, CH);
-    else
+  // Phase A's LDS side: the push rows, the ring slots and the draws (the
+  // draws' population parameters through the pointer popr).  For a block
+  // of one population, popr is wave-uniform (scalar loads, nothing that
+  // waits on the vector memory counter), so with short rows (DUP: few lanes
+  // per row, where the staging is a sizeable part of the launch) it runs
+  // while the gather's gates are in flight; otherwise after the gather.
+  auto stage = [&](uint32_t popr) {
+    if (act) {
+      s_prow[ul] = prow0;
+      s_pdeg[ul] = (uint32_t)(prow1 - prow0);
+    }
+    if (rl) {
+      int c = 0;
+#pragma unroll
+      for (int li = 0; li < 4; ++li) {
+        if (!(net.lin_mask & (1 << li))) continue;
+        int j = 0;
+        for (int32_t s = s0; s < W; s += L, ++j) {
+          const T v = j < RQ ? (j == 0 ? rv[li][0] : j == 1 ? rv[li][1] : rv[li][2])
+                             : net.ring[((size_t)li * net.D + slot_of(tm0, k0 + s - 1, net.D)) * (size_t)N + nr];
+          s_inc[((size_t)c * W + s) * npb + ir] = v;
+        }
+        ++c;
+      }
+      for (int32_t s = s0; s < W; s += L) {
+        uint32_t db;
+        float iv;
+        neuron_draws<T, REPLAY>(net, *(PopC<T> *)(net.pops + popr), Cr, nr, tb + k0 + s, db, iv);
+        s_draw[s * npb + ir] = (uint8_t)db;
+        if (inj) s_inj[s * npb + ir] = iv;
+      }
+    }
+  };
+  const bool upop = DUP && bpop != 0xFFFFFFFFu;  // block-uniform
+  if (net.has_nmda_syn && flat) {
+    gather_block<T>(net, nb, nbc, ldc(net.wgat + 2 * blockIdx.x), ldc(net.wgat + 2 * blockIdx.x + 1),
+                    slot_of(tm1, k0 - 1, net.DN), W, s_gn, npb, s_rpn, s_m, s_w, s_x, CH, [&]() {
+                      if (upop) stage((uint32_t)__builtin_amdgcn_readfirstlane((int)bpop));
+                    });
+    if (!upop) stage(pr);
+  } else {
+    if (net.has_nmda_syn)
       gather_window<T, UNI>(net, tid / L < nbc ? (int64_t)nb + tid / L : (int64_t)N,
                             slot_of(tm1, k0 - 1, net.DN), W, L, tid % L, s_gn, npb, tid / L);
-  }
-  if (act) {
-    s_prow[ul] = prow0;
-    s_pdeg[ul] = (uint32_t)(prow1 - prow0);
-  }
-  if (rl) {
-    int c = 0;
-#pragma unroll
-    for (int li = 0; li < 4; ++li) {
-      if (!(net.lin_mask & (1 << li))) continue;
-      int j = 0;
-      for (int32_t s = s0; s < W; s += L, ++j) {
-        const T v = j < RQ ? (j == 0 ? rv[li][0] : j == 1 ? rv[li][1] : rv[li][2])
-                           : net.ring[((size_t)li * net.D + slot_of(tm0, k0 + s - 1, net.D)) * (size_t)N + nr];
-        s_inc[((size_t)c * W + s) * npb + ir] = v;
-      }
-      ++c;
-    }
-    for (int32_t s = s0; s < W; s += L) {
-      uint32_t db;
-      float iv;
-      neuron_draws<T, REPLAY>(net, *(PopC<T> *)(net.pops + pr), Cr, nr, tb + k0 + s, db, iv);
-      s_draw[s * npb + ir] = (uint8_t)db;
-      if (inj) s_inj[s * npb + ir] = iv;
-    }
+    stage(pr);
   }
   lds_barrier();
 
