@@ -440,6 +440,44 @@ __device__ __forceinline__ void load_const(const Net<T> &net, PopC<T> &P, int32_
   C.rep = (uint32_t)rep;
 }
 
+// load_const in two halves for the windowed kernel: the loads alone (no
+// arithmetic on what they return, so issuing them never waits), then the
+// derived fields, once the values have landed.
+struct NConstRaw {
+  float pb[5];
+  float eg;
+  uint64_t skey;
+  uint32_t key_lo, key_hi;
+  int32_t first;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_const_raw(const Net<T> &net, PopC<T> &P, int32_t rep, uint32_t p0,
+                                               NConstRaw &R) {
+  const float *pbp = net.pbr + (size_t)p0 * net.B + rep;
+  const size_t pbs = (size_t)net.P0 * net.B;  // receptor stride of pbr
+#pragma unroll
+  for (int r = 0; r < 5; ++r) R.pb[r] = (net.ext_mask & (1 << r)) ? pbp[r * pbs] : -1.0f;
+  R.eg = net.gainr[5 * net.B + rep];
+  R.skey = net.skey[rep];
+  R.key_lo = P.key_lo;
+  R.key_hi = P.key_hi;
+  R.first = P.first;
+}
+
+// the fields load_const derives, from load_const_raw's values
+template <typename T>
+__device__ __forceinline__ void finish_const(const Net<T> &net, const NConstRaw &R, int32_t nl, int32_t rep,
+                                             NConst<T> &C) {
+#pragma unroll
+  for (int r = 0; r < 5; ++r) C.pb[r] = R.pb[r];
+  C.eg = T(R.eg);
+  C.key = net.seed ^ R.skey ^ (((uint64_t)R.key_hi << 32) | R.key_lo);
+  C.ctr = (uint32_t)(nl - R.first);
+  C.nl = (uint32_t)nl;
+  C.rep = (uint32_t)rep;
+}
+
 // External event outcomes and the Gaussian injection draw of neuron n at step
 // t, the stochastic part of BgInit (code/EqsGnl2.cpp:284-337): bit rid of
 // dbits is set when receptor rid's external gate receives an event this step
@@ -1441,9 +1479,10 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   }
   NState<T> S;
   NConst<T> C;
+  NConstRaw Rc;  // C's loads; its derived fields follow after the gather
   uint64_t prow0, prow1;
   if (uwave) {
-    load_const(net, *(PopC<T> *)(net.pops + p0), nl, rep, p0, C);
+    load_const_raw(net, *(PopC<T> *)(net.pops + p0), rep, p0, Rc);
     load_state(net, nl, rep, nld, slot_of(tm1, k0 - 1, net.DN), S);
     prow0 = ldc(net.rp_p + nl);
     prow1 = ldc(net.rp_p + nl + 1);
@@ -1459,6 +1498,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   constexpr int RQ = 3;      // ring slots per receptor held in registers across the gather
   T rv[4][RQ];
   NConst<T> Cr;
+  NConstRaw Rr;  // Cr's loads; its derived fields are formed in the staging
   if (rwave) {
 #pragma unroll
     for (int li = 0; li < 4; ++li) {
@@ -1471,7 +1511,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
                         : T(0);
       }
     }
-    load_const(net, *(PopC<T> *)(net.pops + pr), nlr, repr, pr, Cr);
+    load_const_raw(net, *(PopC<T> *)(net.pops + pr), repr, pr, Rr);
   }  // (a wave with no step of the window stages no ring slot or draw)
   // Phase A's LDS side: the push rows, the ring slots and the draws (the
   // draws' population parameters through the pointer popr).  For a block
@@ -1479,7 +1519,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
   // waits on the vector memory counter), so with short rows (DUP: few lanes
   // per row, where the staging is a sizeable part of the launch) it runs
   // while the gather's gates are in flight; otherwise after the gather.
-  auto stage = [&](uint32_t popr) {
+  auto stage = [&](uint32_t popr) __attribute__((always_inline)) {
     if (act) {
       s_prow[ul] = prow0;
       s_pdeg[ul] = (uint32_t)(prow1 - prow0);
@@ -1497,6 +1537,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
         }
         ++c;
       }
+      finish_const(net, Rr, nlr, repr, Cr);
       for (int32_t s = s0; s < W; s += L) {
         uint32_t db;
         float iv;
@@ -1519,6 +1560,7 @@ __global__ void __launch_bounds__(512) k_window(Net<T> net_, int32_t k0, int32_t
                             slot_of(tm1, k0 - 1, net.DN), W, L, tid % L, s_gn, npb, tid / L);
     stage(pr);
   }
+  if (uwave) finish_const(net, Rc, nl, rep, C);
   lds_barrier();
 
   // ---- B: the W updates of the block's neurons, spikes staged per wave;
